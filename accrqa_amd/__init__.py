@@ -1,0 +1,201 @@
+"""accrqa_amd -- RR / DET / LAM recurrence quantification on AMD MI355X (gfx950).
+
+Thin ctypes layer over the C ABI of ``libaccrqa_amd.so`` (include/accrqa_amd.h),
+which is the drop-in GPU platform for AccRQA's ``accrqa_RR`` / ``accrqa_DET`` /
+``accrqa_LAM`` (src/AccRQA_functions.cpp:525-555, 780-810, 945-975).  The same
+shared library also exports the reference's C++ entry points with the new
+platform value ``PLT_AMD_GPU = 2048`` (include/AccRQA_amd.hpp).
+
+The functions below mirror the reference's Python API (src/accrqa/accrqa.py:85,
+189, 329) minus the pandas "tidy" output: they return numpy arrays shaped like
+the reference's output cubes.  There is no CPU fallback: if the library is not
+built or no GPU is present, every call raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+__all__ = ["RR", "DET", "LAM", "histograms", "rqa_device", "AccrqaError", "library_path", "load", "PLT_AMD_GPU",
+           "DST_EUCLIDEAN", "DST_MAXIMAL"]
+
+PLT_AMD_GPU = 2048
+DST_EUCLIDEAN = 1
+DST_MAXIMAL = 2
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libaccrqa_amd.so")
+_lib = None
+
+ERRORS = {
+    0: "success",
+    1: "generic runtime error",
+    2: "wrong arguments",
+    3: "unsupported data type",
+    4: "array not allocated",
+    5: "memory copy failure host<->device",
+    6: "wrong memory location",
+    7: "GPU runtime not found",
+    8: "could not locate GPU device",
+    9: "not enough memory on the device (GPU)",
+    10: "other GPU error",
+    11: "invalid metric selected",
+}
+
+
+class AccrqaError(RuntimeError):
+    def __init__(self, code, detail=""):
+        self.code = code
+        super().__init__(f"accrqa error {code}: {ERRORS.get(code, 'unknown')}" + (f" ({detail})" if detail else ""))
+
+
+def library_path():
+    return _LIB_PATH
+
+
+def load():
+    """Load libaccrqa_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        raise AccrqaError(7, f"{_LIB_PATH} is not built; run accrqa_amd.build.build()")
+    L = C.CDLL(_LIB_PATH)
+    P, I, SZ = C.c_void_p, C.c_int, C.c_size_t
+    for sfx in ("f32", "f64"):
+        f = getattr(L, f"accrqa_amd_RR_{sfx}")
+        f.argtypes = [P, P, SZ, P, I, P, I, P, I, I, P]
+        f.restype = None
+        for kind in ("DET", "LAM"):
+            f = getattr(L, f"accrqa_amd_{kind}_{sfx}")
+            f.argtypes = [P, P, SZ, P, I, P, I, P, I, P, I, I, I, P]
+            f.restype = None
+        f = getattr(L, f"accrqa_amd_histograms_{sfx}")
+        f.argtypes = [P, SZ, I, I, P, I, I, I, P, P, P, P]
+        f.restype = None
+        f = getattr(L, f"accrqa_amd_rqa_device_{sfx}")
+        f.argtypes = [P, SZ, I, I, P, I, P, I, I, P, P, P, P]
+        f.restype = I
+    L.accrqa_amd_last_error.restype = C.c_char_p
+    L.accrqa_amd_version.restype = C.c_char_p
+    _lib = L
+    return L
+
+
+def _dist(d):
+    if isinstance(d, str):
+        d = {"euclidean": DST_EUCLIDEAN, "maximal": DST_MAXIMAL}.get(d.lower(), 0)
+    return int(d)
+
+
+def _sfx(dt):
+    if dt == np.float64:
+        return "f64"
+    if dt == np.float32:
+        return "f32"
+    raise AccrqaError(3, f"input dtype {dt} (float32 or float64 expected)")
+
+
+def _ints(v):
+    return np.ascontiguousarray(np.asarray(v, dtype=np.int32).reshape(-1))
+
+
+def _raise(err):
+    if err.value != 0:
+        L = load()
+        raise AccrqaError(err.value, L.accrqa_amd_last_error().decode())
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def RR(input_data, tau_values, emb_values, threshold_values, distance_type="euclidean"):
+    """Recurrence rate, shape [nTaus, nEmbs, nThresholds] (input dtype)."""
+    L = load()
+    x = np.ascontiguousarray(input_data)
+    sfx = _sfx(x.dtype)
+    tau, emb = _ints(tau_values), _ints(emb_values)
+    thr = np.ascontiguousarray(np.asarray(threshold_values, dtype=x.dtype).reshape(-1))
+    out = np.zeros((tau.size, emb.size, thr.size), dtype=x.dtype)
+    err = C.c_int(-1)
+    getattr(L, f"accrqa_amd_RR_{sfx}")(_p(out), _p(x), x.size, _p(tau), tau.size, _p(emb), emb.size, _p(thr),
+                                       thr.size, _dist(distance_type), C.byref(err))
+    _raise(err)
+    return out
+
+
+def _dl(kind, input_data, tau_values, emb_values, l_values, threshold_values, distance_type, calculate_ENTR):
+    L = load()
+    x = np.ascontiguousarray(input_data)
+    sfx = _sfx(x.dtype)
+    tau, emb, lm = _ints(tau_values), _ints(emb_values), _ints(l_values)
+    thr = np.ascontiguousarray(np.asarray(threshold_values, dtype=x.dtype).reshape(-1))
+    if kind == "DET":
+        shape = (tau.size, emb.size, thr.size, lm.size, 5)
+    else:
+        shape = (tau.size, emb.size, lm.size, thr.size, 5)
+    out = np.zeros(shape, dtype=x.dtype)
+    err = C.c_int(-1)
+    getattr(L, f"accrqa_amd_{kind}_{sfx}")(_p(out), _p(x), x.size, _p(tau), tau.size, _p(emb), emb.size, _p(lm),
+                                           lm.size, _p(thr), thr.size, _dist(distance_type), int(bool(calculate_ENTR)),
+                                           C.byref(err))
+    _raise(err)
+    return out
+
+
+def DET(input_data, tau_values, emb_values, lmin_values, threshold_values, distance_type="euclidean",
+        calculate_ENTR=True):
+    """DET, L, Lmax, ENTR, RR; shape [nTaus, nEmbs, nThresholds, nLmins, 5] (the reference CPU layout)."""
+    return _dl("DET", input_data, tau_values, emb_values, lmin_values, threshold_values, distance_type,
+               calculate_ENTR)
+
+
+def LAM(input_data, tau_values, emb_values, vmin_values, threshold_values, distance_type="euclidean",
+        calculate_ENTR=True):
+    """LAM, TT, TTmax, ENTR, RR; shape [nTaus, nEmbs, nVmins, nThresholds, 5]."""
+    return _dl("LAM", input_data, tau_values, emb_values, vmin_values, threshold_values, distance_type,
+               calculate_ENTR)
+
+
+def histograms(input_data, tau, emb, threshold_values, distance_type="euclidean", which=7):
+    """Raw integer results for one (tau, emb): dict with 'rr' [T] (full-matrix
+    recurrence counts), 'det' [T, N_c+1] (h_D) and 'lam' [T, N_c+1] (h_V)."""
+    L = load()
+    x = np.ascontiguousarray(input_data)
+    sfx = _sfx(x.dtype)
+    thr = np.ascontiguousarray(np.asarray(threshold_values, dtype=x.dtype).reshape(-1))
+    nc = x.size - (int(emb) - 1) * int(tau)
+    hs = max(nc + 1, 1)
+    rr = np.zeros(thr.size, dtype=np.uint64)
+    hd = np.zeros((thr.size, hs), dtype=np.uint64)
+    hv = np.zeros((thr.size, hs), dtype=np.uint64)
+    err = C.c_int(-1)
+    getattr(L, f"accrqa_amd_histograms_{sfx}")(_p(x), x.size, int(tau), int(emb), _p(thr), thr.size,
+                                               _dist(distance_type), int(which), _p(rr), _p(hd), _p(hv), C.byref(err))
+    _raise(err)
+    out = {}
+    if which & 1:
+        out["rr"] = rr
+    if which & 2:
+        out["det"] = hd
+    if which & 4:
+        out["lam"] = hv
+    return out
+
+
+def rqa_device(d_input, n, tau, emb, threshold_values, lmin_values, distance_type, d_rr=0, d_det=0, d_lam=0,
+               stream=0, dtype=np.float32):
+    """Device-resident RR+DET+LAM for one (tau, emb).  d_* are device pointers
+    (ints, e.g. torch tensor .data_ptr()); outputs use the layouts documented in
+    include/accrqa_amd.h.  Raises on error."""
+    L = load()
+    sfx = _sfx(np.dtype(dtype))
+    thr = np.ascontiguousarray(np.asarray(threshold_values, dtype=dtype).reshape(-1))
+    lm = _ints(lmin_values)
+    e = getattr(L, f"accrqa_amd_rqa_device_{sfx}")(C.c_void_p(d_input), n, int(tau), int(emb), _p(thr), thr.size,
+                                                   _p(lm), lm.size, _dist(distance_type), C.c_void_p(d_rr or None),
+                                                   C.c_void_p(d_det or None), C.c_void_p(d_lam or None),
+                                                   C.c_void_p(stream or None))
+    if e != 0:
+        raise AccrqaError(e, L.accrqa_amd_last_error().decode())

@@ -1,0 +1,717 @@
+// rqa_engine.hip -- host side of the gfx950 RR / DET / LAM engine and its C ABI
+// (include/accrqa_amd.h).
+//
+// One call handles one (tau, emb) block for all thresholds: thresholds are
+// turned into exact comparison bounds, sorted, split into groups of 4/8/16/32
+// (one pass over the pairs per group), and each group runs
+//   k_rr   -> k_rr_out                      (RR)
+//   k_lam  -> k_stitch -> k_finalize        (LAM, TT, TTmax, ENTR, RR)
+//   k_det  -> k_stitch -> k_finalize        (DET, L, Lmax, ENTR, RR)
+// on one HIP stream.  Device buffers live in a per-process context that grows
+// on demand and is reused across calls (the reference allocates and frees per
+// call, src/AccRQA-GPU-HST.cu:1855-1945).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "accrqa_amd.h"
+#include "rqa_device.hpp"
+
+namespace {
+
+enum : int {
+    E_OK = 0,
+    E_INVALID = 2,
+    E_NULL = 4,
+    E_NO_DEVICE = 8,
+    E_NO_MEMORY = 9,
+    E_DEVICE = 10,
+};
+
+std::string g_last_error;
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+};
+
+struct Ctx {
+    int device = -1;
+    int ncu = 0;
+    hipStream_t stream = nullptr;
+    DevBuf x, hist, carry, metric, scan, items_rr, items_det, counts, perm, lmins, out_rr, out_det, out_lam;
+    int64_t items_nc = -1;     // (nc, seg_len) the item tables were built for
+    int items_L = -1;
+    int64_t n_items_rr = 0, n_items_det = 0;
+};
+
+std::mutex g_mu;
+Ctx g_ctx;
+
+int map_err(hipError_t e) {
+    if (e == hipSuccess) return E_OK;
+    g_last_error = hipGetErrorString(e);
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return E_NO_MEMORY;
+    if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) return E_NO_DEVICE;
+    return E_DEVICE;
+}
+
+#define HIP_TRY(expr)                             \
+    do {                                          \
+        hipError_t e_ = (expr);                   \
+        if (e_ != hipSuccess) return map_err(e_); \
+    } while (0)
+
+int ensure(DevBuf &b, size_t bytes) {
+    if (bytes == 0) bytes = 256;
+    if (b.cap >= bytes) return E_OK;
+    if (b.p) {
+        hipError_t e = hipFree(b.p);
+        b.p = nullptr;
+        b.cap = 0;
+        if (e != hipSuccess) return map_err(e);
+    }
+    HIP_TRY(hipMalloc(&b.p, bytes));
+    b.cap = bytes;
+    return E_OK;
+}
+
+int init_ctx() {
+    int dev = -1;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+        g_last_error = "no HIP device";
+        return E_NO_DEVICE;
+    }
+    HIP_TRY(hipGetDevice(&dev));
+    if (g_ctx.device != dev) {
+        // A different current device: start a fresh context on it.  Buffers of
+        // the old device are released on that device.
+        if (g_ctx.device >= 0) {
+            int cur = dev;
+            (void)hipSetDevice(g_ctx.device);
+            DevBuf *all[] = {&g_ctx.x, &g_ctx.hist, &g_ctx.carry, &g_ctx.metric, &g_ctx.scan, &g_ctx.items_rr,
+                             &g_ctx.items_det, &g_ctx.counts, &g_ctx.perm, &g_ctx.lmins, &g_ctx.out_rr,
+                             &g_ctx.out_det, &g_ctx.out_lam};
+            g_ctx.items_nc = -1;
+            for (DevBuf *b : all) {
+                if (b->p) (void)hipFree(b->p);
+                *b = DevBuf();
+            }
+            if (g_ctx.stream) (void)hipStreamDestroy(g_ctx.stream);
+            g_ctx.stream = nullptr;
+            (void)hipSetDevice(cur);
+        }
+        hipDeviceProp_t prop;
+        HIP_TRY(hipGetDeviceProperties(&prop, dev));
+        g_ctx.ncu = prop.multiProcessorCount;
+        HIP_TRY(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
+        g_ctx.device = dev;
+    }
+    return E_OK;
+}
+
+// ---------------------------------------------------------------- thresholds
+// Euclidean: largest s >= 0 with (eps - RN(sqrt(s))) >= 0 in F, so that the
+// GPU decision [s <= S] equals the reference's (eps - sqrt(s)) >= 0
+// (AccRQA_CPU_function.cpp:22,45).  Values that never recur map to -1.
+template <typename F>
+bool euclid_ok(F eps, F s) {
+    F d = std::sqrt(s);
+    F diff = eps - d;
+    return diff >= F(0);
+}
+
+template <typename F>
+F euclid_bound(F eps) {
+    const F big = std::numeric_limits<F>::max();
+    const F inf = std::numeric_limits<F>::infinity();
+    if (std::isnan(eps) || eps < F(0)) return F(-1);
+    if (eps == inf) return big;
+    if (eps == F(0)) return F(0);   // +0 or -0: only s == 0 recurs
+    double e2 = (double)eps * (double)eps;
+    F s = e2 > (double)big ? big : (F)e2;
+    while (s < big && euclid_ok(eps, std::nextafter(s, inf))) s = std::nextafter(s, inf);
+    while (s > F(0) && !euclid_ok(eps, s)) s = std::nextafter(s, -inf);
+    return s;
+}
+
+template <typename F>
+F bound_of(F eps, int dist) {
+    if (dist == 1) return euclid_bound(eps);
+    // maximal norm: R = [(eps - d) >= 0] = [d <= eps] except eps = d = +inf
+    if (eps == std::numeric_limits<F>::infinity()) return std::numeric_limits<F>::max();
+    return eps;   // NaN compares false, negative eps never recurs
+}
+
+// Split T sorted thresholds into groups of TG in {4, 8, 16, 32}, minimising
+// (per-pair distance work + 2 ops per threshold slot) summed over groups.
+std::vector<int> plan_groups(int T, int m, int dist) {
+    const int cand[4] = {4, 8, 16, 32};
+    const double D = dist == 1 ? 3.0 * m - 1.0 : 2.0 * m;
+    std::vector<double> best(T + 1, 1e300);
+    std::vector<int> pick(T + 1, 0);
+    best[0] = 0;
+    for (int n = 1; n <= T; n++)
+        for (int g : cand) {
+            int take = std::min(g, n);
+            double c = best[n - take] + D + 2.0 * g;
+            if (c < best[n]) {
+                best[n] = c;
+                pick[n] = g;
+            }
+        }
+    std::vector<int> sizes;
+    for (int n = T; n > 0;) {
+        int g = pick[n];
+        int take = std::min(g, n);
+        sizes.push_back(g);
+        n -= take;
+    }
+    return sizes;   // TG per group; the last groups may be padded
+}
+
+// ---------------------------------------------------------------- launches
+__global__ void k_fill_u64(unsigned long long *p, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = 0ull;
+}
+
+template <typename F>
+__global__ void k_pad_copy(const F *__restrict__ src, F *__restrict__ dst, size_t n, size_t ntot) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < ntot; i += stride) dst[i] = i < n ? src[i] : F(0);
+}
+
+void fill_zero(unsigned long long *p, size_t n, hipStream_t st) {
+    if (!n) return;
+    unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_fill_u64, dim3(blocks), dim3(256), 0, st, p, n);
+}
+
+// ---------------------------------------------------------------- profiling
+// Optional per-kernel timing with HIP events recorded on the launch stream
+// (accrqa_amd_profile_*).  Off by default; when on, run_block synchronises
+// its stream and folds the event pairs into per-kernel totals.
+struct ProfEntry {
+    const char *name;
+    hipEvent_t a, b;
+};
+struct Prof {
+    bool on = false;
+    std::vector<ProfEntry> pending;
+    std::vector<hipEvent_t> pool;
+    std::vector<std::pair<std::string, std::pair<double, long long>>> acc;
+    const char *cur = nullptr;
+    hipEvent_t cur_a = nullptr;
+} g_prof;
+
+hipEvent_t prof_event() {
+    if (!g_prof.pool.empty()) {
+        hipEvent_t e = g_prof.pool.back();
+        g_prof.pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
+void prof_begin(const char *name, hipStream_t st) {
+    if (!g_prof.on) return;
+    g_prof.cur = name;
+    g_prof.cur_a = prof_event();
+    (void)hipEventRecord(g_prof.cur_a, st);
+}
+
+void prof_end(hipStream_t st) {
+    if (!g_prof.on || !g_prof.cur) return;
+    hipEvent_t b = prof_event();
+    (void)hipEventRecord(b, st);
+    g_prof.pending.push_back({g_prof.cur, g_prof.cur_a, b});
+    g_prof.cur = nullptr;
+}
+
+void prof_collect(hipStream_t st) {
+    if (!g_prof.on || g_prof.pending.empty()) return;
+    (void)hipStreamSynchronize(st);
+    for (ProfEntry &p : g_prof.pending) {
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, p.a, p.b);
+        bool found = false;
+        for (auto &kv : g_prof.acc)
+            if (kv.first == p.name) {
+                kv.second.first += ms;
+                kv.second.second += 1;
+                found = true;
+            }
+        if (!found) g_prof.acc.push_back({p.name, {(double)ms, 1}});
+        g_prof.pool.push_back(p.a);
+        g_prof.pool.push_back(p.b);
+    }
+    g_prof.pending.clear();
+}
+
+constexpr int PAD = 128;   // zero samples after N (kernel windows read up to N + 95)
+
+int seg_len_for(int64_t nc) {
+    int64_t L = (nc + 15) / 16;
+    L = std::max<int64_t>(L, 1024);
+    L = (L + rqa::BLK - 1) / rqa::BLK * rqa::BLK;
+    return (int)std::min<int64_t>(L, 1 << 30);
+}
+
+unsigned grid_for(int64_t nitems, int ncu) {
+    int64_t wgs = (nitems + rqa::WAVES - 1) / rqa::WAVES;
+    int64_t cap = (int64_t)ncu * 8;
+    return (unsigned)std::max<int64_t>(1, std::min(wgs, cap));
+}
+
+// Work-item tables (uploaded once per (N_c, segment length)):
+//   k_rr : (band b, row segment q) over rows 0 .. min(64(b+1), N_c) - 1
+//   k_det: (strip b, step segment q) over steps 0 .. N_c - (1 + 64b) - 1
+int build_items(int64_t nc, hipStream_t st) {
+    Ctx &c = g_ctx;
+    const int L = seg_len_for(nc);
+    if (c.items_nc == nc && c.items_L == L) return E_OK;
+    std::vector<int2> rr, det;
+    const int64_t nb = (nc + rqa::WAVE - 1) / rqa::WAVE;
+    for (int64_t b = 0; b < nb; b++) {
+        int64_t rows = std::min<int64_t>((b + 1) * rqa::WAVE, nc);
+        for (int64_t q = 0; q * L < rows; q++) rr.push_back(make_int2((int)b, (int)q));
+    }
+    const int64_t nstrips = (nc - 1 + rqa::WAVE - 1) / rqa::WAVE;
+    for (int64_t b = 0; b < nstrips; b++) {
+        int64_t steps = nc - (1 + b * rqa::WAVE);
+        for (int64_t q = 0; q * L < steps; q++) det.push_back(make_int2((int)b, (int)q));
+    }
+    int e;
+    if ((e = ensure(c.items_rr, rr.size() * sizeof(int2)))) return e;
+    if ((e = ensure(c.items_det, det.size() * sizeof(int2)))) return e;
+    if (!rr.empty())
+        HIP_TRY(hipMemcpyAsync(c.items_rr.p, rr.data(), rr.size() * sizeof(int2), hipMemcpyHostToDevice, st));
+    if (!det.empty())
+        HIP_TRY(hipMemcpyAsync(c.items_det.p, det.data(), det.size() * sizeof(int2), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipStreamSynchronize(st));   // the host vectors die here
+    c.n_items_rr = (int64_t)rr.size();
+    c.n_items_det = (int64_t)det.size();
+    c.items_nc = nc;
+    c.items_L = L;
+    return E_OK;
+}
+
+template <typename F, int NORM, int TG>
+int launch_group(const F *dx, int64_t nc, int tau, int m, const F *bounds, int nt, bool want_rr, bool want_det,
+                 bool want_lam, const int *d_lmins, int nl, const int *d_perm, int T, F *d_rr, F *d_det, F *d_lam,
+                 hipStream_t st) {
+    Ctx &c = g_ctx;
+    rqa::Bounds<F, TG> B;
+    for (int t = 0; t < TG; t++) B.s[t] = t < nt ? bounds[t] : F(-1);
+    const int L = seg_len_for(nc);
+    const int nseg = (int)((nc + L - 1) / L);
+    const size_t hs = (size_t)nc + 1;
+
+    int e;
+    if ((e = ensure(c.counts, 32 * sizeof(unsigned long long)))) return e;
+    unsigned long long *counts = (unsigned long long *)c.counts.p;
+
+    if (want_rr && !want_lam) {
+        fill_zero(counts, 32, st);
+        const int64_t ni = c.n_items_rr;
+        prof_begin("k_rr", st);
+        hipLaunchKernelGGL((rqa::k_rr<F, NORM, TG>), dim3(grid_for(ni, c.ncu)), dim3(rqa::WG), 0, st, dx, nc, tau, m,
+                           B, nt, L, (const int2 *)c.items_rr.p, ni, counts);
+        prof_end(st);
+        HIP_TRY(hipGetLastError());
+    }
+
+    if (want_lam || want_det) {
+        if ((e = ensure(c.hist, (size_t)nt * hs * 8))) return e;
+        if ((e = ensure(c.metric, (size_t)nt * hs * 8))) return e;
+        if ((e = ensure(c.scan, (size_t)nt * hs * 8))) return e;
+        if ((e = ensure(c.carry, (size_t)nt * (size_t)nc * nseg * sizeof(uint2)))) return e;
+    }
+    unsigned long long *hist = (unsigned long long *)c.hist.p;
+    uint2 *carry = (uint2 *)c.carry.p;
+
+    if (want_lam) {
+        fill_zero(hist, (size_t)nt * hs, st);
+        const int64_t ni = ((nc + rqa::WAVE - 1) / rqa::WAVE) * nseg;
+        prof_begin("k_lam", st);
+        hipLaunchKernelGGL((rqa::k_lam<F, NORM, TG>), dim3(grid_for(ni, c.ncu)), dim3(rqa::WG), 0, st, dx, nc, tau,
+                           m, B, nt, L, nseg, hist, carry);
+        prof_end(st);
+        HIP_TRY(hipGetLastError());
+        const int64_t nthr = (int64_t)nt * nc;
+        prof_begin("k_stitch", st);
+        hipLaunchKernelGGL(rqa::k_stitch, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, st,
+                           (const uint2 *)carry, nt, nc, nc, 0, L, nseg, hist);
+        prof_end(st);
+        HIP_TRY(hipGetLastError());
+        // The LAM histogram covers every recurrent pair of the full matrix once,
+        // so its sum of l*h[l] is the RR count as well.
+        prof_begin("k_finalize", st);
+        hipLaunchKernelGGL((rqa::k_finalize<F>), dim3(nt), dim3(rqa::FIN), 0, st, hist, nc, nt, 0, d_lmins, nl,
+                           (unsigned long long *)c.metric.p, (unsigned long long *)c.scan.p, d_lam, 0, d_perm, T,
+                           counts);
+        prof_end(st);
+        HIP_TRY(hipGetLastError());
+    }
+    if (want_rr) {
+        hipLaunchKernelGGL((rqa::k_rr_out<F>), dim3(1), dim3(64), 0, st, (const unsigned long long *)counts, nt, nc,
+                           d_perm, d_rr);
+        HIP_TRY(hipGetLastError());
+    }
+    if (want_det) {
+        fill_zero(hist, (size_t)nt * hs, st);
+        if (nc > 1) {
+            const int64_t ni = c.n_items_det;
+            prof_begin("k_det", st);
+            hipLaunchKernelGGL((rqa::k_det<F, NORM, TG>), dim3(grid_for(ni, c.ncu)), dim3(rqa::WG), 0, st, dx, nc,
+                               tau, m, B, nt, L, nseg, (const int2 *)c.items_det.p, ni, hist, carry);
+            prof_end(st);
+            HIP_TRY(hipGetLastError());
+            const int64_t nthr = (int64_t)nt * (nc - 1);
+            prof_begin("k_stitch", st);
+            hipLaunchKernelGGL(rqa::k_stitch, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, st,
+                               (const uint2 *)carry, nt, nc - 1, nc, 1, L, nseg, hist);
+            prof_end(st);
+            HIP_TRY(hipGetLastError());
+        }
+        prof_begin("k_finalize", st);
+        hipLaunchKernelGGL((rqa::k_finalize<F>), dim3(nt), dim3(rqa::FIN), 0, st, hist, nc, nt, 1, d_lmins, nl,
+                           (unsigned long long *)c.metric.p, (unsigned long long *)c.scan.p, d_det, 1, d_perm, T,
+                           (unsigned long long *)nullptr);
+        prof_end(st);
+        HIP_TRY(hipGetLastError());
+    }
+    return E_OK;
+}
+
+template <typename F, int NORM>
+int launch_tg(int TG, const F *dx, int64_t nc, int tau, int m, const F *bounds, int nt, bool rr, bool det, bool lam,
+              const int *d_lmins, int nl, const int *d_perm, int T, F *d_rr, F *d_det, F *d_lam, hipStream_t st) {
+    switch (TG) {
+        case 4:
+            return launch_group<F, NORM, 4>(dx, nc, tau, m, bounds, nt, rr, det, lam, d_lmins, nl, d_perm, T, d_rr,
+                                            d_det, d_lam, st);
+        case 8:
+            return launch_group<F, NORM, 8>(dx, nc, tau, m, bounds, nt, rr, det, lam, d_lmins, nl, d_perm, T, d_rr,
+                                            d_det, d_lam, st);
+        case 16:
+            return launch_group<F, NORM, 16>(dx, nc, tau, m, bounds, nt, rr, det, lam, d_lmins, nl, d_perm, T,
+                                             d_rr, d_det, d_lam, st);
+        default:
+            return launch_group<F, NORM, 32>(dx, nc, tau, m, bounds, nt, rr, det, lam, d_lmins, nl, d_perm, T,
+                                             d_rr, d_det, d_lam, st);
+    }
+}
+
+// One (tau, emb) block: dx is the padded device copy of the series.
+template <typename F>
+int run_block(const F *dx, size_t n, int tau, int emb, const F *thr, int T, const int *lmins, int nl, int dist,
+              F *d_rr, F *d_det, F *d_lam, hipStream_t st) {
+    Ctx &c = g_ctx;
+    const int64_t nc = (int64_t)n - (int64_t)(emb - 1) * tau;
+    // bounds, sorted ascending; perm maps sorted position -> caller's index
+    std::vector<F> bnd(T);
+    std::vector<int> order(T);
+    for (int t = 0; t < T; t++) {
+        bnd[t] = bound_of(thr[t], dist);
+        order[t] = t;
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        F x = bnd[a], y = bnd[b];
+        if (std::isnan(x)) return false;
+        if (std::isnan(y)) return true;
+        return x < y;
+    });
+    int e;
+    if ((e = ensure(c.perm, (size_t)T * sizeof(int)))) return e;
+    if ((e = ensure(c.lmins, (size_t)std::max(nl, 1) * sizeof(int)))) return e;
+    HIP_TRY(hipMemcpyAsync(c.perm.p, order.data(), (size_t)T * sizeof(int), hipMemcpyHostToDevice, st));
+    if (nl > 0) HIP_TRY(hipMemcpyAsync(c.lmins.p, lmins, (size_t)nl * sizeof(int), hipMemcpyHostToDevice, st));
+
+    if ((e = build_items(nc, st))) return e;
+    std::vector<int> groups = plan_groups(T, emb, dist);
+    int t0 = 0;
+    for (int TG : groups) {
+        const int nt = std::min(TG, T - t0);
+        std::vector<F> gb(nt);
+        for (int t = 0; t < nt; t++) gb[t] = bnd[order[t0 + t]];
+        const int *gperm = (const int *)c.perm.p + t0;
+        if (dist == 1)
+            e = launch_tg<F, 1>(TG, dx, nc, tau, emb, gb.data(), nt, d_rr != nullptr, d_det != nullptr,
+                                d_lam != nullptr, (const int *)c.lmins.p, nl, gperm, T, d_rr, d_det, d_lam, st);
+        else
+            e = launch_tg<F, 2>(TG, dx, nc, tau, emb, gb.data(), nt, d_rr != nullptr, d_det != nullptr,
+                                d_lam != nullptr, (const int *)c.lmins.p, nl, gperm, T, d_rr, d_det, d_lam, st);
+        if (e) return e;
+        t0 += nt;
+    }
+    prof_collect(st);
+    return E_OK;
+}
+
+// ---------------------------------------------------------------- validation
+int check_args(size_t n, const int *tau, int ntau, const int *emb, int nemb, const int *lm, int nl, int dist) {
+    if (dist != 1 && dist != 2) return E_INVALID;
+    for (int a = 0; a < ntau; a++) {
+        if (tau[a] < 0) return E_INVALID;
+        for (int b = 0; b < nemb; b++) {
+            if (emb[b] < 1) return E_INVALID;
+            int64_t nc = (int64_t)n - (int64_t)(emb[b] - 1) * tau[a];
+            if (nc < 1 || nc > (int64_t)INT32_MAX) return E_INVALID;
+            for (int l = 0; l < nl; l++)
+                if (lm[l] < 0 || (int64_t)lm[l] > nc) return E_INVALID;
+        }
+    }
+    return E_OK;
+}
+
+template <typename F>
+int upload_series(const F *src, size_t n, bool src_on_device, hipStream_t st) {
+    Ctx &c = g_ctx;
+    int e = ensure(c.x, (n + PAD) * sizeof(F));
+    if (e) return e;
+    F *dx = (F *)c.x.p;
+    if (src_on_device) {
+        unsigned blocks = (unsigned)std::min<size_t>((n + PAD + 255) / 256, 4096);
+        hipLaunchKernelGGL((k_pad_copy<F>), dim3(blocks), dim3(256), 0, st, src, dx, n, n + PAD);
+        HIP_TRY(hipGetLastError());
+    } else {
+        HIP_TRY(hipMemcpyAsync(dx, src, n * sizeof(F), hipMemcpyHostToDevice, st));
+        std::vector<F> zeros(PAD, F(0));
+        HIP_TRY(hipMemcpyAsync(dx + n, zeros.data(), PAD * sizeof(F), hipMemcpyHostToDevice, st));
+        HIP_TRY(hipStreamSynchronize(st));
+    }
+    return E_OK;
+}
+
+// kind: 0 RR, 1 DET, 2 LAM
+template <typename F>
+void host_api(int kind, F *output, const F *x, size_t n, const int *tau, int ntau, const int *emb, int nemb,
+              const int *lm, int nl, const F *thr, int nthr, int dist, int *error) {
+    if (!error) return;
+    *error = E_OK;
+    const bool has_l = kind != 0;
+    if (n == 0 || ntau <= 0 || nemb <= 0 || nthr <= 0 || (has_l && nl <= 0)) *error = E_INVALID;
+    if (!output || !x || !tau || !emb || !thr || (has_l && !lm)) *error = E_NULL;
+    if (*error) return;
+    std::lock_guard<std::mutex> lk(g_mu);
+    int e = check_args(n, tau, ntau, emb, nemb, lm, has_l ? nl : 0, dist);
+    if (!e) e = init_ctx();
+    if (e) {
+        *error = e;
+        return;
+    }
+    Ctx &c = g_ctx;
+    hipStream_t st = c.stream;
+    if ((e = upload_series(x, n, false, st))) {
+        *error = e;
+        return;
+    }
+    const size_t per = kind == 0 ? (size_t)nthr : (size_t)nthr * nl * 5;
+    DevBuf &ob = kind == 0 ? c.out_rr : (kind == 1 ? c.out_det : c.out_lam);
+    if ((e = ensure(ob, per * sizeof(F)))) {
+        *error = e;
+        return;
+    }
+    for (int a = 0; a < ntau; a++)
+        for (int b = 0; b < nemb; b++) {
+            F *dout = (F *)ob.p;
+            e = run_block<F>((const F *)c.x.p, n, tau[a], emb[b], thr, nthr, has_l ? lm : nullptr, has_l ? nl : 0,
+                             dist, kind == 0 ? dout : nullptr, kind == 1 ? dout : nullptr, kind == 2 ? dout : nullptr,
+                             st);
+            if (!e) {
+                hipError_t he = hipMemcpyAsync(output + ((size_t)a * nemb + b) * per, dout, per * sizeof(F),
+                                               hipMemcpyDeviceToHost, st);
+                if (he == hipSuccess) he = hipStreamSynchronize(st);
+                if (he != hipSuccess) e = map_err(he);
+            }
+            if (e) {
+                *error = e;
+                return;
+            }
+        }
+}
+
+template <typename F>
+void host_hist(const F *x, size_t n, int tau, int emb, const F *thr, int T, int dist, int which,
+               unsigned long long *rr, unsigned long long *hd, unsigned long long *hv, int *error) {
+    if (!error) return;
+    *error = E_OK;
+    if (n == 0 || T <= 0) *error = E_INVALID;
+    if (!x || !thr || ((which & 1) && !rr) || ((which & 2) && !hd) || ((which & 4) && !hv)) *error = E_NULL;
+    if (*error) return;
+    std::lock_guard<std::mutex> lk(g_mu);
+    int e = check_args(n, &tau, 1, &emb, 1, nullptr, 0, dist);
+    if (!e) e = init_ctx();
+    if (!e) e = upload_series(x, n, false, g_ctx.stream);
+    if (e) {
+        *error = e;
+        return;
+    }
+    Ctx &c = g_ctx;
+    hipStream_t st = c.stream;
+    const int64_t nc = (int64_t)n - (int64_t)(emb - 1) * tau;
+    const size_t hs = (size_t)nc + 1;
+    // Histograms come from the same kernels, one threshold at a time so the
+    // group buffer holds exactly the threshold asked for; the finalize output
+    // is discarded (l_min = 0 is always valid).
+    int lm0 = 0;
+    if ((e = ensure(c.out_rr, 64 * sizeof(F))) || (e = ensure(c.out_det, 64 * sizeof(F))) ||
+        (e = ensure(c.out_lam, 64 * sizeof(F)))) {
+        *error = e;
+        return;
+    }
+    for (int t = 0; t < T; t++) {
+        for (int k = 0; k < 3; k++) {
+            if (!(which & (1 << k))) continue;
+            e = run_block<F>((const F *)c.x.p, n, tau, emb, thr + t, 1, &lm0, 1, dist,
+                             k == 0 ? (F *)c.out_rr.p : nullptr, k == 1 ? (F *)c.out_det.p : nullptr,
+                             k == 2 ? (F *)c.out_lam.p : nullptr, st);
+            hipError_t he = hipSuccess;
+            if (!e && k == 0) he = hipMemcpyAsync(rr + t, c.counts.p, 8, hipMemcpyDeviceToHost, st);
+            if (!e && k == 1) he = hipMemcpyAsync(hd + (size_t)t * hs, c.hist.p, hs * 8, hipMemcpyDeviceToHost, st);
+            if (!e && k == 2) he = hipMemcpyAsync(hv + (size_t)t * hs, c.hist.p, hs * 8, hipMemcpyDeviceToHost, st);
+            if (!e && he == hipSuccess) he = hipStreamSynchronize(st);
+            if (!e && he != hipSuccess) e = map_err(he);
+            if (e) {
+                *error = e;
+                return;
+            }
+        }
+    }
+}
+
+template <typename F>
+int device_api(const F *d_in, size_t n, int tau, int emb, const F *thr, int T, const int *lm, int nl, int dist,
+               F *d_rr, F *d_det, F *d_lam, void *stream) {
+    if (n == 0 || T <= 0 || ((d_det || d_lam) && nl <= 0)) return E_INVALID;
+    if (!d_in || !thr || ((d_det || d_lam) && !lm)) return E_NULL;
+    std::lock_guard<std::mutex> lk(g_mu);
+    int e = check_args(n, &tau, 1, &emb, 1, lm, (d_det || d_lam) ? nl : 0, dist);
+    if (!e) e = init_ctx();
+    if (e) return e;
+    hipStream_t st = (hipStream_t)stream;
+    if ((e = upload_series(d_in, n, true, st))) return e;
+    return run_block<F>((const F *)g_ctx.x.p, n, tau, emb, thr, T, lm, (d_det || d_lam) ? nl : 0, dist, d_rr, d_det,
+                        d_lam, st);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- C ABI
+extern "C" {
+
+void accrqa_amd_RR_f32(float *output, const float *input_data, size_t data_size, const int *tau_values, int nTaus,
+                       const int *emb_values, int nEmbs, const float *threshold_values, int nThresholds,
+                       int distance_type, int *error) {
+    host_api<float>(0, output, input_data, data_size, tau_values, nTaus, emb_values, nEmbs, nullptr, 0,
+                    threshold_values, nThresholds, distance_type, error);
+}
+void accrqa_amd_RR_f64(double *output, const double *input_data, size_t data_size, const int *tau_values, int nTaus,
+                       const int *emb_values, int nEmbs, const double *threshold_values, int nThresholds,
+                       int distance_type, int *error) {
+    host_api<double>(0, output, input_data, data_size, tau_values, nTaus, emb_values, nEmbs, nullptr, 0,
+                     threshold_values, nThresholds, distance_type, error);
+}
+void accrqa_amd_DET_f32(float *output, const float *input_data, size_t data_size, const int *tau_values, int nTaus,
+                        const int *emb_values, int nEmbs, const int *lmin_values, int nLmins,
+                        const float *threshold_values, int nThresholds, int distance_type, int calc_ENTR,
+                        int *error) {
+    (void)calc_ENTR;
+    host_api<float>(1, output, input_data, data_size, tau_values, nTaus, emb_values, nEmbs, lmin_values, nLmins,
+                    threshold_values, nThresholds, distance_type, error);
+}
+void accrqa_amd_DET_f64(double *output, const double *input_data, size_t data_size, const int *tau_values,
+                        int nTaus, const int *emb_values, int nEmbs, const int *lmin_values, int nLmins,
+                        const double *threshold_values, int nThresholds, int distance_type, int calc_ENTR,
+                        int *error) {
+    (void)calc_ENTR;
+    host_api<double>(1, output, input_data, data_size, tau_values, nTaus, emb_values, nEmbs, lmin_values, nLmins,
+                     threshold_values, nThresholds, distance_type, error);
+}
+void accrqa_amd_LAM_f32(float *output, const float *input_data, size_t data_size, const int *tau_values, int nTaus,
+                        const int *emb_values, int nEmbs, const int *vmin_values, int nVmins,
+                        const float *threshold_values, int nThresholds, int distance_type, int calc_ENTR,
+                        int *error) {
+    (void)calc_ENTR;
+    host_api<float>(2, output, input_data, data_size, tau_values, nTaus, emb_values, nEmbs, vmin_values, nVmins,
+                    threshold_values, nThresholds, distance_type, error);
+}
+void accrqa_amd_LAM_f64(double *output, const double *input_data, size_t data_size, const int *tau_values,
+                        int nTaus, const int *emb_values, int nEmbs, const int *vmin_values, int nVmins,
+                        const double *threshold_values, int nThresholds, int distance_type, int calc_ENTR,
+                        int *error) {
+    (void)calc_ENTR;
+    host_api<double>(2, output, input_data, data_size, tau_values, nTaus, emb_values, nEmbs, vmin_values, nVmins,
+                     threshold_values, nThresholds, distance_type, error);
+}
+
+void accrqa_amd_histograms_f32(const float *input_data, size_t data_size, int tau, int emb,
+                               const float *threshold_values, int nThresholds, int distance_type, int which,
+                               unsigned long long *rr_count, unsigned long long *hist_det,
+                               unsigned long long *hist_lam, int *error) {
+    host_hist<float>(input_data, data_size, tau, emb, threshold_values, nThresholds, distance_type, which, rr_count,
+                     hist_det, hist_lam, error);
+}
+void accrqa_amd_histograms_f64(const double *input_data, size_t data_size, int tau, int emb,
+                               const double *threshold_values, int nThresholds, int distance_type, int which,
+                               unsigned long long *rr_count, unsigned long long *hist_det,
+                               unsigned long long *hist_lam, int *error) {
+    host_hist<double>(input_data, data_size, tau, emb, threshold_values, nThresholds, distance_type, which, rr_count,
+                      hist_det, hist_lam, error);
+}
+
+int accrqa_amd_rqa_device_f32(const float *d_input, size_t data_size, int tau, int emb,
+                              const float *threshold_values, int nThresholds, const int *lmin_values, int nL,
+                              int distance_type, float *d_rr, float *d_det, float *d_lam, void *stream) {
+    return device_api<float>(d_input, data_size, tau, emb, threshold_values, nThresholds, lmin_values, nL,
+                             distance_type, d_rr, d_det, d_lam, stream);
+}
+int accrqa_amd_rqa_device_f64(const double *d_input, size_t data_size, int tau, int emb,
+                              const double *threshold_values, int nThresholds, const int *lmin_values, int nL,
+                              int distance_type, double *d_rr, double *d_det, double *d_lam, void *stream) {
+    return device_api<double>(d_input, data_size, tau, emb, threshold_values, nThresholds, lmin_values, nL,
+                              distance_type, d_rr, d_det, d_lam, stream);
+}
+
+const char *accrqa_amd_last_error(void) { return g_last_error.c_str(); }
+
+void accrqa_amd_profile(int enable) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_prof.on = enable != 0;
+}
+
+void accrqa_amd_profile_reset(void) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_prof.acc.clear();
+}
+
+int accrqa_amd_profile_read(const char *kernel, double *total_ms, long long *launches) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    for (auto &kv : g_prof.acc)
+        if (kernel && kv.first == kernel) {
+            if (total_ms) *total_ms = kv.second.first;
+            if (launches) *launches = kv.second.second;
+            return 0;
+        }
+    if (total_ms) *total_ms = 0.0;
+    if (launches) *launches = 0;
+    return 1;
+}
+
+const char *accrqa_amd_version(void) { return "0.1.0"; }
+
+}  // extern "C"

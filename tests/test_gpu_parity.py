@@ -1,0 +1,269 @@
+"""GPU parity: the HIP path (through the C ABI) against the oracle and the
+reference's golden fixtures.
+
+Bar (from the build's contract): recurrence counts and line-length histograms
+bit-exact; RR, DET/LAM, L/TT and Lmax/TTmax bit-exact (they are integer ratios
+divided exactly as the reference divides); ENTR within 1e-6 relative for fp32
+outputs and 1e-12 relative for fp64 outputs (the GPU sums p*log(p) in a
+different order with the device log).
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import case_arrays, dense, same_bits, unhex
+
+pytestmark = pytest.mark.gpu
+
+ENTR_RTOL = {np.float32: 1e-6, np.float64: 1e-12}
+
+
+@pytest.fixture(scope="module")
+def amd():
+    import accrqa_amd
+    accrqa_amd.load()
+    return accrqa_amd
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def assert_measures(got, exp, dt, what):
+    """got/exp: [..., 5] arrays of one measure tuple layout."""
+    got = np.asarray(got).reshape(-1, 5)
+    exp = np.asarray(exp).reshape(-1, 5)
+    for k in (0, 1, 2, 4):
+        assert same_bits(got[:, k].copy(), exp[:, k].copy()), (what, k, got[:, k], exp[:, k])
+    g, e = got[:, 3].astype(np.float64), exp[:, 3].astype(np.float64)
+    tol = ENTR_RTOL[dt]
+    for a, b in zip(g, e):
+        if math.isnan(b):
+            assert math.isnan(a), (what, a, b)
+        else:
+            assert abs(a - b) <= tol * max(abs(b), 1e-300) or abs(a - b) <= 4 * np.finfo(dt).eps * max(abs(b), 1), \
+                (what, a, b)
+
+
+def check_case(amd, oracle, x, taus, embs, lmins, thrs, dist, what):
+    dt = x.dtype.type
+    rr = amd.RR(x, taus, embs, thrs, dist)
+    rr_o, e = oracle.RR(x, taus, embs, thrs, dist)
+    assert e == 0
+    assert same_bits(rr.reshape(-1), rr_o), (what, "RR", rr.reshape(-1), rr_o)
+    det = amd.DET(x, taus, embs, lmins, thrs, dist)
+    det_o, e = oracle.DET(x, taus, embs, lmins, thrs, dist)
+    assert_measures(det, det_o, dt, what + " DET")
+    lam = amd.LAM(x, taus, embs, lmins, thrs, dist)
+    lam_o, e = oracle.LAM(x, taus, embs, lmins, thrs, dist)
+    assert_measures(lam, lam_o, dt, what + " LAM")
+
+
+# ----------------------------------------------------------------- golden fixtures
+
+
+def test_golden_histograms_bit_exact(amd, golden):
+    for case in golden["cases"]:
+        say("golden hist", case["name"])
+        dt, x, _ = case_arrays(case)
+        by_key = {}
+        for h in case["hist"]:
+            by_key.setdefault((h["tau"], h["emb"]), []).append(h)
+        for (tau, emb), hs in by_key.items():
+            thr = np.array([float.fromhex(h["thr"]) for h in hs], dtype=dt)
+            got = amd.histograms(x, tau, emb, thr, case["dist"], which=7)
+            for t, h in enumerate(hs):
+                hd = dense(h["hD"], h["nc"])
+                hv = dense(h["hV"], h["nc"])
+                assert np.array_equal(got["det"][t], hd), (case["name"], tau, emb, h["thr"], "h_D")
+                assert np.array_equal(got["lam"][t], hv), (case["name"], tau, emb, h["thr"], "h_V")
+                lv = np.arange(h["nc"] + 1, dtype=np.uint64)
+                assert int(got["rr"][t]) == int((lv * hv).sum()), (case["name"], "count")
+
+
+def test_golden_api_outputs(amd, golden):
+    for case in golden["cases"]:
+        say("golden api", case["name"])
+        dt, x, thr = case_arrays(case)
+        d = case["dist"]
+        rr = amd.RR(x, case["tau"], case["emb"], thr, d)
+        assert same_bits(rr.reshape(-1), unhex(case["RR"], dt)), case["name"]
+        det = amd.DET(x, case["tau"], case["emb"], case["lmin"], thr, d)
+        assert_measures(det, unhex(case["DET"], dt), dt, case["name"] + " DET")
+        lam = amd.LAM(x, case["tau"], case["emb"], case["lmin"], thr, d)
+        assert_measures(lam, unhex(case["LAM"], dt), dt, case["name"] + " LAM")
+
+
+def test_crqa_known_answers_on_gpu(amd, crqa_kat, crqa_series):
+    for rec in crqa_kat["DET"]:
+        d = 1 if rec["norm"] == "euclidean" else 2
+        out = amd.DET(crqa_series, [int(rec["tau"])], [int(rec["emb"])], [int(rec["lmin"])], [rec["thr"]], d)
+        v = out.reshape(5)
+        assert abs(v[0] - rec["CRQA_DET"]) <= 1e-4 and abs(v[1] - rec["CRQA_L"]) <= 1e-4
+        assert abs(v[3] - rec["CRQA_ENTR"]) <= 1e-4 and abs(v[4] - rec["CRQA_RR"]) <= 1e-4
+    for rec in crqa_kat["LAM"]:
+        d = 1 if rec["norm"] == "euclidean" else 2
+        out = amd.LAM(crqa_series, [int(rec["tau"])], [int(rec["emb"])], [int(rec["vmin"])], [rec["thr"]], d)
+        v = out.reshape(5)
+        assert abs(v[0] - rec["CRQA_LAM"]) <= 1e-4 and abs(v[1] - rec["CRQA_TT"]) <= 1e-4
+        assert abs(v[4] - rec["CRQA_RR"]) <= 1e-4
+
+
+# ----------------------------------------------------------------- oracle, seeded inputs
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+@pytest.mark.parametrize("dist", [1, 2])
+def test_multi_segment_multi_group(amd, oracle, dt, dist):
+    """N_c of a few thousand: several row/diagonal segments (stitch path), and
+    more thresholds than one group (group planner + permutation)."""
+    rng = np.random.default_rng(100 + dist)
+    x = np.cumsum(rng.standard_normal(4500)).astype(dt)
+    x = (x - x.mean()) / x.std()
+    thrs = sorted(rng.uniform(0.01, 0.6, size=37).astype(dt))[::-1]   # unsorted order on purpose
+    say("multi", dt, dist)
+    check_case(amd, oracle, x.astype(dt), [1, 3], [2, 5], [2, 7], thrs, dist, f"multi {dt} {dist}")
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_random_shapes(amd, oracle, dt):
+    rng = np.random.default_rng(7)
+    for trial in range(5):
+        n = int(rng.integers(50, 2600))
+        x = rng.standard_normal(n).astype(dt)
+        tau = int(rng.integers(1, 9))
+        emb = int(rng.integers(1, 12))
+        if n - (emb - 1) * tau < 2:
+            continue
+        thrs = list(rng.uniform(0.1, 4.0, size=int(rng.integers(1, 9))).astype(dt))
+        dist = 1 + trial % 2
+        say("random", trial, n, tau, emb, len(thrs), dist)
+        check_case(amd, oracle, x, [tau], [emb], [1, 2, 3], thrs, dist, f"random {trial}")
+
+
+def test_long_lines_beyond_lds_cap_and_segments(amd, oracle):
+    """Runs longer than the LDS histogram cap (256) and than a segment (1024
+    steps): plateaus of 1,300 nearly equal samples give long diagonal and
+    vertical lines that cross segment borders."""
+    rng = np.random.default_rng(4)
+    levels = [0.0, 1.0, 0.0, 2.0, 1.0]
+    x = np.concatenate([np.full(1300, v) for v in levels]) + 1e-3 * rng.standard_normal(6500)
+    x = x.astype(np.float32)
+    for dist in (1, 2):
+        say("long lines", dist)
+        got = amd.histograms(x, 1, 2, np.array([0.05, 0.3], np.float32), dist, which=7)
+        for k, thr in enumerate([0.05, 0.3]):
+            hd = oracle.hist("det", x, np.float32(thr), 1, 2, dist)
+            hv = oracle.hist("lam", x, np.float32(thr), 1, 2, dist)
+            assert np.array_equal(got["det"][k], hd)
+            assert np.array_equal(got["lam"][k], hv)
+            assert hv[1025:].sum() > 0 and hd[1025:-1].sum() > 0   # the case really has long lines
+
+
+def test_constant_series_full_recurrence(amd):
+    for dt in (np.float32, np.float64):
+        for n in (1, 2, 3, 64, 65, 1500):
+            x = np.full(n, 3.25, dtype=dt)
+            say("constant", dt, n)
+            got = amd.histograms(x, 1, 1, np.array([0.0, 1.0], dtype=dt), 1, which=7)
+            for k in range(2):
+                assert int(got["rr"][k]) == n * n
+                hv = np.zeros(n + 1, np.uint64)
+                hv[n] = n
+                assert np.array_equal(got["lam"][k], hv)
+                hd = np.zeros(n + 1, np.uint64)
+                for r in range(1, n):
+                    hd[n - r] += 2
+                hd[n] += 1
+                assert np.array_equal(got["det"][k], hd)
+
+
+def test_special_values(amd, oracle):
+    rng = np.random.default_rng(9)
+    for dt in (np.float32, np.float64):
+        x = rng.standard_normal(700).astype(dt)
+        x[[5, 300]] = np.nan
+        x[[100, 101, 102]] = np.inf
+        x[400] = -np.inf
+        x[500:520] = 0.0
+        x[520] = np.finfo(dt).tiny / 4   # subnormal differences
+        thrs = [-1.0, -0.0, 0.0, np.finfo(dt).tiny, 1e-30, 0.7, np.inf, np.nan]
+        for dist in (1, 2):
+            say("special", dt, dist)
+            check_case(amd, oracle, x, [1, 2], [1, 3], [1, 2], thrs, dist, f"special {dt} {dist}")
+
+
+def test_tie_thresholds_exact(amd, oracle):
+    """Integer data: squared distances are exact integers and thresholds sit
+    exactly on attainable distances (sqrt(k) rounded), the worst case for the
+    squared-bound rewrite of the square root."""
+    rng = np.random.default_rng(13)
+    for dt in (np.float32, np.float64):
+        x = rng.integers(-6, 7, size=1300).astype(dt)
+        thrs = [dt(np.sqrt(dt(k))) for k in (1, 2, 3, 5, 8, 13, 50, 99)]
+        for dist in (1, 2):
+            say("ties", dt, dist)
+            check_case(amd, oracle, x, [1, 4], [3, 6], [2], thrs, dist, f"ties {dt} {dist}")
+
+
+def test_argument_errors_on_gpu(amd):
+    x = np.ones(10, dtype=np.float32)
+    with pytest.raises(amd.AccrqaError) as ei:
+        amd.RR(x, [4], [4], [0.5], 1)        # N_c < 1
+    assert ei.value.code == 2
+    with pytest.raises(amd.AccrqaError) as ei:
+        amd.DET(x, [1], [1], [11], [0.5], 1)   # l_min > N_c
+    assert ei.value.code == 2
+    with pytest.raises(amd.AccrqaError) as ei:
+        amd.LAM(x, [1], [1], [2], [0.5], 0)    # invalid distance
+    assert ei.value.code == 2
+
+
+def test_cxx_api_platform_dispatch(amd, oracle):
+    """The reference's mangled C++ entry point with PLT_AMD_GPU runs on the GPU;
+    other platform values are refused with ERR_INVALID_ARGUMENT."""
+    import ctypes as C
+    L = amd.load()
+    f = getattr(L, "_Z10accrqa_DETPfS_mPiiS0_iS0_iS_i15Accrqa_Distancei19Accrqa_CompPlatformP12Accrqa_Error")
+    rng = np.random.default_rng(21)
+    x = rng.standard_normal(900).astype(np.float32)
+    tau = np.array([2], np.int32)
+    emb = np.array([3], np.int32)
+    lm = np.array([2, 3], np.int32)
+    thr = np.array([0.5, 1.0], np.float32)
+    out = np.zeros(2 * 2 * 5, np.float32)
+    err = C.c_int(-1)
+    args = [out.ctypes.data, x.ctypes.data, C.c_size_t(x.size), tau.ctypes.data, 1, emb.ctypes.data, 1,
+            lm.ctypes.data, 2, thr.ctypes.data, 2, 1, 1]
+    f(*args, 2048, C.byref(err))
+    assert err.value == 0
+    exp, _ = oracle.DET(x, [2], [3], [2, 3], [0.5, 1.0], 1)
+    assert_measures(out, exp, np.float32, "cxx DET")
+    f(*args, 1, C.byref(err))
+    assert err.value == 2
+
+
+def test_device_entry_point_matches_host(amd, oracle):
+    import torch
+    rng = np.random.default_rng(31)
+    x = rng.standard_normal(3000).astype(np.float32)
+    thr = [0.3, 0.6, 0.9]
+    lm = [2, 4]
+    dx = torch.from_numpy(x).cuda()
+    d_rr = torch.zeros(3, dtype=torch.float32, device="cuda")
+    d_det = torch.zeros(3 * 2 * 5, dtype=torch.float32, device="cuda")
+    d_lam = torch.zeros(2 * 3 * 5, dtype=torch.float32, device="cuda")
+    s = torch.cuda.current_stream()
+    say("device entry")
+    amd.rqa_device(dx.data_ptr(), x.size, 2, 4, thr, lm, 1, d_rr.data_ptr(), d_det.data_ptr(), d_lam.data_ptr(),
+                   stream=s.cuda_stream, dtype=np.float32)
+    torch.cuda.synchronize()
+    rr_o, _ = oracle.RR(x, [2], [4], thr, 1)
+    assert same_bits(d_rr.cpu().numpy(), rr_o)
+    det_o, _ = oracle.DET(x, [2], [4], lm, thr, 1)
+    assert_measures(d_det.cpu().numpy(), det_o, np.float32, "device DET")
+    lam_o, _ = oracle.LAM(x, [2], [4], lm, thr, 1)
+    assert_measures(d_lam.cpu().numpy(), lam_o, np.float32, "device LAM")
