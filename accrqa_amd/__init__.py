@@ -60,6 +60,15 @@ def load():
         return _lib
     if not os.path.exists(_LIB_PATH):
         raise AccrqaError(7, f"{_LIB_PATH} is not built; run accrqa_amd.build.build()")
+    # PyTorch wheels ship their own HIP runtime under the same soname
+    # (libamdhip64.so.7).  Loading torch first makes the dynamic loader bind
+    # this library to that one copy; loading ours first would bring in
+    # /opt/rocm's runtime and torch would then fail to initialise a second one
+    # in the same process.
+    try:
+        import torch  # noqa: F401
+    except Exception:
+        pass
     L = C.CDLL(_LIB_PATH)
     P, I, SZ = C.c_void_p, C.c_int, C.c_size_t
     for sfx in ("f32", "f64"):

@@ -228,6 +228,9 @@ def test_cxx_api_platform_dispatch(amd, oracle):
     import ctypes as C
     L = amd.load()
     f = getattr(L, "_Z10accrqa_DETPfS_mPiiS0_iS0_iS_i15Accrqa_Distancei19Accrqa_CompPlatformP12Accrqa_Error")
+    P, I = C.c_void_p, C.c_int
+    f.argtypes = [P, P, C.c_size_t, P, I, P, I, P, I, P, I, I, I, I, P]
+    f.restype = None
     rng = np.random.default_rng(21)
     x = rng.standard_normal(900).astype(np.float32)
     tau = np.array([2], np.int32)
@@ -236,7 +239,7 @@ def test_cxx_api_platform_dispatch(amd, oracle):
     thr = np.array([0.5, 1.0], np.float32)
     out = np.zeros(2 * 2 * 5, np.float32)
     err = C.c_int(-1)
-    args = [out.ctypes.data, x.ctypes.data, C.c_size_t(x.size), tau.ctypes.data, 1, emb.ctypes.data, 1,
+    args = [out.ctypes.data, x.ctypes.data, x.size, tau.ctypes.data, 1, emb.ctypes.data, 1,
             lm.ctypes.data, 2, thr.ctypes.data, 2, 1, 1]
     f(*args, 2048, C.byref(err))
     assert err.value == 0
