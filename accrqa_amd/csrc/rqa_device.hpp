@@ -42,7 +42,6 @@ constexpr int BLK = 32;            // steps per decision word
 constexpr int WG = 256;            // threads per workgroup (4 waves)
 constexpr int WAVES = WG / WAVE;
 constexpr int LCAP = 256;          // run lengths < LCAP use the LDS histogram
-constexpr uint32_t OPEN = 0xFFFFFFFFu;
 constexpr int DET_WIN = 1024;      // LDS samples per wave for the k_det lane window
 
 template <typename F, int TG>
@@ -85,29 +84,97 @@ __device__ __forceinline__ void emit(uint32_t len, uint32_t *lh, unsigned long l
         glob_add(&gh[len], 1ull);
 }
 
+// ---------------------------------------------------------------- decision words
+// w = 2w + [s_u <= bound] for u = 0..31, i.e. bit 31-u holds step u.  One
+// v_cmp (writes a lane mask) and one v_addc (consumes it as carry-in) per
+// decision.  Four compares go to four SGPR pairs before the four adds read
+// them, so every mask is read at least four instructions after it is
+// written (gfx950 requires two wait states between a VALU mask write and a
+// VALU carry read).  NaN s never compares true, as in the reference.
+template <typename F>
+__device__ __forceinline__ uint32_t decision_word(const F (&s)[BLK], F bound) {
+    uint32_t w = 0;
+#pragma unroll
+    for (int u = 0; u < BLK; u += 4) {
+        uint64_t m0, m1, m2, m3, c;
+        if constexpr (sizeof(F) == 4) {
+            asm("v_cmp_ge_f32_e64 %[m0], %[b], %[s0]\n\t"
+                "v_cmp_ge_f32_e64 %[m1], %[b], %[s1]\n\t"
+                "v_cmp_ge_f32_e64 %[m2], %[b], %[s2]\n\t"
+                "v_cmp_ge_f32_e64 %[m3], %[b], %[s3]\n\t"
+                "v_addc_co_u32_e64 %[w], %[c], %[w], %[w], %[m0]\n\t"
+                "v_addc_co_u32_e64 %[w], %[c], %[w], %[w], %[m1]\n\t"
+                "v_addc_co_u32_e64 %[w], %[c], %[w], %[w], %[m2]\n\t"
+                "v_addc_co_u32_e64 %[w], %[c], %[w], %[w], %[m3]"
+                : [w] "+v"(w), [m0] "=&s"(m0), [m1] "=&s"(m1), [m2] "=&s"(m2), [m3] "=&s"(m3), [c] "=&s"(c)
+                : [b] "v"(bound), [s0] "v"(s[u]), [s1] "v"(s[u + 1]), [s2] "v"(s[u + 2]), [s3] "v"(s[u + 3]));
+        } else {
+            asm("v_cmp_ge_f64_e64 %[m0], %[b], %[s0]\n\t"
+                "v_cmp_ge_f64_e64 %[m1], %[b], %[s1]\n\t"
+                "v_cmp_ge_f64_e64 %[m2], %[b], %[s2]\n\t"
+                "v_cmp_ge_f64_e64 %[m3], %[b], %[s3]\n\t"
+                "v_addc_co_u32_e64 %[w], %[c], %[w], %[w], %[m0]\n\t"
+                "v_addc_co_u32_e64 %[w], %[c], %[w], %[w], %[m1]\n\t"
+                "v_addc_co_u32_e64 %[w], %[c], %[w], %[w], %[m2]\n\t"
+                "v_addc_co_u32_e64 %[w], %[c], %[w], %[w], %[m3]"
+                : [w] "+v"(w), [m0] "=&s"(m0), [m1] "=&s"(m1), [m2] "=&s"(m2), [m3] "=&s"(m3), [c] "=&s"(c)
+                : [b] "v"(bound), [s0] "v"(s[u]), [s1] "v"(s[u + 1]), [s2] "v"(s[u + 2]), [s3] "v"(s[u + 3]));
+        }
+    }
+    return w;
+}
+
+// Weighted count: cnt += [s_u <= bound] for u = 0..31 (same interleaving).
+template <typename F>
+__device__ __forceinline__ uint32_t count_le(const F (&s)[BLK], F bound, uint32_t cnt) {
+#pragma unroll
+    for (int u = 0; u < BLK; u += 4) {
+        uint64_t m0, m1, m2, m3, c;
+        if constexpr (sizeof(F) == 4) {
+            asm("v_cmp_ge_f32_e64 %[m0], %[b], %[s0]\n\t"
+                "v_cmp_ge_f32_e64 %[m1], %[b], %[s1]\n\t"
+                "v_cmp_ge_f32_e64 %[m2], %[b], %[s2]\n\t"
+                "v_cmp_ge_f32_e64 %[m3], %[b], %[s3]\n\t"
+                "v_addc_co_u32_e64 %[n], %[c], %[n], 0, %[m0]\n\t"
+                "v_addc_co_u32_e64 %[n], %[c], %[n], 0, %[m1]\n\t"
+                "v_addc_co_u32_e64 %[n], %[c], %[n], 0, %[m2]\n\t"
+                "v_addc_co_u32_e64 %[n], %[c], %[n], 0, %[m3]"
+                : [n] "+v"(cnt), [m0] "=&s"(m0), [m1] "=&s"(m1), [m2] "=&s"(m2), [m3] "=&s"(m3), [c] "=&s"(c)
+                : [b] "v"(bound), [s0] "v"(s[u]), [s1] "v"(s[u + 1]), [s2] "v"(s[u + 2]), [s3] "v"(s[u + 3]));
+        } else {
+            asm("v_cmp_ge_f64_e64 %[m0], %[b], %[s0]\n\t"
+                "v_cmp_ge_f64_e64 %[m1], %[b], %[s1]\n\t"
+                "v_cmp_ge_f64_e64 %[m2], %[b], %[s2]\n\t"
+                "v_cmp_ge_f64_e64 %[m3], %[b], %[s3]\n\t"
+                "v_addc_co_u32_e64 %[n], %[c], %[n], 0, %[m0]\n\t"
+                "v_addc_co_u32_e64 %[n], %[c], %[n], 0, %[m1]\n\t"
+                "v_addc_co_u32_e64 %[n], %[c], %[n], 0, %[m2]\n\t"
+                "v_addc_co_u32_e64 %[n], %[c], %[n], 0, %[m3]"
+                : [n] "+v"(cnt), [m0] "=&s"(m0), [m1] "=&s"(m1), [m2] "=&s"(m2), [m3] "=&s"(m3), [c] "=&s"(c)
+                : [b] "v"(bound), [s0] "v"(s[u]), [s1] "v"(s[u + 1]), [s2] "v"(s[u + 2]), [s3] "v"(s[u + 3]));
+        }
+    }
+    return cnt;
+}
+
 // Consume one 32-step word (bit 31 = oldest step).  run: open run entering
-// the word; pre: OPEN until the segment's first zero, then the length of the
-// segment's prefix run (reported to the stitch instead of being counted).
-__device__ __forceinline__ void consume_word(uint32_t w, uint32_t &run, uint32_t &pre, uint32_t *lh,
-                                             unsigned long long *gh, uint32_t hmax) {
+// the word.  Bit tb of popen is set until the segment's first zero; the run
+// it closes is the segment's prefix run, written to *pre (the stitch joins
+// it with the previous segment) instead of being counted here.
+__device__ __forceinline__ void consume_word(uint32_t w, uint32_t &run, uint32_t &popen, uint32_t tb, uint32_t *pre,
+                                             uint32_t *lh, unsigned long long *gh, uint32_t hmax) {
     if (w == 0xFFFFFFFFu) {
         run += BLK;
         return;
     }
-    if (w == 0u) {
-        if (pre == OPEN)
-            pre = run;
-        else if (run)
-            emit(run, lh, gh, hmax);
-        run = 0;
-        return;
-    }
     const uint32_t lead = __clz(~w);             // ones above the highest zero
     const uint32_t first = run + lead;
-    if (pre == OPEN)
-        pre = first;
-    else if (first)
+    if (popen & tb) {
+        *pre = first;
+        popen &= ~tb;
+    } else if (first) {
         emit(first, lh, gh, hmax);
+    }
     const uint32_t tail = __builtin_ctz(~w);     // ones below the lowest zero
     uint32_t mid = w & (0xFFFFFFFFu >> lead) & ~((1u << tail) - 1u);
     while (mid) {
@@ -119,9 +186,36 @@ __device__ __forceinline__ void consume_word(uint32_t w, uint32_t &run, uint32_t
     run = tail;
 }
 
+// Segment end: (prefix, suffix) of the segment for the stitch.  A segment
+// whose prefix never closed is all ones: prefix = suffix = its length.
+__device__ __forceinline__ void close_segment(uint2 *slot, uint32_t run, uint32_t popen, uint32_t tb) {
+    if (popen & tb)
+        *slot = make_uint2(run, run);
+    else
+        slot->y = run;
+}
+
+// Copy a uniform value into a VGPR (keeps the threshold bounds out of the
+// scalar register file, which the row window and the lane masks need).
+template <typename F>
+__device__ __forceinline__ F to_vgpr(F v) {
+    F r;
+    if constexpr (sizeof(F) == 4)
+        asm volatile("v_mov_b32 %0, %1" : "=v"(r) : "s"(v));
+    else
+        asm volatile("v_mov_b64 %0, %1" : "=v"(r) : "s"(v));
+    return r;
+}
+
+__device__ __forceinline__ int wave_id() {
+    return __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
+}
+
 // ---------------------------------------------------------------- k_lam
-// Work item = (band of 64 columns, row segment q).  x is padded with >= 64
+// Work item = (band of 64 columns, row segment q).  x is padded with >= 128
 // zeros past N so that the unguarded window reads stay inside the buffer.
+// The row window x[i0 + u + k*tau] (u < 32) is uniform across the wave and
+// comes through scalar loads; x[j + k*tau] is one vector load per k and block.
 template <typename F, int NORM, int TG>
 __global__ __launch_bounds__(WG) void k_lam(const F *__restrict__ x, int64_t nc, int tau, int m, Bounds<F, TG> S,
                                             int nt, int seg_len, int nseg, unsigned long long *__restrict__ hist,
@@ -131,9 +225,13 @@ __global__ __launch_bounds__(WG) void k_lam(const F *__restrict__ x, int64_t nc,
     __syncthreads();
 
     const int lane = threadIdx.x & (WAVE - 1);
-    const int wave = threadIdx.x / WAVE;
+    const int wave = wave_id();
     const int64_t nbands = (nc + WAVE - 1) / WAVE;
     const int64_t nitems = nbands * nseg;
+    const size_t cstride = (size_t)nc * nseg;   // carry stride between thresholds
+    F bnd[TG];
+#pragma unroll
+    for (int t = 0; t < TG; t++) bnd[t] = to_vgpr(S.s[t]);
 
     for (int64_t item = (int64_t)blockIdx.x * WAVES + wave; item < nitems; item += (int64_t)gridDim.x * WAVES) {
         const int64_t band = item / nseg;
@@ -142,44 +240,43 @@ __global__ __launch_bounds__(WG) void k_lam(const F *__restrict__ x, int64_t nc,
         const bool jv = j < nc;
         const int64_t ib = (int64_t)q * seg_len;
         const int64_t ie = ib + seg_len < nc ? ib + seg_len : nc;
+        uint2 *cslot = carry + (size_t)(jv ? j : 0) * nseg + q;
 
-        uint32_t run[TG], pre[TG];
+        uint32_t run[TG];
 #pragma unroll
-        for (int t = 0; t < TG; t++) {
-            run[t] = 0;
-            pre[t] = OPEN;
-        }
+        for (int t = 0; t < TG; t++) run[t] = 0;
+        uint32_t popen = jv ? 0xFFFFFFFFu : 0u;
 
         for (int64_t i0 = ib; i0 < ie; i0 += BLK) {
             const int nb = ie - i0 < BLK ? (int)(ie - i0) : BLK;
             F s[BLK];
 #pragma unroll
             for (int u = 0; u < BLK; u++) s[u] = F(0);
+            F xj = x[j];
             for (int k = 0; k < m; k++) {
-                const F xj = x[j + (int64_t)k * tau];
+                // next dimension's column sample, loaded while this one computes
+                const F xjn = x[j + (int64_t)(k + 1 < m ? k + 1 : k) * tau];
                 const F *xr = x + i0 + (int64_t)k * tau;
 #pragma unroll
                 for (int u = 0; u < BLK; u++) s[u] = dim_step<NORM>(s[u], xr[u], xj);
+                xj = xjn;
             }
             const uint32_t fill = nb == BLK ? 0u : ((1u << (BLK - nb)) - 1u);
 #pragma unroll
             for (int t = 0; t < TG; t++) {
-                uint32_t w = 0;
-#pragma unroll
-                for (int u = 0; u < BLK; u++) w = w + w + (s[u] <= S.s[t] ? 1u : 0u);
-                w = jv ? (w | fill) : 0u;
-                consume_word(w, run[t], pre[t], lh + t * LCAP, hist + (size_t)t * (nc + 1), (uint32_t)nc);
-                if (fill) run[t] -= jv ? (uint32_t)(BLK - nb) : 0u;
+                if (t < nt) {
+                    uint32_t w = decision_word(s, bnd[t]);
+                    w = jv ? (w | fill) : 0u;
+                    consume_word(w, run[t], popen, 1u << t, &cslot[(size_t)t * cstride].x, lh + t * LCAP,
+                                 hist + (size_t)t * (nc + 1), (uint32_t)nc);
+                    if (fill && jv) run[t] -= (uint32_t)(BLK - nb);
+                }
             }
         }
         if (jv) {
 #pragma unroll
-            for (int t = 0; t < TG; t++) {
-                if (t < nt) {
-                    const uint32_t p = pre[t] == OPEN ? run[t] : pre[t];
-                    carry[((size_t)t * nc + j) * nseg + q] = make_uint2(p, run[t]);
-                }
-            }
+            for (int t = 0; t < TG; t++)
+                if (t < nt) close_segment(&cslot[(size_t)t * cstride], run[t], popen, 1u << t);
         }
     }
 
@@ -194,8 +291,9 @@ __global__ __launch_bounds__(WG) void k_lam(const F *__restrict__ x, int64_t nc,
 // ---------------------------------------------------------------- k_det
 // Work item = (strip of 64 diagonals r = r0..r0+63, step segment q) taken
 // from a host-built table.  Step s of lane l is the pair (s, s + r0 + l);
-// steps past the end of a diagonal decide 0.  The per-lane operand window
-// x[s + r0 + l + k*tau] is staged in LDS once per 32-step block.
+// steps past the end of a diagonal decide 0.  The row window x[s + k*tau] is
+// uniform (scalar loads); the per-lane window x[s + r0 + l + k*tau] is staged
+// in LDS once per 32-step block and dimension chunk.
 template <typename F, int NORM, int TG>
 __global__ __launch_bounds__(WG) void k_det(const F *__restrict__ x, int64_t nc, int tau, int m, Bounds<F, TG> S,
                                             int nt, int seg_len, int nseg, const int2 *__restrict__ items,
@@ -207,11 +305,15 @@ __global__ __launch_bounds__(WG) void k_det(const F *__restrict__ x, int64_t nc,
     __syncthreads();
 
     const int lane = threadIdx.x & (WAVE - 1);
-    const int wave = threadIdx.x / WAVE;
+    const int wave = wave_id();
     F *wn = win[wave];
     // dimensions staged per window: span = BLK + WAVE - 1 + (kc-1)*tau <= DET_WIN
     const int kc_max = tau > 0 ? (DET_WIN - (BLK + WAVE - 1)) / tau + 1 : m;
     const int kc = kc_max < m ? kc_max : m;
+    const size_t cstride = (size_t)(nc - 1) * nseg;
+    F bnd[TG];
+#pragma unroll
+    for (int t = 0; t < TG; t++) bnd[t] = to_vgpr(S.s[t]);
 
     for (int64_t item = (int64_t)blockIdx.x * WAVES + wave; item < nitems; item += (int64_t)gridDim.x * WAVES) {
         const int2 it = items[item];
@@ -224,13 +326,12 @@ __global__ __launch_bounds__(WG) void k_det(const F *__restrict__ x, int64_t nc,
         const int64_t se_nom = sb + seg_len;
         const int64_t smax = nc - r0;                // lane 0's diagonal length
         const int64_t se = se_nom < smax ? se_nom : smax;
+        uint2 *cslot = carry + (size_t)(rv ? r - 1 : 0) * nseg + q;
 
-        uint32_t run[TG], pre[TG];
+        uint32_t run[TG];
 #pragma unroll
-        for (int t = 0; t < TG; t++) {
-            run[t] = 0;
-            pre[t] = OPEN;
-        }
+        for (int t = 0; t < TG; t++) run[t] = 0;
+        uint32_t popen = rv ? 0xFFFFFFFFu : 0u;
 
         for (int64_t s0 = sb; s0 < se; s0 += BLK) {
             F s[BLK];
@@ -241,8 +342,8 @@ __global__ __launch_bounds__(WG) void k_det(const F *__restrict__ x, int64_t nc,
                 const int64_t base = s0 + r0 + (int64_t)k0 * tau;
                 const int span = BLK + WAVE - 1 + (k1 - 1 - k0) * tau;
                 for (int e = lane; e < span; e += WAVE) wn[e] = x[base + e];
-                __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 for (int k = k0; k < k1; k++) {
                     const F *xr = x + s0 + (int64_t)k * tau;
@@ -250,8 +351,8 @@ __global__ __launch_bounds__(WG) void k_det(const F *__restrict__ x, int64_t nc,
 #pragma unroll
                     for (int u = 0; u < BLK; u++) s[u] = dim_step<NORM>(s[u], xr[u], wl[u]);
                 }
-                __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             }
             // bit (31-u) valid iff s0+u < dlen; invalid steps decide 0
@@ -260,25 +361,20 @@ __global__ __launch_bounds__(WG) void k_det(const F *__restrict__ x, int64_t nc,
             const uint32_t vmask = nvalid == 0 ? 0u : (0xFFFFFFFFu << (BLK - nvalid));
 #pragma unroll
             for (int t = 0; t < TG; t++) {
-                uint32_t w = 0;
-#pragma unroll
-                for (int u = 0; u < BLK; u++) w = w + w + (s[u] <= S.s[t] ? 1u : 0u);
-                w &= vmask;
-                consume_word(w, run[t], pre[t], lh + t * LCAP, hist + (size_t)t * (nc + 1), (uint32_t)nc);
-            }
-        }
-        // The last block of a strip's final segment can run past se_nom only
-        // when se == smax, which is a multiple-of-nothing boundary: steps past
-        // se decide 0 for every lane, so a segment shorter than seg_len ends
-        // with the same (pre, suf) as one padded with zeros.
-        if (rv) {
-#pragma unroll
-            for (int t = 0; t < TG; t++) {
                 if (t < nt) {
-                    const uint32_t p = pre[t] == OPEN ? run[t] : pre[t];
-                    carry[((size_t)t * (nc - 1) + (r - 1)) * nseg + q] = make_uint2(p, run[t]);
+                    const uint32_t w = decision_word(s, bnd[t]) & vmask;
+                    consume_word(w, run[t], popen, 1u << t, &cslot[(size_t)t * cstride].x, lh + t * LCAP,
+                                 hist + (size_t)t * (nc + 1), (uint32_t)nc);
                 }
             }
+        }
+        // Steps past the strip's end decide 0 for every lane, so a segment cut
+        // short by the end of the diagonals reports the same (prefix, suffix)
+        // as one padded with zeros.
+        if (rv) {
+#pragma unroll
+            for (int t = 0; t < TG; t++)
+                if (t < nt) close_segment(&cslot[(size_t)t * cstride], run[t], popen, 1u << t);
         }
     }
 
@@ -298,10 +394,13 @@ __global__ __launch_bounds__(WG) void k_rr(const F *__restrict__ x, int64_t nc, 
                                            int nt, int seg_len, const int2 *__restrict__ items, int64_t nitems,
                                            unsigned long long *__restrict__ counts) {
     const int lane = threadIdx.x & (WAVE - 1);
-    const int wave = threadIdx.x / WAVE;
+    const int wave = wave_id();
     __shared__ unsigned long long acc[TG];
     for (int e = threadIdx.x; e < TG; e += WG) acc[e] = 0;
     __syncthreads();
+    F bnd[TG];
+#pragma unroll
+    for (int t = 0; t < TG; t++) bnd[t] = to_vgpr(S.s[t]);
 
     for (int64_t item = (int64_t)blockIdx.x * WAVES + wave; item < nitems; item += (int64_t)gridDim.x * WAVES) {
         const int2 it = items[item];
@@ -330,28 +429,31 @@ __global__ __launch_bounds__(WG) void k_rr(const F *__restrict__ x, int64_t nc, 
             const bool full = (i0 + BLK <= j0) && (i0 + BLK <= ie) && (j0 + WAVE <= nc);
             if (full) {
 #pragma unroll
-                for (int t = 0; t < TG; t++) {
-#pragma unroll
-                    for (int u = 0; u < BLK; u++) c2[t] += (s[u] <= S.s[t]) ? 1u : 0u;
-                }
+                for (int t = 0; t < TG; t++)
+                    if (t < nt) c2[t] = count_le(s, bnd[t], c2[t]);
             } else {
 #pragma unroll
                 for (int t = 0; t < TG; t++) {
+                    if (t < nt) {
+                        const uint32_t w = decision_word(s, bnd[t]);
 #pragma unroll
-                    for (int u = 0; u < BLK; u++) {
-                        const int64_t i = i0 + u;
-                        const bool r = jv && (i < ie) && (s[u] <= S.s[t]);
-                        c2[t] += (r && i < j) ? 1u : 0u;
-                        c1[t] += (r && i == j) ? 1u : 0u;
+                        for (int u = 0; u < BLK; u++) {
+                            const int64_t i = i0 + u;
+                            const bool r = jv && (i < ie) && ((w >> (BLK - 1 - u)) & 1u);
+                            c2[t] += (r && i < j) ? 1u : 0u;
+                            c1[t] += (r && i == j) ? 1u : 0u;
+                        }
                     }
                 }
             }
         }
 #pragma unroll
         for (int t = 0; t < TG; t++) {
-            unsigned long long v = 2ull * c2[t] + c1[t];
-            for (int off = WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off);
-            if (lane == 0 && t < nt && v) atomicAdd(&acc[t], v);
+            if (t < nt) {
+                unsigned long long v = 2ull * c2[t] + c1[t];
+                for (int off = WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off);
+                if (lane == 0 && v) atomicAdd(&acc[t], v);
+            }
         }
     }
     __syncthreads();
