@@ -41,7 +41,7 @@ constexpr int WAVE = 64;
 constexpr int BLK = 32;            // steps per decision word
 constexpr int WG = 256;            // threads per workgroup (4 waves)
 constexpr int WAVES = WG / WAVE;
-constexpr int LCAP = 256;          // run lengths < LCAP use the LDS histogram
+constexpr int LCAP = 128;          // run lengths < LCAP use the LDS histogram
 constexpr int DET_WIN = 1024;      // LDS samples per wave for the k_det lane window
 
 template <typename F, int TG>
@@ -74,114 +74,143 @@ __device__ __forceinline__ void glob_add(unsigned long long *p, unsigned long lo
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// gh points at a histogram row of length hmax + 1; a run longer than the
-// sequence would be an engine bug, and is dropped rather than written out of
-// bounds (the parity tests would then fail on the missing count).
-__device__ __forceinline__ void emit(uint32_t len, uint32_t *lh, unsigned long long *gh, uint32_t hmax) {
+// A uniform value the compiler may not hoist out of a rare branch: keeps
+// per-threshold addresses from occupying scalar registers for the whole loop.
+__device__ __forceinline__ uint32_t opaque(uint32_t v) {
+    asm volatile("" : "+s"(v));
+    return v;
+}
+
+// Histogram row t starts at gh + t*(hmax+1); a run longer than the sequence
+// would be an engine bug, and is dropped rather than written out of bounds
+// (the parity tests would then fail on the missing count).
+__device__ __forceinline__ void emit(uint32_t len, uint32_t *lh, unsigned long long *gh, uint32_t t, uint32_t hmax) {
     if (len < (uint32_t)LCAP)
         lds_inc(&lh[len]);
     else if (len <= hmax)
-        glob_add(&gh[len], 1ull);
+        glob_add(&gh[(size_t)opaque(t) * (hmax + 1) + len], 1ull);
 }
 
-// ---------------------------------------------------------------- decision words
-// w = 2w + [s_u <= bound] for u = 0..31, i.e. bit 31-u holds step u.  One
-// v_cmp (writes a lane mask) and one v_addc (consumes it as carry-in) per
-// decision.  Four compares go to four SGPR pairs before the four adds read
-// them, so every mask is read at least four instructions after it is
-// written (gfx950 requires two wait states between a VALU mask write and a
-// VALU carry read).  NaN s never compares true, as in the reference.
+// ---------------------------------------------------------------- threshold ranks
+// All thresholds of a group are decided through one integer per pair, its
+// rank r = #{t : B_t < s} over the group's bounds sorted ascending: then
+// R_t = [s <= B_t] = [t >= r].  The rank comes from a bucket table indexed by
+// the high bits of s (exact: a bucket holds at most one bound, whose key is
+// stored with the bucket; a bucket with several bounds sends the wave to a
+// direct count).  Keys are the IEEE bit patterns, which order non-negative
+// values exactly; NaN maps to +inf, i.e. rank T, never recurrent.
+//
+// Ranks of 32 steps are packed one per byte (register q holds steps q, q+8,
+// q+16, q+24), and one subtraction per register yields four decisions:
+// byte 0x80 + t - r has its top bit set iff r <= t.  Shifting register q's
+// top bits down by 7-q puts step u at bit u of the decision word.
+constexpr int NB_MAX = 2048;   // bucket table entries (LDS)
+
+struct LutParams {
+    uint32_t base;   // high key bits of the first bucket
+    int sh;          // bucket width = 1 << sh (in high key bits)
+    int nb;          // number of buckets
+};
+
 template <typename F>
-__device__ __forceinline__ uint32_t decision_word(const F (&s)[BLK], F bound) {
+struct LutEntry;
+template <>
+struct LutEntry<float> {
+    uint32_t next;   // key of the first bound >= bucket start (0xFFFFFFFF if none)
+    uint32_t info;   // bits 0-7: bounds below the bucket start; bit 8: several bounds in the bucket
+};
+template <>
+struct LutEntry<double> {
+    unsigned long long next;
+    uint32_t info;
+    uint32_t pad;
+};
+
+__device__ __forceinline__ uint32_t skey(float s) { return min(__float_as_uint(s), 0x7F800000u); }
+__device__ __forceinline__ unsigned long long skey(double s) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(s);
+    return b > 0x7FF0000000000000ull ? 0x7FF0000000000000ull : b;
+}
+__device__ __forceinline__ uint32_t khi(uint32_t k) { return k; }
+__device__ __forceinline__ uint32_t khi(unsigned long long k) { return (uint32_t)(k >> 32); }
+
+template <typename F>
+__device__ __forceinline__ uint32_t rank_of(F s, const LutEntry<F> *lut, const LutParams &P, uint32_t &flags) {
+    const auto key = skey(s);
+    int b = (int)(khi(key) - P.base) >> P.sh;
+    b = b < 0 ? 0 : (b > P.nb - 1 ? P.nb - 1 : b);
+    const LutEntry<F> e = lut[b];
+    flags |= e.info;
+    return (e.info & 0xFFu) + (key > e.next ? 1u : 0u);
+}
+
+// Direct count against the sorted bounds (buckets holding several bounds).
+template <typename F, int TG>
+__device__ __forceinline__ uint32_t rank_direct(F s, const Bounds<F, TG> &S, int nt) {
+    const auto key = skey(s);
+    uint32_t r = 0;
+#pragma unroll
+    for (int t = 0; t < TG; t++)
+        if (t < nt) r += key > skey(S.s[t]) ? 1u : 0u;
+    return r;
+}
+
+template <typename F, int TG>
+__device__ __forceinline__ void pack_ranks(const F (&s)[BLK], uint32_t (&R)[8], const LutEntry<F> *lut,
+                                           const LutParams &P, const Bounds<F, TG> &S, int nt) {
+    uint32_t flags = 0;
+#pragma unroll
+    for (int q = 0; q < 8; q++) R[q] = 0;
+#pragma unroll
+    for (int u = 0; u < BLK; u++) R[u & 7] |= rank_of(s[u], lut, P, flags) << (8 * (u >> 3));
+    if (__any((flags & 0x100u) != 0)) {
+#pragma unroll
+        for (int q = 0; q < 8; q++) R[q] = 0;
+#pragma unroll
+        for (int u = 0; u < BLK; u++) R[u & 7] |= rank_direct(s[u], S, nt) << (8 * (u >> 3));
+    }
+}
+
+// Decision word of threshold t: bit u = [rank of step u <= t].
+__device__ __forceinline__ uint32_t swar_word(const uint32_t (&R)[8], uint32_t t) {
+    const uint32_t K = 0x80808080u + t * 0x01010101u;
     uint32_t w = 0;
 #pragma unroll
-    for (int u = 0; u < BLK; u += 4) {
-        uint64_t m0, m1, m2, m3, c;
-        if constexpr (sizeof(F) == 4) {
-            asm("v_cmp_ge_f32_e64 %[m0], %[b], %[s0]\n\t"
-                "v_cmp_ge_f32_e64 %[m1], %[b], %[s1]\n\t"
-                "v_cmp_ge_f32_e64 %[m2], %[b], %[s2]\n\t"
-                "v_cmp_ge_f32_e64 %[m3], %[b], %[s3]\n\t"
-                "v_addc_co_u32_e64 %[w], %[c], %[w], %[w], %[m0]\n\t"
-                "v_addc_co_u32_e64 %[w], %[c], %[w], %[w], %[m1]\n\t"
-                "v_addc_co_u32_e64 %[w], %[c], %[w], %[w], %[m2]\n\t"
-                "v_addc_co_u32_e64 %[w], %[c], %[w], %[w], %[m3]"
-                : [w] "+v"(w), [m0] "=&s"(m0), [m1] "=&s"(m1), [m2] "=&s"(m2), [m3] "=&s"(m3), [c] "=&s"(c)
-                : [b] "v"(bound), [s0] "v"(s[u]), [s1] "v"(s[u + 1]), [s2] "v"(s[u + 2]), [s3] "v"(s[u + 3]));
-        } else {
-            asm("v_cmp_ge_f64_e64 %[m0], %[b], %[s0]\n\t"
-                "v_cmp_ge_f64_e64 %[m1], %[b], %[s1]\n\t"
-                "v_cmp_ge_f64_e64 %[m2], %[b], %[s2]\n\t"
-                "v_cmp_ge_f64_e64 %[m3], %[b], %[s3]\n\t"
-                "v_addc_co_u32_e64 %[w], %[c], %[w], %[w], %[m0]\n\t"
-                "v_addc_co_u32_e64 %[w], %[c], %[w], %[w], %[m1]\n\t"
-                "v_addc_co_u32_e64 %[w], %[c], %[w], %[w], %[m2]\n\t"
-                "v_addc_co_u32_e64 %[w], %[c], %[w], %[w], %[m3]"
-                : [w] "+v"(w), [m0] "=&s"(m0), [m1] "=&s"(m1), [m2] "=&s"(m2), [m3] "=&s"(m3), [c] "=&s"(c)
-                : [b] "v"(bound), [s0] "v"(s[u]), [s1] "v"(s[u + 1]), [s2] "v"(s[u + 2]), [s3] "v"(s[u + 3]));
-        }
-    }
+    for (int q = 0; q < 8; q++) w |= ((K - R[q]) >> (7 - q)) & (0x01010101u << q);
     return w;
 }
 
-// Weighted count: cnt += [s_u <= bound] for u = 0..31 (same interleaving).
 template <typename F>
-__device__ __forceinline__ uint32_t count_le(const F (&s)[BLK], F bound, uint32_t cnt) {
-#pragma unroll
-    for (int u = 0; u < BLK; u += 4) {
-        uint64_t m0, m1, m2, m3, c;
-        if constexpr (sizeof(F) == 4) {
-            asm("v_cmp_ge_f32_e64 %[m0], %[b], %[s0]\n\t"
-                "v_cmp_ge_f32_e64 %[m1], %[b], %[s1]\n\t"
-                "v_cmp_ge_f32_e64 %[m2], %[b], %[s2]\n\t"
-                "v_cmp_ge_f32_e64 %[m3], %[b], %[s3]\n\t"
-                "v_addc_co_u32_e64 %[n], %[c], %[n], 0, %[m0]\n\t"
-                "v_addc_co_u32_e64 %[n], %[c], %[n], 0, %[m1]\n\t"
-                "v_addc_co_u32_e64 %[n], %[c], %[n], 0, %[m2]\n\t"
-                "v_addc_co_u32_e64 %[n], %[c], %[n], 0, %[m3]"
-                : [n] "+v"(cnt), [m0] "=&s"(m0), [m1] "=&s"(m1), [m2] "=&s"(m2), [m3] "=&s"(m3), [c] "=&s"(c)
-                : [b] "v"(bound), [s0] "v"(s[u]), [s1] "v"(s[u + 1]), [s2] "v"(s[u + 2]), [s3] "v"(s[u + 3]));
-        } else {
-            asm("v_cmp_ge_f64_e64 %[m0], %[b], %[s0]\n\t"
-                "v_cmp_ge_f64_e64 %[m1], %[b], %[s1]\n\t"
-                "v_cmp_ge_f64_e64 %[m2], %[b], %[s2]\n\t"
-                "v_cmp_ge_f64_e64 %[m3], %[b], %[s3]\n\t"
-                "v_addc_co_u32_e64 %[n], %[c], %[n], 0, %[m0]\n\t"
-                "v_addc_co_u32_e64 %[n], %[c], %[n], 0, %[m1]\n\t"
-                "v_addc_co_u32_e64 %[n], %[c], %[n], 0, %[m2]\n\t"
-                "v_addc_co_u32_e64 %[n], %[c], %[n], 0, %[m3]"
-                : [n] "+v"(cnt), [m0] "=&s"(m0), [m1] "=&s"(m1), [m2] "=&s"(m2), [m3] "=&s"(m3), [c] "=&s"(c)
-                : [b] "v"(bound), [s0] "v"(s[u]), [s1] "v"(s[u + 1]), [s2] "v"(s[u + 2]), [s3] "v"(s[u + 3]));
-        }
-    }
-    return cnt;
+__device__ __forceinline__ void load_lut(LutEntry<F> *dst, const LutEntry<F> *src, int nb) {
+    for (int e = threadIdx.x; e < nb; e += WG) dst[e] = src[e];
 }
 
-// Consume one 32-step word (bit 31 = oldest step).  run: open run entering
-// the word.  Bit tb of popen is set until the segment's first zero; the run
-// it closes is the segment's prefix run, written to *pre (the stitch joins
-// it with the previous segment) instead of being counted here.
-__device__ __forceinline__ void consume_word(uint32_t w, uint32_t &run, uint32_t &popen, uint32_t tb, uint32_t *pre,
-                                             uint32_t *lh, unsigned long long *gh, uint32_t hmax) {
+// Consume one 32-step word, bit u = step u (bit 0 oldest).  run: open run
+// entering the word.  Bit tb of popen is set until the segment's first zero;
+// the run it closes is the segment's prefix run, written to *pre (the stitch
+// joins it with the previous segment) instead of being counted here.
+__device__ __forceinline__ void consume_word(uint32_t w, uint32_t &run, uint32_t &popen, uint32_t t, uint2 *cslot,
+                                             size_t cstride, uint32_t *lh, unsigned long long *gh, uint32_t hmax) {
     if (w == 0xFFFFFFFFu) {
         run += BLK;
         return;
     }
-    const uint32_t lead = __clz(~w);             // ones above the highest zero
+    const uint32_t tb = 1u << t;
+    const uint32_t lead = __builtin_ctz(~w);     // ones from the oldest step up to the first zero
     const uint32_t first = run + lead;
     if (popen & tb) {
-        *pre = first;
+        cslot[(size_t)opaque(t) * cstride].x = first;
         popen &= ~tb;
     } else if (first) {
-        emit(first, lh, gh, hmax);
+        emit(first, lh, gh, t, hmax);
     }
-    const uint32_t tail = __builtin_ctz(~w);     // ones below the lowest zero
-    uint32_t mid = w & (0xFFFFFFFFu >> lead) & ~((1u << tail) - 1u);
+    const uint32_t tail = __clz(~w);             // ones from the newest step down to the last zero
+    uint32_t mid = w & (0xFFFFFFFFu << lead) & (0xFFFFFFFFu >> tail);
     while (mid) {
-        const uint32_t h = 31u - __clz(mid);
-        const uint32_t len = __clz(~(mid << (31u - h)));
-        emit(len, lh, gh, hmax);
-        mid &= ~(((1u << len) - 1u) << (h + 1u - len));
+        const uint32_t lo = __builtin_ctz(mid);
+        const uint32_t len = __builtin_ctz(~(mid >> lo));
+        emit(len, lh, gh, t, hmax);
+        mid &= ~(((1u << len) - 1u) << lo);
     }
     run = tail;
 }
@@ -195,18 +224,6 @@ __device__ __forceinline__ void close_segment(uint2 *slot, uint32_t run, uint32_
         slot->y = run;
 }
 
-// Copy a uniform value into a VGPR (keeps the threshold bounds out of the
-// scalar register file, which the row window and the lane masks need).
-template <typename F>
-__device__ __forceinline__ F to_vgpr(F v) {
-    F r;
-    if constexpr (sizeof(F) == 4)
-        asm volatile("v_mov_b32 %0, %1" : "=v"(r) : "s"(v));
-    else
-        asm volatile("v_mov_b64 %0, %1" : "=v"(r) : "s"(v));
-    return r;
-}
-
 __device__ __forceinline__ int wave_id() {
     return __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
 }
@@ -218,10 +235,13 @@ __device__ __forceinline__ int wave_id() {
 // comes through scalar loads; x[j + k*tau] is one vector load per k and block.
 template <typename F, int NORM, int TG>
 __global__ __launch_bounds__(WG) void k_lam(const F *__restrict__ x, int64_t nc, int tau, int m, Bounds<F, TG> S,
-                                            int nt, int seg_len, int nseg, unsigned long long *__restrict__ hist,
+                                            int nt, const LutEntry<F> *__restrict__ lut, LutParams P, int seg_len,
+                                            int nseg, unsigned long long *__restrict__ hist,
                                             uint2 *__restrict__ carry) {
     __shared__ uint32_t lh[TG * LCAP];
+    __shared__ LutEntry<F> lut_s[NB_MAX];
     for (int e = threadIdx.x; e < TG * LCAP; e += WG) lh[e] = 0;
+    load_lut(lut_s, lut, P.nb);
     __syncthreads();
 
     const int lane = threadIdx.x & (WAVE - 1);
@@ -229,9 +249,6 @@ __global__ __launch_bounds__(WG) void k_lam(const F *__restrict__ x, int64_t nc,
     const int64_t nbands = (nc + WAVE - 1) / WAVE;
     const int64_t nitems = nbands * nseg;
     const size_t cstride = (size_t)nc * nseg;   // carry stride between thresholds
-    F bnd[TG];
-#pragma unroll
-    for (int t = 0; t < TG; t++) bnd[t] = to_vgpr(S.s[t]);
 
     for (int64_t item = (int64_t)blockIdx.x * WAVES + wave; item < nitems; item += (int64_t)gridDim.x * WAVES) {
         const int64_t band = item / nseg;
@@ -261,14 +278,15 @@ __global__ __launch_bounds__(WG) void k_lam(const F *__restrict__ x, int64_t nc,
                 for (int u = 0; u < BLK; u++) s[u] = dim_step<NORM>(s[u], xr[u], xj);
                 xj = xjn;
             }
-            const uint32_t fill = nb == BLK ? 0u : ((1u << (BLK - nb)) - 1u);
+            uint32_t R[8];
+            pack_ranks(s, R, lut_s, P, S, nt);
+            const uint32_t fill = nb == BLK ? 0u : (0xFFFFFFFFu << nb);   // rows past the segment
 #pragma unroll
             for (int t = 0; t < TG; t++) {
                 if (t < nt) {
-                    uint32_t w = decision_word(s, bnd[t]);
+                    uint32_t w = swar_word(R, t);
                     w = jv ? (w | fill) : 0u;
-                    consume_word(w, run[t], popen, 1u << t, &cslot[(size_t)t * cstride].x, lh + t * LCAP,
-                                 hist + (size_t)t * (nc + 1), (uint32_t)nc);
+                    consume_word(w, run[t], popen, t, cslot, cstride, lh + t * LCAP, hist, (uint32_t)nc);
                     if (fill && jv) run[t] -= (uint32_t)(BLK - nb);
                 }
             }
@@ -276,7 +294,7 @@ __global__ __launch_bounds__(WG) void k_lam(const F *__restrict__ x, int64_t nc,
         if (jv) {
 #pragma unroll
             for (int t = 0; t < TG; t++)
-                if (t < nt) close_segment(&cslot[(size_t)t * cstride], run[t], popen, 1u << t);
+                if (t < nt) close_segment(&cslot[(size_t)opaque(t) * cstride], run[t], popen, 1u << t);
         }
     }
 
@@ -296,12 +314,14 @@ __global__ __launch_bounds__(WG) void k_lam(const F *__restrict__ x, int64_t nc,
 // in LDS once per 32-step block and dimension chunk.
 template <typename F, int NORM, int TG>
 __global__ __launch_bounds__(WG) void k_det(const F *__restrict__ x, int64_t nc, int tau, int m, Bounds<F, TG> S,
-                                            int nt, int seg_len, int nseg, const int2 *__restrict__ items,
-                                            int64_t nitems, unsigned long long *__restrict__ hist,
-                                            uint2 *__restrict__ carry) {
+                                            int nt, const LutEntry<F> *__restrict__ lut, LutParams P, int seg_len,
+                                            int nseg, const int2 *__restrict__ items, int64_t nitems,
+                                            unsigned long long *__restrict__ hist, uint2 *__restrict__ carry) {
     __shared__ uint32_t lh[TG * LCAP];
+    __shared__ LutEntry<F> lut_s[NB_MAX];
     __shared__ F win[WAVES][DET_WIN];
     for (int e = threadIdx.x; e < TG * LCAP; e += WG) lh[e] = 0;
+    load_lut(lut_s, lut, P.nb);
     __syncthreads();
 
     const int lane = threadIdx.x & (WAVE - 1);
@@ -311,9 +331,6 @@ __global__ __launch_bounds__(WG) void k_det(const F *__restrict__ x, int64_t nc,
     const int kc_max = tau > 0 ? (DET_WIN - (BLK + WAVE - 1)) / tau + 1 : m;
     const int kc = kc_max < m ? kc_max : m;
     const size_t cstride = (size_t)(nc - 1) * nseg;
-    F bnd[TG];
-#pragma unroll
-    for (int t = 0; t < TG; t++) bnd[t] = to_vgpr(S.s[t]);
 
     for (int64_t item = (int64_t)blockIdx.x * WAVES + wave; item < nitems; item += (int64_t)gridDim.x * WAVES) {
         const int2 it = items[item];
@@ -355,16 +372,17 @@ __global__ __launch_bounds__(WG) void k_det(const F *__restrict__ x, int64_t nc,
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             }
-            // bit (31-u) valid iff s0+u < dlen; invalid steps decide 0
+            uint32_t R[8];
+            pack_ranks(s, R, lut_s, P, S, nt);
+            // bit u valid iff s0+u < dlen; invalid steps decide 0
             const int64_t nv64 = dlen - s0;
             const int nvalid = nv64 <= 0 ? 0 : (nv64 >= BLK ? BLK : (int)nv64);
-            const uint32_t vmask = nvalid == 0 ? 0u : (0xFFFFFFFFu << (BLK - nvalid));
+            const uint32_t vmask = nvalid >= BLK ? 0xFFFFFFFFu : ((1u << nvalid) - 1u);
 #pragma unroll
             for (int t = 0; t < TG; t++) {
                 if (t < nt) {
-                    const uint32_t w = decision_word(s, bnd[t]) & vmask;
-                    consume_word(w, run[t], popen, 1u << t, &cslot[(size_t)t * cstride].x, lh + t * LCAP,
-                                 hist + (size_t)t * (nc + 1), (uint32_t)nc);
+                    const uint32_t w = swar_word(R, t) & vmask;
+                    consume_word(w, run[t], popen, t, cslot, cstride, lh + t * LCAP, hist, (uint32_t)nc);
                 }
             }
         }
@@ -374,7 +392,7 @@ __global__ __launch_bounds__(WG) void k_det(const F *__restrict__ x, int64_t nc,
         if (rv) {
 #pragma unroll
             for (int t = 0; t < TG; t++)
-                if (t < nt) close_segment(&cslot[(size_t)t * cstride], run[t], popen, 1u << t);
+                if (t < nt) close_segment(&cslot[(size_t)opaque(t) * cstride], run[t], popen, 1u << t);
         }
     }
 
@@ -391,16 +409,16 @@ __global__ __launch_bounds__(WG) void k_det(const F *__restrict__ x, int64_t nc,
 // of the band.  Pair weight: 2 if i < j, 1 if i == j, 0 otherwise.
 template <typename F, int NORM, int TG>
 __global__ __launch_bounds__(WG) void k_rr(const F *__restrict__ x, int64_t nc, int tau, int m, Bounds<F, TG> S,
-                                           int nt, int seg_len, const int2 *__restrict__ items, int64_t nitems,
+                                           int nt, const LutEntry<F> *__restrict__ lut, LutParams P, int seg_len,
+                                           const int2 *__restrict__ items, int64_t nitems,
                                            unsigned long long *__restrict__ counts) {
+    __shared__ unsigned long long acc[TG];
+    __shared__ LutEntry<F> lut_s[NB_MAX];
+    for (int e = threadIdx.x; e < TG; e += WG) acc[e] = 0;
+    load_lut(lut_s, lut, P.nb);
+    __syncthreads();
     const int lane = threadIdx.x & (WAVE - 1);
     const int wave = wave_id();
-    __shared__ unsigned long long acc[TG];
-    for (int e = threadIdx.x; e < TG; e += WG) acc[e] = 0;
-    __syncthreads();
-    F bnd[TG];
-#pragma unroll
-    for (int t = 0; t < TG; t++) bnd[t] = to_vgpr(S.s[t]);
 
     for (int64_t item = (int64_t)blockIdx.x * WAVES + wave; item < nitems; item += (int64_t)gridDim.x * WAVES) {
         const int2 it = items[item];
@@ -419,31 +437,28 @@ __global__ __launch_bounds__(WG) void k_rr(const F *__restrict__ x, int64_t nc, 
             F s[BLK];
 #pragma unroll
             for (int u = 0; u < BLK; u++) s[u] = F(0);
+            F xj = x[j];
             for (int k = 0; k < m; k++) {
-                const F xj = x[j + (int64_t)k * tau];
+                const F xjn = x[j + (int64_t)(k + 1 < m ? k + 1 : k) * tau];
                 const F *xr = x + i0 + (int64_t)k * tau;
 #pragma unroll
                 for (int u = 0; u < BLK; u++) s[u] = dim_step<NORM>(s[u], xr[u], xj);
+                xj = xjn;
             }
-            // whole block strictly above every column of a complete band
-            const bool full = (i0 + BLK <= j0) && (i0 + BLK <= ie) && (j0 + WAVE <= nc);
-            if (full) {
+            uint32_t R[8];
+            pack_ranks(s, R, lut_s, P, S, nt);
+            // rows i0+u with u < nrow are in the segment; weight 2 for i < j, 1 for i == j
+            const int64_t nr64 = ie - i0;
+            const uint32_t mrow = nr64 >= BLK ? 0xFFFFFFFFu : ((1u << nr64) - 1u);
+            const int64_t du = j - i0;   // step index of the diagonal element
+            const uint32_t mup = !jv ? 0u : (du >= BLK ? 0xFFFFFFFFu : (du <= 0 ? 0u : ((1u << du) - 1u)));
+            const uint32_t mdg = (jv && du >= 0 && du < BLK) ? (1u << du) : 0u;
 #pragma unroll
-                for (int t = 0; t < TG; t++)
-                    if (t < nt) c2[t] = count_le(s, bnd[t], c2[t]);
-            } else {
-#pragma unroll
-                for (int t = 0; t < TG; t++) {
-                    if (t < nt) {
-                        const uint32_t w = decision_word(s, bnd[t]);
-#pragma unroll
-                        for (int u = 0; u < BLK; u++) {
-                            const int64_t i = i0 + u;
-                            const bool r = jv && (i < ie) && ((w >> (BLK - 1 - u)) & 1u);
-                            c2[t] += (r && i < j) ? 1u : 0u;
-                            c1[t] += (r && i == j) ? 1u : 0u;
-                        }
-                    }
+            for (int t = 0; t < TG; t++) {
+                if (t < nt) {
+                    const uint32_t w = swar_word(R, t) & mrow;
+                    c2[t] += __popc(w & mup);
+                    c1[t] += __popc(w & mdg);
                 }
             }
         }

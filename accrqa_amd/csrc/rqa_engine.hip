@@ -46,7 +46,7 @@ struct Ctx {
     int device = -1;
     int ncu = 0;
     hipStream_t stream = nullptr;
-    DevBuf x, hist, carry, metric, scan, items_rr, items_det, counts, perm, lmins, out_rr, out_det, out_lam;
+    DevBuf x, hist, carry, metric, scan, items_rr, items_det, counts, perm, lmins, lut, out_rr, out_det, out_lam;
     int64_t items_nc = -1;     // (nc, seg_len) the item tables were built for
     int items_L = -1;
     int64_t n_items_rr = 0, n_items_det = 0;
@@ -98,7 +98,7 @@ int init_ctx() {
             int cur = dev;
             (void)hipSetDevice(g_ctx.device);
             DevBuf *all[] = {&g_ctx.x, &g_ctx.hist, &g_ctx.carry, &g_ctx.metric, &g_ctx.scan, &g_ctx.items_rr,
-                             &g_ctx.items_det, &g_ctx.counts, &g_ctx.perm, &g_ctx.lmins, &g_ctx.out_rr,
+                             &g_ctx.items_det, &g_ctx.counts, &g_ctx.perm, &g_ctx.lmins, &g_ctx.lut, &g_ctx.out_rr,
                              &g_ctx.out_det, &g_ctx.out_lam};
             g_ctx.items_nc = -1;
             for (DevBuf *b : all) {
@@ -151,31 +151,60 @@ F bound_of(F eps, int dist) {
     return eps;   // NaN compares false, negative eps never recurs
 }
 
-// Split T sorted thresholds into groups of TG in {4, 8, 16, 32}, minimising
-// (per-pair distance work + 2 ops per threshold slot) summed over groups.
-std::vector<int> plan_groups(int T, int m, int dist) {
-    const int cand[4] = {4, 8, 16, 32};
-    const double D = dist == 1 ? 3.0 * m - 1.0 : 2.0 * m;
-    std::vector<double> best(T + 1, 1e300);
-    std::vector<int> pick(T + 1, 0);
-    best[0] = 0;
-    for (int n = 1; n <= T; n++)
-        for (int g : cand) {
-            int take = std::min(g, n);
-            double c = best[n - take] + D + 2.0 * g;
-            if (c < best[n]) {
-                best[n] = c;
-                pick[n] = g;
-            }
+// A threshold whose bound is negative or NaN never recurs; it skips the pair
+// kernels (its histogram is empty).  -0 behaves as +0.
+template <typename F>
+bool never_recurs(F b) {
+    return std::isnan(b) || b < F(0);
+}
+
+int tg_for(int nt) { return nt <= 4 ? 4 : (nt <= 8 ? 8 : (nt <= 16 ? 16 : 32)); }
+
+inline uint32_t hkey(float v) {
+    uint32_t b;
+    std::memcpy(&b, &v, 4);
+    return b;
+}
+inline unsigned long long hkey(double v) {
+    unsigned long long b;
+    std::memcpy(&b, &v, 8);
+    return b;
+}
+inline uint32_t hhi(uint32_t k) { return k; }
+inline uint32_t hhi(unsigned long long k) { return (uint32_t)(k >> 32); }
+inline unsigned long long key_at(float, unsigned long long hi) { return hi; }
+inline unsigned long long key_at(double, unsigned long long hi) {
+    return hi >= (1ull << 32) ? ~0ull : (hi << 32);
+}
+
+// Bucket table for the rank of a pair (see rank_of in rqa_device.hpp).  b:
+// the group's bounds, ascending, non-negative and finite.  Bucket bk covers
+// high key bits [base + bk<<sh, base + (bk+1)<<sh); the last bucket lies past
+// the largest bound, so keys clamped into it get rank nt.
+template <typename F>
+void build_lut(const F *b, int nt, rqa::LutEntry<F> *out, rqa::LutParams &P) {
+    std::vector<unsigned long long> key(nt);
+    for (int t = 0; t < nt; t++) key[t] = hkey(b[t]);
+    const uint32_t base = hhi(decltype(hkey(F(0)))(key[0]));
+    const uint32_t range = hhi(decltype(hkey(F(0)))(key[nt - 1])) - base;
+    int sh = 0;
+    while ((range >> sh) > (uint32_t)(rqa::NB_MAX - 2)) sh++;
+    const int nb = (int)(range >> sh) + 2;
+    for (int bk = 0; bk < nb; bk++) {
+        const unsigned long long s_hi = (unsigned long long)base + ((unsigned long long)bk << sh);
+        const unsigned long long start = key_at(F(0), s_hi);
+        const unsigned long long end = key_at(F(0), s_hi + (1ull << sh));
+        int r_lo = 0, inside = 0;
+        for (int t = 0; t < nt; t++) {
+            if (key[t] < start) r_lo++;
+            else if (key[t] < end) inside++;
         }
-    std::vector<int> sizes;
-    for (int n = T; n > 0;) {
-        int g = pick[n];
-        int take = std::min(g, n);
-        sizes.push_back(g);
-        n -= take;
+        out[bk].next = r_lo < nt ? (decltype(out[bk].next))key[r_lo] : (decltype(out[bk].next))~0ull;
+        out[bk].info = (uint32_t)r_lo | (inside >= 2 ? 0x100u : 0u);
     }
-    return sizes;   // TG per group; the last groups may be padded
+    P.base = base;
+    P.sh = sh;
+    P.nb = nb;
 }
 
 // ---------------------------------------------------------------- launches
@@ -309,19 +338,18 @@ int build_items(int64_t nc, hipStream_t st) {
     return E_OK;
 }
 
+// One group of up to 32 possibly-recurrent thresholds (sorted bounds).
 template <typename F, int NORM, int TG>
-int launch_group(const F *dx, int64_t nc, int tau, int m, const F *bounds, int nt, bool want_rr, bool want_det,
-                 bool want_lam, const int *d_lmins, int nl, const int *d_perm, int T, F *d_rr, F *d_det, F *d_lam,
-                 hipStream_t st) {
+int launch_group(const F *dx, int64_t nc, int tau, int m, const F *bounds, int nt, const rqa::LutEntry<F> *d_lut,
+                 const rqa::LutParams &P, bool want_rr, bool want_det, bool want_lam, const int *d_lmins, int nl,
+                 const int *d_perm, int T, F *d_rr, F *d_det, F *d_lam, hipStream_t st) {
     Ctx &c = g_ctx;
     rqa::Bounds<F, TG> B;
-    for (int t = 0; t < TG; t++) B.s[t] = t < nt ? bounds[t] : F(-1);
+    for (int t = 0; t < TG; t++) B.s[t] = t < nt ? bounds[t] : bounds[nt - 1];
     const int L = seg_len_for(nc);
     const int nseg = (int)((nc + L - 1) / L);
     const size_t hs = (size_t)nc + 1;
-
     int e;
-    if ((e = ensure(c.counts, 32 * sizeof(unsigned long long)))) return e;
     unsigned long long *counts = (unsigned long long *)c.counts.p;
 
     if (want_rr && !want_lam) {
@@ -329,7 +357,7 @@ int launch_group(const F *dx, int64_t nc, int tau, int m, const F *bounds, int n
         const int64_t ni = c.n_items_rr;
         prof_begin("k_rr", st);
         hipLaunchKernelGGL((rqa::k_rr<F, NORM, TG>), dim3(grid_for(ni, c.ncu)), dim3(rqa::WG), 0, st, dx, nc, tau, m,
-                           B, nt, L, (const int2 *)c.items_rr.p, ni, counts);
+                           B, nt, d_lut, P, L, (const int2 *)c.items_rr.p, ni, counts);
         prof_end(st);
         HIP_TRY(hipGetLastError());
     }
@@ -348,7 +376,7 @@ int launch_group(const F *dx, int64_t nc, int tau, int m, const F *bounds, int n
         const int64_t ni = ((nc + rqa::WAVE - 1) / rqa::WAVE) * nseg;
         prof_begin("k_lam", st);
         hipLaunchKernelGGL((rqa::k_lam<F, NORM, TG>), dim3(grid_for(ni, c.ncu)), dim3(rqa::WG), 0, st, dx, nc, tau,
-                           m, B, nt, L, nseg, hist, carry);
+                           m, B, nt, d_lut, P, L, nseg, hist, carry);
         prof_end(st);
         HIP_TRY(hipGetLastError());
         const int64_t nthr = (int64_t)nt * nc;
@@ -377,7 +405,7 @@ int launch_group(const F *dx, int64_t nc, int tau, int m, const F *bounds, int n
             const int64_t ni = c.n_items_det;
             prof_begin("k_det", st);
             hipLaunchKernelGGL((rqa::k_det<F, NORM, TG>), dim3(grid_for(ni, c.ncu)), dim3(rqa::WG), 0, st, dx, nc,
-                               tau, m, B, nt, L, nseg, (const int2 *)c.items_det.p, ni, hist, carry);
+                               tau, m, B, nt, d_lut, P, L, nseg, (const int2 *)c.items_det.p, ni, hist, carry);
             prof_end(st);
             HIP_TRY(hipGetLastError());
             const int64_t nthr = (int64_t)nt * (nc - 1);
@@ -397,22 +425,53 @@ int launch_group(const F *dx, int64_t nc, int tau, int m, const F *bounds, int n
     return E_OK;
 }
 
+// Thresholds that never recur: empty histograms, measures straight from finalize.
+template <typename F>
+int launch_never(int64_t nc, int nt, bool want_rr, bool want_det, bool want_lam, const int *d_lmins, int nl,
+                 const int *d_perm, int T, F *d_rr, F *d_det, F *d_lam, hipStream_t st) {
+    Ctx &c = g_ctx;
+    const size_t hs = (size_t)nc + 1;
+    int e;
+    if ((e = ensure(c.hist, (size_t)nt * hs * 8))) return e;
+    if ((e = ensure(c.metric, (size_t)nt * hs * 8))) return e;
+    if ((e = ensure(c.scan, (size_t)nt * hs * 8))) return e;
+    unsigned long long *hist = (unsigned long long *)c.hist.p;
+    unsigned long long *counts = (unsigned long long *)c.counts.p;
+    if (want_rr) {
+        fill_zero(counts, (size_t)nt, st);
+        hipLaunchKernelGGL((rqa::k_rr_out<F>), dim3((nt + 63) / 64), dim3(64), 0, st,
+                           (const unsigned long long *)counts, nt, nc, d_perm, d_rr);
+        HIP_TRY(hipGetLastError());
+    }
+    for (int kind = 0; kind < 2; kind++) {
+        const bool det = kind == 1;
+        if (det ? !want_det : !want_lam) continue;
+        fill_zero(hist, (size_t)nt * hs, st);
+        hipLaunchKernelGGL((rqa::k_finalize<F>), dim3(nt), dim3(rqa::FIN), 0, st, hist, nc, nt, det ? 1 : 0,
+                           d_lmins, nl, (unsigned long long *)c.metric.p, (unsigned long long *)c.scan.p,
+                           det ? d_det : d_lam, det ? 1 : 0, d_perm, T, (unsigned long long *)nullptr);
+        HIP_TRY(hipGetLastError());
+    }
+    return E_OK;
+}
+
 template <typename F, int NORM>
-int launch_tg(int TG, const F *dx, int64_t nc, int tau, int m, const F *bounds, int nt, bool rr, bool det, bool lam,
-              const int *d_lmins, int nl, const int *d_perm, int T, F *d_rr, F *d_det, F *d_lam, hipStream_t st) {
+int launch_tg(int TG, const F *dx, int64_t nc, int tau, int m, const F *bounds, int nt, const rqa::LutEntry<F> *lut,
+              const rqa::LutParams &P, bool rr, bool det, bool lam, const int *d_lmins, int nl, const int *d_perm,
+              int T, F *d_rr, F *d_det, F *d_lam, hipStream_t st) {
     switch (TG) {
         case 4:
-            return launch_group<F, NORM, 4>(dx, nc, tau, m, bounds, nt, rr, det, lam, d_lmins, nl, d_perm, T, d_rr,
-                                            d_det, d_lam, st);
+            return launch_group<F, NORM, 4>(dx, nc, tau, m, bounds, nt, lut, P, rr, det, lam, d_lmins, nl, d_perm, T,
+                                            d_rr, d_det, d_lam, st);
         case 8:
-            return launch_group<F, NORM, 8>(dx, nc, tau, m, bounds, nt, rr, det, lam, d_lmins, nl, d_perm, T, d_rr,
-                                            d_det, d_lam, st);
+            return launch_group<F, NORM, 8>(dx, nc, tau, m, bounds, nt, lut, P, rr, det, lam, d_lmins, nl, d_perm, T,
+                                            d_rr, d_det, d_lam, st);
         case 16:
-            return launch_group<F, NORM, 16>(dx, nc, tau, m, bounds, nt, rr, det, lam, d_lmins, nl, d_perm, T,
-                                             d_rr, d_det, d_lam, st);
+            return launch_group<F, NORM, 16>(dx, nc, tau, m, bounds, nt, lut, P, rr, det, lam, d_lmins, nl, d_perm,
+                                             T, d_rr, d_det, d_lam, st);
         default:
-            return launch_group<F, NORM, 32>(dx, nc, tau, m, bounds, nt, rr, det, lam, d_lmins, nl, d_perm, T,
-                                             d_rr, d_det, d_lam, st);
+            return launch_group<F, NORM, 32>(dx, nc, tau, m, bounds, nt, lut, P, rr, det, lam, d_lmins, nl, d_perm,
+                                             T, d_rr, d_det, d_lam, st);
     }
 }
 
@@ -422,41 +481,62 @@ int run_block(const F *dx, size_t n, int tau, int emb, const F *thr, int T, cons
               F *d_rr, F *d_det, F *d_lam, hipStream_t st) {
     Ctx &c = g_ctx;
     const int64_t nc = (int64_t)n - (int64_t)(emb - 1) * tau;
-    // bounds, sorted ascending; perm maps sorted position -> caller's index
+    // bounds; recurring thresholds sorted ascending, then the never-recurring
+    // ones; perm maps that order back to the caller's threshold index.
     std::vector<F> bnd(T);
-    std::vector<int> order(T);
+    std::vector<int> live, dead;
     for (int t = 0; t < T; t++) {
-        bnd[t] = bound_of(thr[t], dist);
-        order[t] = t;
+        F b = bound_of(thr[t], dist);
+        if (never_recurs(b)) {
+            dead.push_back(t);
+        } else {
+            bnd[t] = b == F(0) ? F(0) : b;   // -0 -> +0
+            live.push_back(t);
+        }
     }
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-        F x = bnd[a], y = bnd[b];
-        if (std::isnan(x)) return false;
-        if (std::isnan(y)) return true;
-        return x < y;
-    });
+    std::stable_sort(live.begin(), live.end(), [&](int a, int b) { return bnd[a] < bnd[b]; });
+    std::vector<int> order(live);
+    order.insert(order.end(), dead.begin(), dead.end());
+    const int ngroups = ((int)live.size() + 31) / 32;
+    std::vector<rqa::LutEntry<F>> luts((size_t)std::max(ngroups, 1) * rqa::NB_MAX);
+    std::vector<rqa::LutParams> lps(std::max(ngroups, 1));
+    std::vector<F> sorted(live.size());
+    for (size_t t = 0; t < live.size(); t++) sorted[t] = bnd[live[t]];
+    for (int g = 0; g < ngroups; g++) {
+        const int g0 = g * 32, nt = std::min(32, (int)live.size() - g0);
+        build_lut(sorted.data() + g0, nt, luts.data() + (size_t)g * rqa::NB_MAX, lps[g]);
+    }
     int e;
     if ((e = ensure(c.perm, (size_t)T * sizeof(int)))) return e;
     if ((e = ensure(c.lmins, (size_t)std::max(nl, 1) * sizeof(int)))) return e;
+    if ((e = ensure(c.lut, luts.size() * sizeof(rqa::LutEntry<F>)))) return e;
+    if ((e = ensure(c.counts, 64 * sizeof(unsigned long long)))) return e;
     HIP_TRY(hipMemcpyAsync(c.perm.p, order.data(), (size_t)T * sizeof(int), hipMemcpyHostToDevice, st));
     if (nl > 0) HIP_TRY(hipMemcpyAsync(c.lmins.p, lmins, (size_t)nl * sizeof(int), hipMemcpyHostToDevice, st));
-
+    if (ngroups > 0)
+        HIP_TRY(hipMemcpyAsync(c.lut.p, luts.data(), (size_t)ngroups * rqa::NB_MAX * sizeof(rqa::LutEntry<F>),
+                               hipMemcpyHostToDevice, st));
     if ((e = build_items(nc, st))) return e;
-    std::vector<int> groups = plan_groups(T, emb, dist);
-    int t0 = 0;
-    for (int TG : groups) {
-        const int nt = std::min(TG, T - t0);
-        std::vector<F> gb(nt);
-        for (int t = 0; t < nt; t++) gb[t] = bnd[order[t0 + t]];
-        const int *gperm = (const int *)c.perm.p + t0;
+
+    for (int g = 0; g < ngroups; g++) {
+        const int g0 = g * 32, nt = std::min(32, (int)live.size() - g0);
+        const int *gperm = (const int *)c.perm.p + g0;
+        const rqa::LutEntry<F> *glut = (const rqa::LutEntry<F> *)c.lut.p + (size_t)g * rqa::NB_MAX;
         if (dist == 1)
-            e = launch_tg<F, 1>(TG, dx, nc, tau, emb, gb.data(), nt, d_rr != nullptr, d_det != nullptr,
-                                d_lam != nullptr, (const int *)c.lmins.p, nl, gperm, T, d_rr, d_det, d_lam, st);
+            e = launch_tg<F, 1>(tg_for(nt), dx, nc, tau, emb, sorted.data() + g0, nt, glut, lps[g], d_rr != nullptr,
+                                d_det != nullptr, d_lam != nullptr, (const int *)c.lmins.p, nl, gperm, T, d_rr, d_det,
+                                d_lam, st);
         else
-            e = launch_tg<F, 2>(TG, dx, nc, tau, emb, gb.data(), nt, d_rr != nullptr, d_det != nullptr,
-                                d_lam != nullptr, (const int *)c.lmins.p, nl, gperm, T, d_rr, d_det, d_lam, st);
+            e = launch_tg<F, 2>(tg_for(nt), dx, nc, tau, emb, sorted.data() + g0, nt, glut, lps[g], d_rr != nullptr,
+                                d_det != nullptr, d_lam != nullptr, (const int *)c.lmins.p, nl, gperm, T, d_rr, d_det,
+                                d_lam, st);
         if (e) return e;
-        t0 += nt;
+    }
+    if (!dead.empty()) {
+        e = launch_never<F>(nc, (int)dead.size(), d_rr != nullptr, d_det != nullptr, d_lam != nullptr,
+                            (const int *)c.lmins.p, nl, (const int *)c.perm.p + live.size(), T, d_rr, d_det, d_lam,
+                            st);
+        if (e) return e;
     }
     prof_collect(st);
     return E_OK;

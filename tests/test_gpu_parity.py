@@ -270,3 +270,23 @@ def test_device_entry_point_matches_host(amd, oracle):
     assert_measures(d_det.cpu().numpy(), det_o, np.float32, "device DET")
     lam_o, _ = oracle.LAM(x, [2], [4], lm, thr, 1)
     assert_measures(d_lam.cpu().numpy(), lam_o, np.float32, "device LAM")
+
+
+def test_threshold_sets_for_rank_table(amd, oracle):
+    """Threshold lists that stress the bucket table of the rank computation:
+    exact duplicates, neighbours one ulp apart (several bounds in one bucket),
+    a span of 60 decades, never-recurring values mixed in, and more than 32
+    thresholds (several groups)."""
+    rng = np.random.default_rng(57)
+    for dt in (np.float32, np.float64):
+        x = np.cumsum(rng.standard_normal(1800)).astype(dt)
+        x = ((x - x.mean()) / x.std()).astype(dt)
+        a = dt(0.3)
+        near = [a, np.nextafter(a, dt(1)), np.nextafter(np.nextafter(a, dt(1)), dt(1)), a, dt(0.3000001)]
+        wide = [dt(1e-30), dt(1e-10), dt(1e-3), dt(0.05), dt(2.0), dt(1e10), dt(1e30)]
+        dead = [dt(-2.0), dt(np.nan), dt(-0.0)]
+        many = list(np.linspace(0.01, 1.5, 41).astype(dt))
+        for name, thrs in (("near", near), ("wide", wide), ("mixed", near + dead + wide), ("many", many)):
+            for dist in (1, 2):
+                say("rank table", dt, name, dist)
+                check_case(amd, oracle, x, [1, 3], [2], [2], thrs, dist, f"rank {name} {dt} {dist}")
