@@ -16,7 +16,8 @@ import os
 
 import numpy as np
 
-__all__ = ["RR", "DET", "LAM", "histograms", "rqa_device", "AccrqaError", "library_path", "load", "PLT_AMD_GPU",
+__all__ = ["RR", "DET", "LAM", "histograms", "rqa_device", "set_form", "AccrqaError", "library_path", "load",
+           "PLT_AMD_GPU", "FORM_AUTO", "FORM_EXACT", "FORM_MFMA",
            "DST_EUCLIDEAN", "DST_MAXIMAL"]
 
 PLT_AMD_GPU = 2048
@@ -85,10 +86,24 @@ def load():
         f = getattr(L, f"accrqa_amd_rqa_device_{sfx}")
         f.argtypes = [P, SZ, I, I, P, I, P, I, I, P, P, P, P]
         f.restype = I
+    L.accrqa_amd_set_form.argtypes = [I]
+    L.accrqa_amd_set_form.restype = I
     L.accrqa_amd_last_error.restype = C.c_char_p
     L.accrqa_amd_version.restype = C.c_char_p
     _lib = L
     return L
+
+
+FORM_AUTO, FORM_EXACT, FORM_MFMA = 0, 1, 2
+
+
+def set_form(form):
+    """Euclidean distance form: FORM_AUTO (MFMA Gram + exact recheck for fp32
+    input with m >= 8), FORM_EXACT (exact VALU sums only) or FORM_MFMA (MFMA
+    form wherever it applies).  Results are identical; only speed differs."""
+    e = load().accrqa_amd_set_form(int(form))
+    if e:
+        raise AccrqaError(e, f"form {form}")
 
 
 def _dist(d):

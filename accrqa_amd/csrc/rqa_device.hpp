@@ -476,6 +476,551 @@ __global__ __launch_bounds__(WG) void k_rr(const F *__restrict__ x, int64_t nc, 
         if (acc[e]) glob_add(&counts[e], acc[e]);
 }
 
+// ================================================================ MFMA form
+// Euclidean norm, fp32 input.  The squared distance is approximated as
+//   s~ = (n_i + n_j) - 2 g_ij,   g = x_i . x_j on MFMA (v_mfma_f32_32x32x2f32),
+// n_i = |x_i|^2 (accumulated in fp64, rounded to fp32).  For the reference's
+// exact-order fp32 sum s:
+//   |s - s~| <= (4m + 16) * 2^-24 * (n_i + n_j) + 1e-30 =: E_ij
+// (exact-order sum: gamma_{m+1} * 2(n_i+n_j); rounded norms 2u; the fp32 dot
+// product gamma_m (n_i+n_j)/2 doubled; the final add and fma 3u; 10u spare).
+// A byte table over the bits of s~ gives the rank r = #{t : B_t < s} exactly
+// for every bucket that has no bound within E_max = max E_ij of its value
+// range; pairs in the other buckets are re-decided in exact order on VALU.
+// Inputs whose norms are not finite or exceed 1e30 flag every bucket, so the
+// whole matrix is then decided exactly.
+constexpr int NBX = 32768;   // byte buckets of the MFMA rank table
+constexpr int MXW = 128;     // LDS operand window: 32 + (m-1)*tau samples at most
+
+struct MxParams {
+    uint32_t base;   // key bits of bucket 0 (half the smallest bound)
+    int sh;          // bucket width in key bits
+    int nb;          // buckets used (<= NBX)
+    float cE;        // (4m + 16) * 2^-24
+};
+
+// n_i = sum_k x[i + k tau]^2 (fp64, rounded to fp32); entries nc .. nc+127 = 0.
+// nmax_bits: atomic max of the float bits (non-negative floats order as uints;
+// NaN bits are above +inf, so a NaN norm forces the all-exact fallback).
+__global__ void k_norms(const float *__restrict__ x, int64_t nc, int tau, int m, float *__restrict__ nrm,
+                        unsigned int *__restrict__ nmax_bits) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nc + 2 * WAVE) return;
+    float v = 0.f;
+    if (i < nc) {
+        double a = 0.0;
+        for (int k = 0; k < m; k++) {
+            const double xv = (double)x[i + (int64_t)k * tau];
+            a += xv * xv;
+        }
+        v = (float)a;
+        const float mag = fabsf(v);
+        atomicMax(nmax_bits, __float_as_uint(mag > 1e30f || mag != mag ? __int_as_float(0x7F800000) : mag));
+    }
+    nrm[i] = v;
+}
+
+// Byte table.  Bucket without any bound within E_max of its value range:
+// bits 0-5 = rank (number of bounds below it), bit 7 clear.  Otherwise bit 7
+// set and bits 0-5 = index of the one bound within reach (bit 6 set if there
+// are several): such pairs are checked against that bound with their own
+// E_ij, and only the truly ambiguous ones are re-decided in exact order.
+template <int TG>
+__global__ void k_mx_lut(Bounds<float, TG> S, int nt, MxParams P, const unsigned int *__restrict__ nmax_bits,
+                         uint8_t *__restrict__ lut) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= P.nb) return;
+    const float nmax = __uint_as_float(*nmax_bits);
+    float emax = P.cE * 2.0f * nmax;
+    emax = emax * 1.0001f + 1e-30f;
+    if (!(nmax <= 1e30f)) emax = __int_as_float(0x7F800000);
+    const uint32_t k0 = P.base + ((uint32_t)b << P.sh);
+    const uint64_t k1l = (uint64_t)P.base + ((uint64_t)(b + 1) << P.sh);
+    const float v0 = __uint_as_float(k0);
+    const float v1 = k1l >= 0x7F800000ull ? __int_as_float(0x7F800000) : __uint_as_float((uint32_t)k1l);
+    const bool last = b == P.nb - 1;
+    // keys below bucket 0 and above the last bucket are clamped into them
+    const float lo = b == 0 ? 0.0f : v0;
+    const float hi = last ? __int_as_float(0x7F800000) : v1;
+    int r_lo = 0, near = 0, idx = 0;
+    for (int t = 0; t < nt; t++) {
+        const float B = S.s[t];
+        if (B < v0) r_lo++;
+        if (B >= lo - emax && B <= hi + emax) {
+            if (near == 0) idx = t;
+            near++;
+        }
+    }
+    uint32_t e = (uint32_t)r_lo;
+    if (near == 1) e = 0x80u | (uint32_t)idx;
+    if (near >= 2) e = 0xC0u;
+    if (!(emax < __int_as_float(0x7F800000))) e = 0xC0u;   // non-finite or huge norms: all exact
+    lut[b] = (uint8_t)e;
+}
+
+// Second stage for a flagged pair: rank if the one nearby bound is farther
+// than E_ij = cE * (n_i + n_j), else flag for exact re-decision.
+__device__ __forceinline__ uint32_t mx_resolve(uint32_t ent, float st, float nsum, const float *bnd, float cE,
+                                               uint32_t &amb) {
+    if (!(ent & 0x80u)) return ent & 0x3Fu;
+    if (ent & 0x40u) {
+        amb = 1u;
+        return 0u;
+    }
+    const uint32_t idx = ent & 0x3Fu;
+    const float d = st - bnd[idx];
+    const float E = fmaf(cE, nsum, 1e-30f) * 1.0001f;
+    if (fabsf(d) > E) return idx + (d > 0.0f ? 1u : 0u);
+    amb = 1u;
+    return 0u;
+}
+
+__device__ __forceinline__ uint32_t mx_key_bucket(float st, const MxParams &P) {
+    const uint32_t key = __float_as_uint(st);   // st >= +0, finite under the norm guard
+    int b = (int)(key - P.base) >> P.sh;
+    return (uint32_t)(b < 0 ? 0 : (b > P.nb - 1 ? P.nb - 1 : b));
+}
+
+// Exact-order fp32 squared distance of one pair (the reference's sum).
+// Loads go out eight dimensions at a time; the sum keeps the order k = 0..m-1.
+__device__ __forceinline__ float exact_sqdist(const float *__restrict__ x, int64_t i, int64_t j, int tau, int m) {
+    float s = 0.f;
+    for (int k0 = 0; k0 < m; k0 += 8) {
+        float a[8], b[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const int64_t o = (int64_t)(k0 + k < m ? k0 + k : m - 1) * tau;
+            a[k] = x[i + o];
+            b[k] = x[j + o];
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            if (k0 + k < m) {
+                const float d = a[k] - b[k];
+                const float q = d * d;
+                s = s + q;
+            }
+        }
+    }
+    return s;
+}
+
+template <int TG>
+__device__ __forceinline__ uint32_t exact_rank(float s, const Bounds<float, TG> &S, int nt) {
+    return rank_direct<float, TG>(s, S, nt);
+}
+
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+// Wave-wide exclusive prefix sum of a small per-lane count.
+__device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, uint32_t &total) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    uint32_t inc = v;
+#pragma unroll
+    for (int off = 1; off < WAVE; off <<= 1) {
+        const uint32_t o = __shfl_up(inc, off);
+        if (lane >= off) inc += o;
+    }
+    total = __shfl(inc, WAVE - 1);
+    return inc - v;
+}
+
+// ---------------------------------------------------------------- k_lam_mx
+// LAM pass of the MFMA form.  Work item = (band of 32 columns, row segment).
+// Lane l owns column j = j0 + l%32; the 32x32 MFMA tile gives it 16 rows of
+// that column: rows 8g + 4h + r (g, r < 4, h = l/32), i.e. exactly the rank
+// registers R[4h .. 4h+3] of the byte packing.  The two halves swap their
+// four registers, then each half runs the words of every other threshold
+// (t = 2t' + h) over all 32 rows.
+template <int TG>
+__global__ __launch_bounds__(WG) void k_lam_mx(const float *__restrict__ x, const float *__restrict__ nrm, int64_t nc,
+                                               int tau, int m, Bounds<float, TG> S, int nt,
+                                               const uint8_t *__restrict__ lut, MxParams P, int seg_len, int nseg,
+                                               unsigned long long *__restrict__ hist, uint2 *__restrict__ carry) {
+    constexpr int TH = (TG + 1) / 2;
+    constexpr int QCAP = 256;                      // recheck queue entries per wave and round
+    __shared__ uint32_t lh[TG * LCAP];
+    __shared__ uint8_t lut_s[NBX];
+    __shared__ uint32_t rbuf[WAVES][WAVE][4];      // packed ranks, re-read after rechecks
+    __shared__ uint16_t queue[WAVES][QCAP];
+    __shared__ float bnd_s[TG];
+    __shared__ float rwin[WAVES][MXW];   // x[i0 .. i0 + 31 + (m-1)tau] of the current block
+    __shared__ float cwin[WAVES][MXW];   // x[j0 .. j0 + 31 + (m-1)tau] of the current item
+    for (int e = threadIdx.x; e < TG * LCAP; e += WG) lh[e] = 0;
+    for (int e = threadIdx.x; e < P.nb / 4; e += WG) ((uint32_t *)lut_s)[e] = ((const uint32_t *)lut)[e];
+    if (threadIdx.x < TG) bnd_s[threadIdx.x] = S.s[threadIdx.x];
+    __syncthreads();
+
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wave = wave_id();
+    const int col = lane & 31;
+    const int h = lane >> 5;
+    const int span = 32 + (m - 1) * tau;   // <= MXW (checked by the host)
+    const int64_t nbands = (nc + 31) / 32;
+    const int64_t nitems = nbands * nseg;
+    const size_t cstride = (size_t)nc * nseg;
+    const int m2 = (m + 1) / 2;
+
+    for (int64_t item = (int64_t)blockIdx.x * WAVES + wave; item < nitems; item += (int64_t)gridDim.x * WAVES) {
+        const int64_t band = item / nseg;
+        const int q = (int)(item - band * nseg);
+        const int64_t j0 = band * 32;
+        const int64_t j = j0 + col;
+        const bool jv = j < nc;
+        const int64_t ib = (int64_t)q * seg_len;
+        const int64_t ie = ib + seg_len < nc ? ib + seg_len : nc;
+        const float nj = nrm[j];
+        uint2 *cslot = carry + (size_t)(jv ? j : 0) * nseg + q;
+
+        uint32_t run[TH];
+#pragma unroll
+        for (int t = 0; t < TH; t++) run[t] = 0;
+        uint32_t popen = jv ? 0xFFFFFFFFu : 0u;
+
+        // column window once per item; row window of the first block
+        float *rw = rwin[wave], *cw = cwin[wave];
+        for (int e = lane; e < span; e += WAVE) cw[e] = x[j0 + e];
+        float pre0 = lane < span ? x[ib + lane] : 0.f;
+        float pre1 = lane + WAVE < span ? x[ib + WAVE + lane] : 0.f;
+
+        for (int64_t i0 = ib; i0 < ie; i0 += BLK) {
+            const int nb = ie - i0 < BLK ? (int)(ie - i0) : BLK;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (lane < span) rw[lane] = pre0;
+            if (lane + WAVE < span) rw[WAVE + lane] = pre1;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            // next block's row window, in flight while this block computes
+            // (only if there is a next block: its window ends by x[N + 30])
+            const bool more = i0 + BLK < ie;
+            pre0 = more && lane < span ? x[i0 + BLK + lane] : 0.f;
+            pre1 = more && lane + WAVE < span ? x[i0 + BLK + WAVE + lane] : 0.f;
+            // Gram tile G[i0 + a][j0 + b] on MFMA: lane supplies A[row=col][k=h], B[k=h][col]
+            floatx16 acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll 4
+            for (int k2 = 0; k2 < m2; k2++) {
+                const int dim = 2 * k2 + h;
+                const float a = dim < m ? rw[col + dim * tau] : 0.f;
+                const float b = dim < m ? cw[col + dim * tau] : 0.f;
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
+            }
+            // rows of element e: i0 + 8(e>>2) + 4h + (e&3)
+            float ni[16];
+#pragma unroll
+            for (int g = 0; g < 4; g++) {
+                const float4 v = *(const float4 *)(nrm + i0 + 8 * g + 4 * h);
+                ni[4 * g + 0] = v.x;
+                ni[4 * g + 1] = v.y;
+                ni[4 * g + 2] = v.z;
+                ni[4 * g + 3] = v.w;
+            }
+            uint32_t Rh[4] = {0u, 0u, 0u, 0u};   // R[4h + c]: rows 4h + c + 8g in byte g
+            uint32_t fm = 0;                      // elements to re-decide exactly
+#pragma unroll
+            for (int e = 0; e < 16; e++) {
+                const float sum = ni[e] + nj;
+                float st = fmaf(-2.0f, acc[e], sum);
+                st = fmaxf(st, 0.0f);
+                const uint32_t ent = lut_s[mx_key_bucket(st, P)];
+                uint32_t amb = 0;
+                const uint32_t rk = mx_resolve(ent, st, sum, bnd_s, P.cE, amb);
+                Rh[e & 3] |= rk << (8 * (e >> 2));
+                fm |= amb << e;
+            }
+            if (!jv) fm = 0;
+            // rows past the segment are filled later; do not re-decide them
+            {
+                const int nvr = nb;   // valid rows are i0 .. i0+nb-1
+                uint32_t vm = 0;
+#pragma unroll
+                for (int e = 0; e < 16; e++) vm |= ((8 * (e >> 2) + 4 * h + (e & 3)) < nvr ? 1u : 0u) << e;
+                fm &= vm;
+            }
+            if (__any(fm != 0)) {
+                uint32_t total;
+                const uint32_t cnt = __popc(fm);
+                uint32_t off = wave_excl_scan(cnt, total);
+#pragma unroll
+                for (int c = 0; c < 4; c++) rbuf[wave][lane][c] = Rh[c];
+                uint8_t *rb = (uint8_t *)rbuf[wave];
+                for (uint32_t base = 0; base < total; base += QCAP) {
+                    uint32_t f = fm;
+                    uint32_t o = off;
+                    while (f) {
+                        const uint32_t e = __builtin_ctz(f);
+                        f &= f - 1;
+                        if (o >= base && o < base + QCAP) queue[wave][o - base] = (uint16_t)((lane << 4) | e);
+                        o++;
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    const uint32_t nq = total - base < (uint32_t)QCAP ? total - base : (uint32_t)QCAP;
+                    for (uint32_t qi = lane; qi < nq; qi += WAVE) {
+                        const uint32_t it = queue[wave][qi];
+                        const uint32_t ow = it >> 4, e = it & 15u;
+                        const int64_t i = i0 + 8 * (e >> 2) + 4 * (ow >> 5) + (e & 3);
+                        const int64_t jj = j0 + (ow & 31);
+                        const float sx = exact_sqdist(x, i, jj, tau, m);
+                        const uint32_t r = exact_rank<TG>(sx, S, nt);
+                        // byte (e>>2) of register (e&3) of lane ow
+                        rb[(ow * 4 + (e & 3)) * 4 + (e >> 2)] = (uint8_t)r;
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                }
+#pragma unroll
+                for (int c = 0; c < 4; c++) Rh[c] = rbuf[wave][lane][c];
+            }
+            // full rank set of this column: R[0..3] from half 0, R[4..7] from half 1
+            uint32_t R[8];
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                const uint32_t other = __shfl_xor(Rh[c], 32);
+                R[c] = h ? other : Rh[c];
+                R[4 + c] = h ? Rh[c] : other;
+            }
+            const uint32_t fill = nb == BLK ? 0u : (0xFFFFFFFFu << nb);
+#pragma unroll
+            for (int tp = 0; tp < TH; tp++) {
+                const uint32_t t = 2 * tp + h;
+                if ((int)t < nt) {
+                    const uint32_t K = 0x80808080u + t * 0x01010101u;
+                    uint32_t w = 0;
+#pragma unroll
+                    for (int c = 0; c < 8; c++) w |= ((K - R[c]) >> (7 - c)) & (0x01010101u << c);
+                    w = jv ? (w | fill) : 0u;
+                    consume_word(w, run[tp], popen, (uint32_t)tp, cslot + (size_t)t * cstride - (size_t)tp * cstride,
+                                 cstride, lh + t * LCAP, hist + (size_t)t * (nc + 1) - (size_t)tp * (nc + 1),
+                                 (uint32_t)nc);
+                    if (fill && jv) run[tp] -= (uint32_t)(BLK - nb);
+                }
+            }
+        }
+        if (jv) {
+#pragma unroll
+            for (int tp = 0; tp < TH; tp++) {
+                const uint32_t t = 2 * tp + h;
+                if ((int)t < nt) close_segment(&cslot[(size_t)t * cstride], run[tp], popen, 1u << tp);
+            }
+        }
+    }
+
+    __syncthreads();
+    for (int e = threadIdx.x; e < TG * LCAP; e += WG) {
+        const uint32_t v = lh[e];
+        const int t = e / LCAP;
+        if (v && t < nt) glob_add(&hist[(size_t)t * (nc + 1) + (e - t * LCAP)], v);
+    }
+}
+
+// ---------------------------------------------------------------- k_det_mx
+// DET pass of the MFMA form.  Work item = (strip of 64 diagonals, step
+// segment), as k_det.  A 32-step block needs G(s0+u, s0+r0+v) for v = u + l;
+// three 32x32 MFMA tiles cover v = 0..95.  Each tile element is written,
+// already combined as n_col - 2 g, into a skewed LDS buffer S[l][u] (l = v - u)
+// so that lane l then reads its own diagonal's 32 steps, adds n_row, and packs
+// the 32 ranks itself (no exchange).  Elements with v - u outside 0..63 are
+// not needed and not written.
+constexpr int NBX_DET = 16384;
+constexpr int SKS = 33;   // skew buffer row stride (floats): conflict-free writes and reads
+
+template <int TG>
+__global__ __launch_bounds__(WG) void k_det_mx(const float *__restrict__ x, const float *__restrict__ nrm, int64_t nc,
+                                               int tau, int m, Bounds<float, TG> S, int nt,
+                                               const uint8_t *__restrict__ lut, MxParams P, int seg_len, int nseg,
+                                               const int2 *__restrict__ items, int64_t nitems,
+                                               unsigned long long *__restrict__ hist, uint2 *__restrict__ carry) {
+    constexpr int QCAP = 256;
+    __shared__ uint32_t lh[TG * LCAP];
+    __shared__ uint8_t lut_s[NBX_DET];
+    __shared__ float skew[WAVES][WAVE * SKS];      // also holds the packed ranks during rechecks
+    __shared__ uint16_t queue[WAVES][QCAP];
+    __shared__ float nwin[WAVES][BLK + WAVE];      // n_j of the block's columns
+    __shared__ float bnd_s[TG];
+    __shared__ float rwin[WAVES][MXW];             // x[s0 .. s0 + 31 + (m-1)tau]
+    __shared__ float cwin[WAVES][MXW + WAVE];      // x[c0 .. c0 + 95 + (m-1)tau]
+    for (int e = threadIdx.x; e < TG * LCAP; e += WG) lh[e] = 0;
+    for (int e = threadIdx.x; e < P.nb / 4; e += WG) ((uint32_t *)lut_s)[e] = ((const uint32_t *)lut)[e];
+    if (threadIdx.x < TG) bnd_s[threadIdx.x] = S.s[threadIdx.x];
+    __syncthreads();
+
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wave = wave_id();
+    const int col = lane & 31;
+    const int h = lane >> 5;
+    float *sk = skew[wave];
+    float *rw = rwin[wave], *cw = cwin[wave];
+    const size_t cstride = (size_t)(nc - 1) * nseg;
+    const int m2 = (m + 1) / 2;
+    const int rspan = 32 + (m - 1) * tau, cspan = 96 + (m - 1) * tau;   // <= MXW, MXW + 64
+
+    for (int64_t item = (int64_t)blockIdx.x * WAVES + wave; item < nitems; item += (int64_t)gridDim.x * WAVES) {
+        const int2 it = items[item];
+        const int64_t r0 = 1 + (int64_t)it.x * WAVE;
+        const int q = it.y;
+        const int64_t r = r0 + lane;
+        const bool rv = r < nc;
+        const int64_t dlen = rv ? nc - r : 0;
+        const int64_t sb = (int64_t)q * seg_len;
+        const int64_t se_nom = sb + seg_len;
+        const int64_t smax = nc - r0;
+        const int64_t se = se_nom < smax ? se_nom : smax;
+        uint2 *cslot = carry + (size_t)(rv ? r - 1 : 0) * nseg + q;
+
+        uint32_t run[TG];
+#pragma unroll
+        for (int t = 0; t < TG; t++) run[t] = 0;
+        uint32_t popen = rv ? 0xFFFFFFFFu : 0u;
+
+        // operand windows of the first block
+        float pr[2], pc[3];
+        pr[0] = lane < rspan ? x[sb + lane] : 0.f;
+        pr[1] = lane + WAVE < rspan ? x[sb + WAVE + lane] : 0.f;
+#pragma unroll
+        for (int c = 0; c < 3; c++) pc[c] = lane + c * WAVE < cspan ? x[sb + r0 + c * WAVE + lane] : 0.f;
+
+        for (int64_t s0 = sb; s0 < se; s0 += BLK) {
+            const int64_t c0 = s0 + r0;   // first column of tile 0
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+            for (int c = 0; c < 2; c++)
+                if (lane + c * WAVE < rspan) rw[lane + c * WAVE] = pr[c];
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+                if (lane + c * WAVE < cspan) cw[lane + c * WAVE] = pc[c];
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            // next block's windows, in flight while this block computes (only
+            // if there is a next block: its windows end by x[N + 94])
+            const bool more = s0 + BLK < se;
+#pragma unroll
+            for (int c = 0; c < 2; c++)
+                pr[c] = more && lane + c * WAVE < rspan ? x[s0 + BLK + c * WAVE + lane] : 0.f;
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+                pc[c] = more && lane + c * WAVE < cspan ? x[c0 + BLK + c * WAVE + lane] : 0.f;
+            floatx16 acc0 = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+            floatx16 acc1 = acc0, acc2 = acc0;
+#pragma unroll 2
+            for (int k2 = 0; k2 < m2; k2++) {
+                const int dim = 2 * k2 + h;
+                const bool dv = dim < m;
+                const int off = dim * tau;
+                const float a = dv ? rw[col + off] : 0.f;
+                const float b0 = dv ? cw[col + off] : 0.f;
+                const float b1 = dv ? cw[32 + col + off] : 0.f;
+                const float b2 = dv ? cw[64 + col + off] : 0.f;
+                acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b0, acc0, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b1, acc1, 0, 0, 0);
+                acc2 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b2, acc2, 0, 0, 0);
+            }
+            // element e of tile p: row u = 8(e>>2) + 4h + (e&3), column v = 32p + col, diagonal l = v - u
+            const float n0 = nrm[c0 + col], n1 = nrm[c0 + 32 + col], n2 = nrm[c0 + 64 + col];
+            nwin[wave][lane] = h ? n1 : n0;       // columns c0 .. c0+63
+            if (lane < 32) nwin[wave][64 + lane] = n2;   // columns c0+64 .. c0+95
+#pragma unroll
+            for (int e = 0; e < 16; e++) {
+                const int u = 8 * (e >> 2) + 4 * h + (e & 3);
+                const int l0 = col - u, l1 = 32 + col - u, l2 = 64 + col - u;
+                if (l0 >= 0) sk[l0 * SKS + u] = fmaf(-2.0f, acc0[e], n0);
+                sk[l1 * SKS + u] = fmaf(-2.0f, acc1[e], n1);          // 1 <= l1 <= 63 always
+                if (l2 < WAVE) sk[l2 * SKS + u] = fmaf(-2.0f, acc2[e], n2);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const float *nrow = nrm + s0;   // uniform: scalar loads
+            uint32_t R[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
+            uint32_t fm = 0;
+#pragma unroll
+            for (int u = 0; u < BLK; u++) {
+                float st = sk[lane * SKS + u] + nrow[u];
+                st = fmaxf(st, 0.0f);
+                const uint32_t ent = lut_s[mx_key_bucket(st, P)];
+                uint32_t amb = 0, rk;
+                if (ent & 0x80u)
+                    rk = mx_resolve(ent, st, nrow[u] + nwin[wave][lane + u], bnd_s, P.cE, amb);
+                else
+                    rk = ent & 0x3Fu;
+                R[u & 7] |= rk << (8 * (u >> 3));
+                fm |= amb << u;
+            }
+            const int64_t nv64 = dlen - s0;
+            const int nvalid = nv64 <= 0 ? 0 : (nv64 >= BLK ? BLK : (int)nv64);
+            const uint32_t vmask = nvalid >= BLK ? 0xFFFFFFFFu : ((1u << nvalid) - 1u);
+            fm &= vmask;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (__any(fm != 0)) {
+                uint32_t *rb32 = (uint32_t *)sk;   // 64 lanes x 8 registers, reused skew buffer
+#pragma unroll
+                for (int c = 0; c < 8; c++) rb32[lane * 8 + c] = R[c];
+                uint8_t *rb = (uint8_t *)sk;
+                uint32_t total;
+                const uint32_t off = wave_excl_scan(__popc(fm), total);
+                for (uint32_t base = 0; base < total; base += QCAP) {
+                    uint32_t f = fm, o = off;
+                    while (f) {
+                        const uint32_t u = __builtin_ctz(f);
+                        f &= f - 1;
+                        if (o >= base && o < base + QCAP) queue[wave][o - base] = (uint16_t)((lane << 5) | u);
+                        o++;
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    const uint32_t nq = total - base < (uint32_t)QCAP ? total - base : (uint32_t)QCAP;
+                    for (uint32_t qi = lane; qi < nq; qi += WAVE) {
+                        const uint32_t itq = queue[wave][qi];
+                        const uint32_t ow = itq >> 5, u = itq & 31u;
+                        const int64_t i = s0 + u;
+                        const int64_t jj = i + r0 + ow;
+                        const float sx = exact_sqdist(x, i, jj, tau, m);
+                        // byte (u>>3) of register (u&7) of lane ow
+                        rb[(ow * 8 + (u & 7)) * 4 + (u >> 3)] = (uint8_t)exact_rank<TG>(sx, S, nt);
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                }
+#pragma unroll
+                for (int c = 0; c < 8; c++) R[c] = rb32[lane * 8 + c];
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            }
+#pragma unroll
+            for (int t = 0; t < TG; t++) {
+                if (t < nt) {
+                    const uint32_t w = swar_word(R, t) & vmask;
+                    consume_word(w, run[t], popen, t, cslot, cstride, lh + t * LCAP, hist, (uint32_t)nc);
+                }
+            }
+        }
+        if (rv) {
+#pragma unroll
+            for (int t = 0; t < TG; t++)
+                if (t < nt) close_segment(&cslot[(size_t)opaque(t) * cstride], run[t], popen, 1u << t);
+        }
+    }
+
+    __syncthreads();
+    for (int e = threadIdx.x; e < TG * LCAP; e += WG) {
+        const uint32_t v = lh[e];
+        const int t = e / LCAP;
+        if (v && t < nt) glob_add(&hist[(size_t)t * (nc + 1) + (e - t * LCAP)], v);
+    }
+}
+
 // ---------------------------------------------------------------- k_stitch
 // One thread per (threshold, sequence): joins the (prefix, suffix) runs of
 // the sequence's segments in order.  seq_len(sq) gives the sequence length;

@@ -46,7 +46,8 @@ struct Ctx {
     int device = -1;
     int ncu = 0;
     hipStream_t stream = nullptr;
-    DevBuf x, hist, carry, metric, scan, items_rr, items_det, counts, perm, lmins, lut, out_rr, out_det, out_lam;
+    DevBuf x, hist, carry, metric, scan, items_rr, items_det, counts, perm, lmins, lut, nrm, nmax, lut8, out_rr,
+        out_det, out_lam;
     int64_t items_nc = -1;     // (nc, seg_len) the item tables were built for
     int items_L = -1;
     int64_t n_items_rr = 0, n_items_det = 0;
@@ -54,6 +55,20 @@ struct Ctx {
 
 std::mutex g_mu;
 Ctx g_ctx;
+
+// Euclidean distance form: 0 = automatic (MFMA Gram + exact recheck for fp32
+// input with m >= 8, exact VALU sums otherwise), 1 = exact VALU sums only,
+// 2 = MFMA + recheck wherever it applies (fp32 input, Euclidean norm).
+int g_form = 0;
+
+template <typename F>
+bool use_mfma(int dist, int m, int tau) {
+    if (sizeof(F) != 4 || dist != 1) return false;
+    if ((int64_t)(m - 1) * tau > rqa::MXW - 32) return false;   // operand window must fit in LDS
+    if (g_form == 1) return false;
+    if (g_form == 2) return true;
+    return m >= 8;
+}
 
 int map_err(hipError_t e) {
     if (e == hipSuccess) return E_OK;
@@ -98,7 +113,8 @@ int init_ctx() {
             int cur = dev;
             (void)hipSetDevice(g_ctx.device);
             DevBuf *all[] = {&g_ctx.x, &g_ctx.hist, &g_ctx.carry, &g_ctx.metric, &g_ctx.scan, &g_ctx.items_rr,
-                             &g_ctx.items_det, &g_ctx.counts, &g_ctx.perm, &g_ctx.lmins, &g_ctx.lut, &g_ctx.out_rr,
+                             &g_ctx.items_det, &g_ctx.counts, &g_ctx.perm, &g_ctx.lmins, &g_ctx.lut, &g_ctx.nrm, &g_ctx.nmax,
+                             &g_ctx.lut8, &g_ctx.out_rr,
                              &g_ctx.out_det, &g_ctx.out_lam};
             g_ctx.items_nc = -1;
             for (DevBuf *b : all) {
@@ -338,6 +354,24 @@ int build_items(int64_t nc, hipStream_t st) {
     return E_OK;
 }
 
+// Parameters of the MFMA form's byte table for sorted bounds b[0..nt).
+rqa::MxParams mx_params(const float *b, int nt, int m, int nbmax) {
+    rqa::MxParams P;
+    const float lo = b[0] * 0.5f;
+    float hi = b[nt - 1] * 2.0f;
+    if (!(hi <= std::numeric_limits<float>::max())) hi = std::numeric_limits<float>::max();
+    const uint32_t base = hkey(lo), top = hkey(hi);
+    int sh = 0;
+    while (((top - base) >> sh) > (uint32_t)(nbmax - 8)) sh++;
+    int nb = (int)((top - base) >> sh) + 2;
+    nb = (nb + 3) & ~3;
+    P.base = base;
+    P.sh = sh;
+    P.nb = nb;
+    P.cE = (float)((4.0 * m + 16.0) * std::ldexp(1.0, -24));
+    return P;
+}
+
 // One group of up to 32 possibly-recurrent thresholds (sorted bounds).
 template <typename F, int NORM, int TG>
 int launch_group(const F *dx, int64_t nc, int tau, int m, const F *bounds, int nt, const rqa::LutEntry<F> *d_lut,
@@ -373,12 +407,38 @@ int launch_group(const F *dx, int64_t nc, int tau, int m, const F *bounds, int n
 
     if (want_lam) {
         fill_zero(hist, (size_t)nt * hs, st);
-        const int64_t ni = ((nc + rqa::WAVE - 1) / rqa::WAVE) * nseg;
-        prof_begin("k_lam", st);
-        hipLaunchKernelGGL((rqa::k_lam<F, NORM, TG>), dim3(grid_for(ni, c.ncu)), dim3(rqa::WG), 0, st, dx, nc, tau,
-                           m, B, nt, d_lut, P, L, nseg, hist, carry);
-        prof_end(st);
-        HIP_TRY(hipGetLastError());
+        if constexpr (sizeof(F) == 4 && NORM == 1) {
+            if (use_mfma<F>(1, m, tau)) {
+                rqa::Bounds<float, TG> Bf;
+                for (int t = 0; t < TG; t++) Bf.s[t] = (float)B.s[t];
+                const rqa::MxParams MP = mx_params((const float *)bounds, nt, m, rqa::NBX);
+                if ((e = ensure(c.lut8, rqa::NBX))) return e;
+                hipLaunchKernelGGL((rqa::k_mx_lut<TG>), dim3((MP.nb + 255) / 256), dim3(256), 0, st, Bf, nt, MP,
+                                   (const unsigned int *)c.nmax.p, (uint8_t *)c.lut8.p);
+                HIP_TRY(hipGetLastError());
+                const int64_t ni = ((nc + 31) / 32) * nseg;
+                prof_begin("k_lam_mx", st);
+                hipLaunchKernelGGL((rqa::k_lam_mx<TG>), dim3(grid_for(ni, c.ncu)), dim3(rqa::WG), 0, st,
+                                   (const float *)dx, (const float *)c.nrm.p, nc, tau, m, Bf, nt,
+                                   (const uint8_t *)c.lut8.p, MP, L, nseg, hist, carry);
+                prof_end(st);
+                HIP_TRY(hipGetLastError());
+            } else {
+                const int64_t ni = ((nc + rqa::WAVE - 1) / rqa::WAVE) * nseg;
+                prof_begin("k_lam", st);
+                hipLaunchKernelGGL((rqa::k_lam<F, NORM, TG>), dim3(grid_for(ni, c.ncu)), dim3(rqa::WG), 0, st, dx,
+                                   nc, tau, m, B, nt, d_lut, P, L, nseg, hist, carry);
+                prof_end(st);
+                HIP_TRY(hipGetLastError());
+            }
+        } else {
+            const int64_t ni = ((nc + rqa::WAVE - 1) / rqa::WAVE) * nseg;
+            prof_begin("k_lam", st);
+            hipLaunchKernelGGL((rqa::k_lam<F, NORM, TG>), dim3(grid_for(ni, c.ncu)), dim3(rqa::WG), 0, st, dx, nc,
+                               tau, m, B, nt, d_lut, P, L, nseg, hist, carry);
+            prof_end(st);
+            HIP_TRY(hipGetLastError());
+        }
         const int64_t nthr = (int64_t)nt * nc;
         prof_begin("k_stitch", st);
         hipLaunchKernelGGL(rqa::k_stitch, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, st,
@@ -403,11 +463,34 @@ int launch_group(const F *dx, int64_t nc, int tau, int m, const F *bounds, int n
         fill_zero(hist, (size_t)nt * hs, st);
         if (nc > 1) {
             const int64_t ni = c.n_items_det;
-            prof_begin("k_det", st);
-            hipLaunchKernelGGL((rqa::k_det<F, NORM, TG>), dim3(grid_for(ni, c.ncu)), dim3(rqa::WG), 0, st, dx, nc,
-                               tau, m, B, nt, d_lut, P, L, nseg, (const int2 *)c.items_det.p, ni, hist, carry);
-            prof_end(st);
-            HIP_TRY(hipGetLastError());
+            bool done = false;
+            if constexpr (sizeof(F) == 4 && NORM == 1) {
+                if (use_mfma<F>(1, m, tau)) {
+                    rqa::Bounds<float, TG> Bf;
+                    for (int t = 0; t < TG; t++) Bf.s[t] = (float)B.s[t];
+                    const rqa::MxParams MP = mx_params((const float *)bounds, nt, m, rqa::NBX_DET);
+                    if ((e = ensure(c.lut8, rqa::NBX))) return e;
+                    uint8_t *lut8d = (uint8_t *)c.lut8.p;
+                    hipLaunchKernelGGL((rqa::k_mx_lut<TG>), dim3((MP.nb + 255) / 256), dim3(256), 0, st, Bf, nt, MP,
+                                       (const unsigned int *)c.nmax.p, lut8d);
+                    HIP_TRY(hipGetLastError());
+                    prof_begin("k_det_mx", st);
+                    hipLaunchKernelGGL((rqa::k_det_mx<TG>), dim3(grid_for(ni, c.ncu)), dim3(rqa::WG), 0, st,
+                                       (const float *)dx, (const float *)c.nrm.p, nc, tau, m, Bf, nt,
+                                       (const uint8_t *)lut8d, MP, L, nseg, (const int2 *)c.items_det.p, ni, hist,
+                                       carry);
+                    prof_end(st);
+                    HIP_TRY(hipGetLastError());
+                    done = true;
+                }
+            }
+            if (!done) {
+                prof_begin("k_det", st);
+                hipLaunchKernelGGL((rqa::k_det<F, NORM, TG>), dim3(grid_for(ni, c.ncu)), dim3(rqa::WG), 0, st, dx,
+                                   nc, tau, m, B, nt, d_lut, P, L, nseg, (const int2 *)c.items_det.p, ni, hist, carry);
+                prof_end(st);
+                HIP_TRY(hipGetLastError());
+            }
             const int64_t nthr = (int64_t)nt * (nc - 1);
             prof_begin("k_stitch", st);
             hipLaunchKernelGGL(rqa::k_stitch, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, st,
@@ -517,6 +600,15 @@ int run_block(const F *dx, size_t n, int tau, int emb, const F *thr, int T, cons
         HIP_TRY(hipMemcpyAsync(c.lut.p, luts.data(), (size_t)ngroups * rqa::NB_MAX * sizeof(rqa::LutEntry<F>),
                                hipMemcpyHostToDevice, st));
     if ((e = build_items(nc, st))) return e;
+    if (use_mfma<F>(dist, emb, tau) && ngroups > 0) {
+        if ((e = ensure(c.nrm, ((size_t)nc + 2 * rqa::WAVE) * sizeof(float)))) return e;
+        if ((e = ensure(c.nmax, 256))) return e;
+        fill_zero((unsigned long long *)c.nmax.p, 1, st);
+        const int64_t nthr = nc + 2 * rqa::WAVE;
+        hipLaunchKernelGGL(rqa::k_norms, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, st, (const float *)dx, nc,
+                           tau, emb, (float *)c.nrm.p, (unsigned int *)c.nmax.p);
+        HIP_TRY(hipGetLastError());
+    }
 
     for (int g = 0; g < ngroups; g++) {
         const int g0 = g * 32, nt = std::min(32, (int)live.size() - g0);
@@ -768,6 +860,13 @@ int accrqa_amd_rqa_device_f64(const double *d_input, size_t data_size, int tau, 
 }
 
 const char *accrqa_amd_last_error(void) { return g_last_error.c_str(); }
+
+int accrqa_amd_set_form(int form) {
+    if (form < 0 || form > 2) return E_INVALID;
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_form = form;
+    return E_OK;
+}
 
 void accrqa_amd_profile(int enable) {
     std::lock_guard<std::mutex> lk(g_mu);

@@ -106,6 +106,8 @@ def main():
     ap.add_argument("--workload", default="ar1_m16", choices=["ar1_m16", "lorenz", "logistic"])
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--form", default="both", choices=["both", "exact", "mfma"],
+                    help="Euclidean distance form(s) to time; 'both' reports the MFMA form as value")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -135,41 +137,69 @@ def main():
         accrqa_amd.rqa_device(dx.data_ptr(), x.size, tau, emb, thr, lmin, dist, d_rr.data_ptr(), d_det.data_ptr(),
                               d_lam.data_ptr(), stream=stream.cuda_stream, dtype=np.float32)
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-
-    L.accrqa_amd_profile_reset()
-    L.accrqa_amd_profile(1)
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    w0 = time.perf_counter()
-    ev0.record(stream)
-    for _ in range(args.steps):
-        step()
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - w0
-    L.accrqa_amd_profile(0)
-    ms_step = max(ev0.elapsed_time(ev1), wall * 1e3) / args.steps
-
     import ctypes as C
-    kern = {}
-    for k in ("k_lam", "k_det", "k_rr", "k_stitch", "k_finalize"):
-        ms, n = C.c_double(0), C.c_longlong(0)
-        L.accrqa_amd_profile_read(k.encode(), C.byref(ms), C.byref(n))
-        kern[k] = {"ms_total": ms.value, "launches": n.value}
+
+    def measure(form):
+        accrqa_amd.set_form(form)
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        L.accrqa_amd_profile_reset()
+        L.accrqa_amd_profile(1)
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        w0 = time.perf_counter()
+        ev0.record(stream)
+        for _ in range(args.steps):
+            step()
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - w0
+        L.accrqa_amd_profile(0)
+        ms = max(ev0.elapsed_time(ev1), wall * 1e3) / args.steps
+        kern = {}
+        for k in ("k_lam", "k_lam_mx", "k_det", "k_det_mx", "k_rr", "k_stitch", "k_finalize"):
+            t, n = C.c_double(0), C.c_longlong(0)
+            L.accrqa_amd_profile_read(k.encode(), C.byref(t), C.byref(n))
+            if n.value:
+                kern[k] = {"ms_total": t.value, "launches": n.value}
+        outs = (d_rr.cpu().numpy().copy(), d_det.cpu().numpy().copy(), d_lam.cpu().numpy().copy())
+        return ms, kern, outs
+
+    forms = {"exact": accrqa_amd.FORM_EXACT, "mfma": accrqa_amd.FORM_MFMA}
+    todo = ["exact", "mfma"] if args.form == "both" else [args.form]
+    if dist != 1:
+        todo = ["exact"]
+    res = {f: measure(forms[f]) for f in todo}
+    accrqa_amd.set_form(accrqa_amd.FORM_AUTO)
+    head = "mfma" if "mfma" in res else "exact"
+    ms_step, kern, outs = res[head]
+    mismatches = None
+    if "mfma" in res and "exact" in res:
+        a, b = res["exact"][2], res["mfma"][2]
+        mismatches = int(sum(int((u.view(np.uint32) != v.view(np.uint32)).sum()) for u, v in zip(a, b)))
 
     value = float(nc) * nc * T / (ms_step / 1e3)
-    lam = kern["k_lam"]
+    # Roofline of the exact form's dominant kernel (k_lam): every pair of the
+    # full matrix, 3m FP ops (m sub, m mul, m add) for the Euclidean distance or
+    # 2m (sub, max) for the maximal norm, plus one compare per threshold.
+    ms_x, kern_x, _ = res.get("exact", res[head])
+    lam = kern_x.get("k_lam", {"ms_total": 0.0, "launches": 1})
     lam_avg_ms = lam["ms_total"] / max(lam["launches"], 1)
-    # algorithmic FP ops of one k_lam launch: every pair of the full matrix,
-    # 3m (m sub, m mul, m add) for the Euclidean distance or 2m (sub, max) for
-    # the maximal norm, plus one compare per threshold of the launch's group.
     per_pair = (3 * emb if dist == 1 else 2 * emb) + T / max(lam["launches"] // args.steps, 1)
     ops = float(nc) * nc * per_pair
     achieved = ops / (lam_avg_ms / 1e3) / 1e12 if lam_avg_ms > 0 else 0.0
+    form_report = {}
+    for f, (ms_f, kern_f, _) in res.items():
+        form_report[f] = {"value": float(nc) * nc * T / (ms_f / 1e3), "ms_per_step": ms_f,
+                          "kernels_ms_per_step": {k: v["ms_total"] / args.steps for k, v in kern_f.items()}}
+    mx = kern.get("k_lam_mx")
+    if mx:
+        mx_ms = mx["ms_total"] / max(mx["launches"], 1)
+        gram_tflops = float(nc) * nc * 2 * emb / (mx_ms / 1e3) / 1e12
+        form_report["mfma"]["k_lam_mx_gram_tflops"] = gram_tflops
+        form_report["mfma"]["k_lam_mx_gram_frac_of_fp32_mfma_peak"] = gram_tflops / PUBLISHED_FP32_TFLOPS
 
     rr = d_rr.cpu().numpy()
     out = {
@@ -186,12 +216,17 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": c["desc"], "N": int(x.size), "N_c": int(nc), "thresholds": T,
-                   "form": "plain FP32 (exact decisions on VALU)"},
+                   "form": {"mfma": "MFMA Gram (fp32) + exact-order recheck near thresholds",
+                            "exact": "exact-order FP32 sums on VALU"}[head]},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TOPS, "unit": "TFLOP/s",
                      "frac": achieved / VALU_PEAK_TOPS, "traffic": None, "kernel": "k_lam",
                      "kernel_avg_ms": lam_avg_ms, "ops_per_launch": ops,
                      "peak_note": f"FP32 VALU non-FMA issue rate; published FMA/packed peak {PUBLISHED_FP32_TFLOPS}"},
         "kernels_ms_per_step": {k: v["ms_total"] / args.steps for k, v in kern.items()},
+        "forms": form_report,
+        "mfma_vs_exact_speedup": (form_report["exact"]["ms_per_step"] / form_report["mfma"]["ms_per_step"]
+                                  if "mfma" in form_report and "exact" in form_report else None),
+        "mismatches_mfma_vs_exact": mismatches,
         "rr_min_max": [float(rr.min()), float(rr.max())],
     }
     if not args.no_cpu_baseline:

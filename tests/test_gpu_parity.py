@@ -290,3 +290,37 @@ def test_threshold_sets_for_rank_table(amd, oracle):
             for dist in (1, 2):
                 say("rank table", dt, name, dist)
                 check_case(amd, oracle, x, [1, 3], [2], [2], thrs, dist, f"rank {name} {dt} {dist}")
+
+
+@pytest.fixture
+def mfma_form(amd):
+    amd.set_form(amd.FORM_MFMA)
+    yield amd
+    amd.set_form(amd.FORM_AUTO)
+
+
+def test_mfma_form_bit_exact(mfma_form, oracle):
+    """The MFMA form (Gram on matrix cores + exact recheck near thresholds)
+    against the oracle: every decision, histogram and measure as the exact
+    form's.  fp32 Euclidean cases of several shapes, ties and special values."""
+    amd = mfma_form
+    rng = np.random.default_rng(77)
+    cases = []
+    x = np.cumsum(rng.standard_normal(3500)).astype(np.float32)
+    cases.append(("walk", ((x - x.mean()) / x.std()).astype(np.float32), [1, 2], [16, 5], [2, 3],
+                  list(np.linspace(0.2, 3.0, 21).astype(np.float32))))
+    cases.append(("ties", rng.integers(-4, 5, size=1300).astype(np.float32), [1, 3], [8, 3], [2],
+                  [np.float32(np.sqrt(np.float32(k))) for k in (1, 4, 9, 17, 40, 61)]))
+    xs = rng.standard_normal(900).astype(np.float32)
+    xs[[10, 500]] = np.nan
+    xs[300] = np.inf
+    cases.append(("special", xs, [1], [9], [2], [-1.0, 0.0, 0.5, 2.0, np.inf]))
+    cases.append(("m33", rng.standard_normal(2100).astype(np.float32), [2], [33], [2, 4],
+                  list(np.linspace(1.0, 12.0, 7).astype(np.float32))))
+    for name, x, taus, embs, lm, thrs in cases:
+        say("mfma", name)
+        check_case(amd, oracle, x, taus, embs, lm, thrs, 1, f"mfma {name}")
+        got = amd.histograms(x, taus[0], embs[0], np.asarray(thrs, np.float32), 1, which=7)
+        for k, t in enumerate(thrs):
+            assert np.array_equal(got["lam"][k], oracle.hist("lam", x, np.float32(t), taus[0], embs[0], 1)), (name, t)
+            assert np.array_equal(got["det"][k], oracle.hist("det", x, np.float32(t), taus[0], embs[0], 1)), (name, t)
