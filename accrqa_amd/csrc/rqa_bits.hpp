@@ -1,0 +1,131 @@
+// rqa_bits.hpp -- word-level pieces of the line scan: decision words from
+// packed threshold ranks, and run extraction from a 32-step word.  Plain
+// integer code, compiled into the kernels and, unchanged, into the host unit
+// test of the line scan (tests/test_line_scan_bits.py).
+//
+// Conventions.  A lane walks one sequence (a column for LAM, a diagonal for
+// DET) in blocks of 32 steps.  For each step the engine has a rank
+// r = #{t : B_t < s} over the group's sorted squared bounds, so threshold t
+// recurs iff r <= t.  Ranks are packed one per byte: register q holds steps
+// q, q+8, q+16, q+24 (byte g = step q + 8g).  Words are "not recurrent" masks:
+// bit u of nw(t) is set iff step u is not recurrent for threshold t.
+#pragma once
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#define RQA_HD __host__ __device__ __forceinline__
+#else
+#define RQA_HD inline
+#endif
+
+namespace rqa {
+
+constexpr int LCAP = 128;              // run lengths < LCAP go to the workgroup's LDS histogram
+constexpr uint32_t PRE = 0x80000000u;  // run counter flag: the segment's first run has not closed yet
+
+// Position of the lowest / highest set bit (as the hardware's find-first-bit
+// instructions): 0xFFFFFFFF for a zero word.
+RQA_HD uint32_t ffbl(uint32_t v) { return v ? (uint32_t)__builtin_ctz(v) : 0xFFFFFFFFu; }
+RQA_HD uint32_t ffbh(uint32_t v) { return v ? (uint32_t)__builtin_clz(v) : 0xFFFFFFFFu; }
+
+// Bits of a rank in [0, TG].
+template <int TG>
+constexpr int rank_bits() {
+    return TG < 2 ? 1 : TG < 4 ? 2 : TG < 8 ? 3 : TG < 16 ? 4 : TG < 32 ? 5 : 6;
+}
+
+// Exchange of the off-diagonal sub-blocks of an 8x8 bit matrix held one row
+// per byte (four matrices side by side): bits (k+s) of row a trade places
+// with bits k of row a+s, for the bits k selected by m.
+RQA_HD void bit_swap(uint32_t &a, uint32_t &b, int s, uint32_t m) {
+    const uint32_t na = (a & ~(m << s)) | ((b << s) & (m << s));
+    const uint32_t nb = (b & ~m) | ((a >> s) & m);
+    a = na;
+    b = nb;
+}
+
+// Bit planes of the packed ranks: P[k] bit (q + 8g) = bit k of the rank of
+// step q + 8g, i.e. bit u of P[k] is bit k of step u's rank.  Each byte
+// column is an 8x8 bit transpose (rows = registers, columns = rank bits),
+// done in three exchange stages.
+template <int B>
+RQA_HD void rank_planes(const uint32_t (&R)[8], uint32_t (&P)[B]) {
+    uint32_t r[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) r[q] = R[q];
+#pragma unroll
+    for (int q = 0; q < 4; q++) bit_swap(r[q], r[q + 4], 4, 0x0F0F0F0Fu);
+#pragma unroll
+    for (int q = 0; q < 8; q += 4) {
+        bit_swap(r[q], r[q + 2], 2, 0x33333333u);
+        bit_swap(r[q + 1], r[q + 3], 2, 0x33333333u);
+    }
+#pragma unroll
+    for (int q = 0; q < 8; q += 2) bit_swap(r[q], r[q + 1], 1, 0x55555555u);
+#pragma unroll
+    for (int k = 0; k < B; k++) P[k] = r[k];
+}
+
+// Comparator tree over the planes: ge(c) = mask of steps with rank >= c, so
+// nw(t) = ge(t + 1).  lo[c] (c < 2^(B-1)) compares the low B-1 bits; the top
+// bit is applied per threshold.  All indices are compile-time constants
+// after unrolling, so the tree costs one logic operation per node.
+template <int B>
+struct GeTree {
+    static constexpr int H = 1 << (B - 1);
+    uint32_t lo[H];
+    uint32_t top;
+
+    RQA_HD GeTree(const uint32_t (&P)[B]) {
+        lo[0] = 0xFFFFFFFFu;
+#pragma unroll
+        for (int j = 0; j < B - 1; j++) {
+            const int n = 1 << j;
+#pragma unroll
+            for (int c = n; c < 2 * n; c++) lo[c] = P[j] & lo[c - n];
+#pragma unroll
+            for (int c = 0; c < n; c++) lo[c] = P[j] | lo[c];
+        }
+        top = P[B - 1];
+    }
+    // rank >= c, for 1 <= c < 2^B
+    RQA_HD uint32_t ge(int c) const { return (c & H) ? (top & lo[c & (H - 1)]) : (top | lo[c & (H - 1)]); }
+};
+
+// One 32-step word of one threshold.  nw: bit u set = step u not recurrent
+// (bit 0 = oldest).  run: recurrent steps carried in from earlier words, with
+// PRE added while the segment's first run is open.  emit(len) counts a closed
+// run of length 1 <= len < LCAP; slow(first) takes the run that closes at the
+// word's first non-recurrent step when it is >= LCAP long or is the
+// segment's first run (first >= PRE: its length is first - PRE, 0 included).
+template <class Emit, class Slow>
+RQA_HD void consume(uint32_t nw, uint32_t &run, Emit &&emit, Slow &&slow) {
+    const uint32_t lead = ffbl(nw);     // recurrent steps before the first non-recurrent one
+    const uint32_t tail = ffbh(nw);     // recurrent steps after the last non-recurrent one
+    const uint32_t first = run + lead;  // the carried run, closed by that first step
+    const bool closes = nw != 0u;
+    if (closes && first - 1u < (uint32_t)(LCAP - 1)) emit(first);
+    if (closes && first >= (uint32_t)LCAP) slow(first);
+    // runs strictly inside the word: recurrent steps between the first and
+    // last non-recurrent ones (for nw == 0 both shifts are by 31 and mid is 0)
+    uint32_t mid = ~nw & (0xFFFFFFFFu << (lead & 31u)) & (0xFFFFFFFFu >> (tail & 31u));
+    while (mid) {
+        const uint32_t lo = ffbl(mid);
+        emit(ffbl(nw >> lo));           // ends at the next non-recurrent step
+        mid &= mid + (1u << lo);        // clear that run
+    }
+    run = closes ? tail : run + 32u;
+}
+
+// Segment end: (prefix run, suffix run) for the stitch.  A segment whose
+// first run never closed is recurrent throughout: prefix = suffix = length.
+struct SegEnds {
+    uint32_t pre, suf;
+    bool pre_open;
+};
+RQA_HD SegEnds seg_ends(uint32_t run) {
+    if (run >= PRE) return {run - PRE, run - PRE, true};
+    return {0u, run, false};
+}
+
+}  // namespace rqa

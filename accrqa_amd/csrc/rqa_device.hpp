@@ -21,17 +21,20 @@
 //             diagonal and 1 on it (rqa_CPU_RR_metric_ref_parallel,
 //             src/AccRQA_CPU_function.cpp:347-378, counts the full matrix).
 //
-// Each lane keeps one 32-bit word per threshold: bit (31-u) is the decision
-// of step u of the current 32-step block.  Runs inside a word are found with
-// count-leading/trailing-zero bit tricks; the open run is carried to the next
-// word in a register.  A sequence is cut into segments processed
-// independently; each segment reports (prefix run, suffix run) and
+// Per 32-step block a lane packs one threshold rank per step into eight
+// registers, turns them into bit planes and, through a comparator tree, into
+// one "not recurrent" word per threshold (bit u = step u, bit 0 oldest); runs
+// are read off each word with find-first-bit tricks and the open run is
+// carried to the next word (rqa_bits.hpp).  A sequence is cut into segments
+// processed independently; each segment reports (prefix run, suffix run) and
 // k_stitch joins them in order.  Histogram bins below LCAP are counted in a
 // workgroup-private LDS histogram and merged with one global atomic per
 // non-zero bin; longer runs go straight to global u64 atomics.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include "rqa_bits.hpp"
 
 #pragma clang fp contract(off)
 
@@ -41,16 +44,17 @@ constexpr int WAVE = 64;
 constexpr int BLK = 32;            // steps per decision word
 constexpr int WG = 256;            // threads per workgroup (4 waves)
 constexpr int WAVES = WG / WAVE;
-constexpr int LCAP = 128;          // run lengths < LCAP use the LDS histogram
-constexpr int DET_WIN = 1024;      // LDS samples per wave for the k_det lane window
+#ifndef RQA_WPE_LAM
+#define RQA_WPE_LAM
+#endif
+#ifndef RQA_WPE_MX
+#define RQA_WPE_MX
+#endif
+constexpr int DET_WIN = 992;       // LDS samples per wave for the k_det lane window
 
 template <typename F, int TG>
 struct Bounds {
     F s[TG];
-};
-
-struct Counts {
-    uint32_t c[2];
 };
 
 // One embedding dimension of the distance, in the reference's order.
@@ -81,15 +85,119 @@ __device__ __forceinline__ uint32_t opaque(uint32_t v) {
     return v;
 }
 
-// Histogram row t starts at gh + t*(hmax+1); a run longer than the sequence
-// would be an engine bug, and is dropped rather than written out of bounds
-// (the parity tests would then fail on the missing count).
-__device__ __forceinline__ void emit(uint32_t len, uint32_t *lh, unsigned long long *gh, uint32_t t, uint32_t hmax) {
-    if (len < (uint32_t)LCAP)
-        lds_inc(&lh[len]);
-    else if (len <= hmax)
-        glob_add(&gh[(size_t)opaque(t) * (hmax + 1) + len], 1ull);
+// LDS traffic between the lanes of one wave only.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
+
+__device__ __forceinline__ int wave_id() {
+    return __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
+}
+
+// ---------------------------------------------------------------- line scan
+// Run-length histogram of one workgroup, in three tiers: lengths below LCAP
+// in LDS; lengths LCAP .. PCAP-1 in the workgroup's own slice of a global
+// scratch histogram, counted with workgroup-scope atomics (no other
+// workgroup touches the slice, so the adds stay in the XCD's L2 without
+// contention); longer runs with device-scope atomics straight into the
+// result.  The slice is zeroed on entry and added to the result on exit.
+constexpr int PCAP = 2048;
+constexpr int PBINS = PCAP - LCAP;   // private bins per threshold
+
+struct Sink {
+    uint32_t *lh;                  // LDS: row t at t * LCAP
+    uint32_t *priv;                // workgroup slice: row t at t * PBINS, bin len - LCAP
+    unsigned long long *hist;      // result: row t at t * (nc + 1)
+    int64_t nc;
+};
+
+__device__ __forceinline__ void wg_add(uint32_t *p) {
+    __hip_atomic_fetch_add(p, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+template <int TG>
+__device__ __forceinline__ uint32_t *priv_slice(uint32_t *priv_base) {
+    return priv_base + (size_t)blockIdx.x * TG * PBINS;
+}
+
+template <int TG>
+__device__ __forceinline__ void priv_zero(uint32_t *priv, int nt, int nthreads) {
+    for (int e = threadIdx.x; e < nt * PBINS; e += nthreads) priv[e] = 0u;
+}
+
+// Adds the workgroup's LDS and private bins to the result (after a barrier).
+template <int TG>
+__device__ __forceinline__ void flush_hist(const uint32_t *lh, uint32_t *priv, int nt, unsigned long long *hist,
+                                           int64_t nc, int nthreads) {
+    for (int e = threadIdx.x; e < TG * LCAP; e += nthreads) {
+        const uint32_t v = lh[e];
+        const int t = e / LCAP;
+        if (v && t < nt) glob_add(&hist[(size_t)t * (nc + 1) + (e - t * LCAP)], v);
+    }
+    for (int e = threadIdx.x; e < nt * PBINS; e += nthreads) {
+        const uint32_t v = __hip_atomic_load(&priv[e], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const int t = e / PBINS;
+        const int64_t len = LCAP + (e - t * PBINS);
+        if (v && len <= nc) glob_add(&hist[(size_t)t * (nc + 1) + len], v);
+    }
+}
+
+// The slow path of consume(): the segment's first run goes to the carry slot
+// (for the stitch), runs of LCAP or more to the private or result bins.  A
+// run longer than the sequence would be an engine bug and is dropped rather
+// than written out of bounds (the parity tests would then fail on the count).
+__device__ __forceinline__ void slow_run(uint32_t first, uint2 *slot, const Sink &K, uint32_t t) {
+    if (first >= PRE)
+        slot->x = first - PRE;
+    else if (first < (uint32_t)PCAP)
+        wg_add(&K.priv[(size_t)t * PBINS + (first - LCAP)]);
+    else if (first <= (uint64_t)K.nc)
+        glob_add(&K.hist[(size_t)t * (K.nc + 1) + first], 1ull);
+}
+
+__device__ __forceinline__ void close_run(uint2 *slot, uint32_t run) {
+    const SegEnds e = seg_ends(run);
+    if (e.pre_open)
+        *slot = make_uint2(e.pre, e.suf);
+    else
+        slot->y = e.suf;
+}
+
+// All thresholds of one 32-step block of one lane.  nw(t) = (ge(t+1) & keep)
+// | force: keep clears steps that count as recurrent padding, force sets
+// steps that decide 0.  Histogram row t starts at hist + t*(nc+1), the
+// segment's carry slot of threshold t at cslot + t*cstride.
+template <int TG>
+__device__ __forceinline__ void scan_words(const uint32_t (&R)[8], uint32_t keep, uint32_t force, uint32_t (&run)[TG],
+                                           int nt, const Sink &K, uint2 *cslot, size_t cstride) {
+    constexpr int B = rank_bits<TG>();
+    uint32_t Pl[B];
+    rank_planes<B>(R, Pl);
+    const GeTree<B> G(Pl);
+#pragma unroll
+    for (int t = 0; t < TG; t++) {
+        if (t < nt) {
+            const uint32_t nw = (G.ge(t + 1) & keep) | force;
+            uint32_t *lht = K.lh + t * LCAP;
+            consume(
+                nw, run[t], [&](uint32_t len) { lds_inc(&lht[len]); },
+                [&](uint32_t first) {
+                    const uint32_t tt = opaque(t);
+                    slow_run(first, cslot + (size_t)tt * cstride, K, tt);
+                });
+        }
+    }
+}
+
+template <int TG>
+__device__ __forceinline__ void close_segment(const uint32_t (&run)[TG], int nt, uint2 *cslot, size_t cstride) {
+#pragma unroll
+    for (int t = 0; t < TG; t++)
+        if (t < nt) close_run(&cslot[(size_t)opaque(t) * cstride], run[t]);
+}
+
 
 // ---------------------------------------------------------------- threshold ranks
 // All thresholds of a group are decided through one integer per pair, its
@@ -99,11 +207,6 @@ __device__ __forceinline__ void emit(uint32_t len, uint32_t *lh, unsigned long l
 // stored with the bucket; a bucket with several bounds sends the wave to a
 // direct count).  Keys are the IEEE bit patterns, which order non-negative
 // values exactly; NaN maps to +inf, i.e. rank T, never recurrent.
-//
-// Ranks of 32 steps are packed one per byte (register q holds steps q, q+8,
-// q+16, q+24), and one subtraction per register yields four decisions:
-// byte 0x80 + t - r has its top bit set iff r <= t.  Shifting register q's
-// top bits down by 7-q puts step u at bit u of the decision word.
 constexpr int NB_MAX = 2048;   // bucket table entries (LDS)
 
 struct LutParams {
@@ -137,27 +240,28 @@ __device__ __forceinline__ uint32_t khi(unsigned long long k) { return (uint32_t
 template <typename F>
 __device__ __forceinline__ uint32_t rank_of(F s, const LutEntry<F> *lut, const LutParams &P, uint32_t &flags) {
     const auto key = skey(s);
-    int b = (int)(khi(key) - P.base) >> P.sh;
-    b = b < 0 ? 0 : (b > P.nb - 1 ? P.nb - 1 : b);
+    const int b = max(min((int)(khi(key) - P.base) >> P.sh, P.nb - 1), 0);
     const LutEntry<F> e = lut[b];
     flags |= e.info;
     return (e.info & 0xFFu) + (key > e.next ? 1u : 0u);
 }
 
-// Direct count against the sorted bounds (buckets holding several bounds).
+// Direct count against the sorted bounds (buckets holding several bounds);
+// bnd is the workgroup's LDS copy of the group's bounds.
 template <typename F, int TG>
-__device__ __forceinline__ uint32_t rank_direct(F s, const Bounds<F, TG> &S, int nt) {
+__device__ __forceinline__ uint32_t rank_direct(F s, const F *bnd, int nt) {
     const auto key = skey(s);
     uint32_t r = 0;
 #pragma unroll
     for (int t = 0; t < TG; t++)
-        if (t < nt) r += key > skey(S.s[t]) ? 1u : 0u;
+        if (t < nt) r += key > skey(bnd[t]) ? 1u : 0u;
     return r;
 }
 
+// Ranks of 32 steps, one per byte: register q holds steps q, q+8, q+16, q+24.
 template <typename F, int TG>
 __device__ __forceinline__ void pack_ranks(const F (&s)[BLK], uint32_t (&R)[8], const LutEntry<F> *lut,
-                                           const LutParams &P, const Bounds<F, TG> &S, int nt) {
+                                           const LutParams &P, const F *bnd, int nt) {
     uint32_t flags = 0;
 #pragma unroll
     for (int q = 0; q < 8; q++) R[q] = 0;
@@ -167,65 +271,13 @@ __device__ __forceinline__ void pack_ranks(const F (&s)[BLK], uint32_t (&R)[8], 
 #pragma unroll
         for (int q = 0; q < 8; q++) R[q] = 0;
 #pragma unroll
-        for (int u = 0; u < BLK; u++) R[u & 7] |= rank_direct(s[u], S, nt) << (8 * (u >> 3));
+        for (int u = 0; u < BLK; u++) R[u & 7] |= rank_direct<F, TG>(s[u], bnd, nt) << (8 * (u >> 3));
     }
-}
-
-// Decision word of threshold t: bit u = [rank of step u <= t].
-__device__ __forceinline__ uint32_t swar_word(const uint32_t (&R)[8], uint32_t t) {
-    const uint32_t K = 0x80808080u + t * 0x01010101u;
-    uint32_t w = 0;
-#pragma unroll
-    for (int q = 0; q < 8; q++) w |= ((K - R[q]) >> (7 - q)) & (0x01010101u << q);
-    return w;
 }
 
 template <typename F>
 __device__ __forceinline__ void load_lut(LutEntry<F> *dst, const LutEntry<F> *src, int nb) {
     for (int e = threadIdx.x; e < nb; e += WG) dst[e] = src[e];
-}
-
-// Consume one 32-step word, bit u = step u (bit 0 oldest).  run: open run
-// entering the word.  Bit tb of popen is set until the segment's first zero;
-// the run it closes is the segment's prefix run, written to *pre (the stitch
-// joins it with the previous segment) instead of being counted here.
-__device__ __forceinline__ void consume_word(uint32_t w, uint32_t &run, uint32_t &popen, uint32_t t, uint2 *cslot,
-                                             size_t cstride, uint32_t *lh, unsigned long long *gh, uint32_t hmax) {
-    if (w == 0xFFFFFFFFu) {
-        run += BLK;
-        return;
-    }
-    const uint32_t tb = 1u << t;
-    const uint32_t lead = __builtin_ctz(~w);     // ones from the oldest step up to the first zero
-    const uint32_t first = run + lead;
-    if (popen & tb) {
-        cslot[(size_t)opaque(t) * cstride].x = first;
-        popen &= ~tb;
-    } else if (first) {
-        emit(first, lh, gh, t, hmax);
-    }
-    const uint32_t tail = __clz(~w);             // ones from the newest step down to the last zero
-    uint32_t mid = w & (0xFFFFFFFFu << lead) & (0xFFFFFFFFu >> tail);
-    while (mid) {
-        const uint32_t lo = __builtin_ctz(mid);
-        const uint32_t len = __builtin_ctz(~(mid >> lo));
-        emit(len, lh, gh, t, hmax);
-        mid &= ~(((1u << len) - 1u) << lo);
-    }
-    run = tail;
-}
-
-// Segment end: (prefix, suffix) of the segment for the stitch.  A segment
-// whose prefix never closed is all ones: prefix = suffix = its length.
-__device__ __forceinline__ void close_segment(uint2 *slot, uint32_t run, uint32_t popen, uint32_t tb) {
-    if (popen & tb)
-        *slot = make_uint2(run, run);
-    else
-        slot->y = run;
-}
-
-__device__ __forceinline__ int wave_id() {
-    return __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
 }
 
 // ---------------------------------------------------------------- k_lam
@@ -234,21 +286,27 @@ __device__ __forceinline__ int wave_id() {
 // The row window x[i0 + u + k*tau] (u < 32) is uniform across the wave and
 // comes through scalar loads; x[j + k*tau] is one vector load per k and block.
 template <typename F, int NORM, int TG>
-__global__ __launch_bounds__(WG) void k_lam(const F *__restrict__ x, int64_t nc, int tau, int m, Bounds<F, TG> S,
+__global__ __launch_bounds__(WG) RQA_WPE_LAM void k_lam(const F *__restrict__ x, int64_t nc, int tau, int m, Bounds<F, TG> S,
                                             int nt, const LutEntry<F> *__restrict__ lut, LutParams P, int seg_len,
                                             int nseg, unsigned long long *__restrict__ hist,
-                                            uint2 *__restrict__ carry) {
+                                            uint2 *__restrict__ carry, uint32_t *__restrict__ priv_base) {
     __shared__ uint32_t lh[TG * LCAP];
     __shared__ LutEntry<F> lut_s[NB_MAX];
+    __shared__ F bnd_s[TG];
     for (int e = threadIdx.x; e < TG * LCAP; e += WG) lh[e] = 0;
     load_lut(lut_s, lut, P.nb);
+    if (threadIdx.x < TG) bnd_s[threadIdx.x] = S.s[threadIdx.x];
+    priv_zero<TG>(priv_slice<TG>(priv_base), nt, WG);
     __syncthreads();
+    uint32_t *priv = priv_slice<TG>(priv_base);
+    const Sink K{lh, priv, hist, nc};
 
     const int lane = threadIdx.x & (WAVE - 1);
     const int wave = wave_id();
     const int64_t nbands = (nc + WAVE - 1) / WAVE;
     const int64_t nitems = nbands * nseg;
     const size_t cstride = (size_t)nc * nseg;   // carry stride between thresholds
+    constexpr int MJ = sizeof(F) == 4 ? 16 : 8;  // column samples kept in registers
 
     for (int64_t item = (int64_t)blockIdx.x * WAVES + wave; item < nitems; item += (int64_t)gridDim.x * WAVES) {
         const int64_t band = item / nseg;
@@ -261,49 +319,46 @@ __global__ __launch_bounds__(WG) void k_lam(const F *__restrict__ x, int64_t nc,
 
         uint32_t run[TG];
 #pragma unroll
-        for (int t = 0; t < TG; t++) run[t] = 0;
-        uint32_t popen = jv ? 0xFFFFFFFFu : 0u;
+        for (int t = 0; t < TG; t++) run[t] = jv ? PRE : 0u;
+        // the column's samples x[j + k tau] are the same for every block of the item
+        F xjc[MJ];
+#pragma unroll
+        for (int k = 0; k < MJ; k++) xjc[k] = x[j + (int64_t)(k < m ? k : 0) * tau];
 
         for (int64_t i0 = ib; i0 < ie; i0 += BLK) {
             const int nb = ie - i0 < BLK ? (int)(ie - i0) : BLK;
             F s[BLK];
 #pragma unroll
             for (int u = 0; u < BLK; u++) s[u] = F(0);
-            F xj = x[j];
-            for (int k = 0; k < m; k++) {
-                // next dimension's column sample, loaded while this one computes
-                const F xjn = x[j + (int64_t)(k + 1 < m ? k + 1 : k) * tau];
+#pragma unroll
+            for (int k = 0; k < MJ; k++) {
+                if (k < m) {
+                    const F *xr = x + i0 + (int64_t)k * tau;
+#pragma unroll
+                    for (int u = 0; u < BLK; u++) s[u] = dim_step<NORM>(s[u], xr[u], xjc[k]);
+                }
+            }
+            for (int k = MJ; k < m; k++) {
+                const F xj = x[j + (int64_t)k * tau];
                 const F *xr = x + i0 + (int64_t)k * tau;
 #pragma unroll
                 for (int u = 0; u < BLK; u++) s[u] = dim_step<NORM>(s[u], xr[u], xj);
-                xj = xjn;
             }
             uint32_t R[8];
-            pack_ranks(s, R, lut_s, P, S, nt);
-            const uint32_t fill = nb == BLK ? 0u : (0xFFFFFFFFu << nb);   // rows past the segment
+            pack_ranks<F, TG>(s, R, lut_s, P, bnd_s, nt);
+            // rows past the segment count as recurrent padding, taken back below
+            const uint32_t fill = nb == BLK ? 0u : (0xFFFFFFFFu << nb);
+            scan_words<TG>(R, jv ? ~fill : 0u, jv ? 0u : 0xFFFFFFFFu, run, nt, K, cslot, cstride);
+            if (fill && jv) {
 #pragma unroll
-            for (int t = 0; t < TG; t++) {
-                if (t < nt) {
-                    uint32_t w = swar_word(R, t);
-                    w = jv ? (w | fill) : 0u;
-                    consume_word(w, run[t], popen, t, cslot, cstride, lh + t * LCAP, hist, (uint32_t)nc);
-                    if (fill && jv) run[t] -= (uint32_t)(BLK - nb);
-                }
+                for (int t = 0; t < TG; t++) run[t] -= (uint32_t)(BLK - nb);
             }
         }
-        if (jv) {
-#pragma unroll
-            for (int t = 0; t < TG; t++)
-                if (t < nt) close_segment(&cslot[(size_t)opaque(t) * cstride], run[t], popen, 1u << t);
-        }
+        if (jv) close_segment<TG>(run, nt, cslot, cstride);
     }
 
     __syncthreads();
-    for (int e = threadIdx.x; e < TG * LCAP; e += WG) {
-        const uint32_t v = lh[e];
-        const int t = e / LCAP;
-        if (v && t < nt) glob_add(&hist[(size_t)t * (nc + 1) + (e - t * LCAP)], v);
-    }
+    flush_hist<TG>(lh, priv, nt, hist, nc, WG);
 }
 
 // ---------------------------------------------------------------- k_det
@@ -316,13 +371,19 @@ template <typename F, int NORM, int TG>
 __global__ __launch_bounds__(WG) void k_det(const F *__restrict__ x, int64_t nc, int tau, int m, Bounds<F, TG> S,
                                             int nt, const LutEntry<F> *__restrict__ lut, LutParams P, int seg_len,
                                             int nseg, const int2 *__restrict__ items, int64_t nitems,
-                                            unsigned long long *__restrict__ hist, uint2 *__restrict__ carry) {
+                                            unsigned long long *__restrict__ hist, uint2 *__restrict__ carry,
+                                            uint32_t *__restrict__ priv_base) {
     __shared__ uint32_t lh[TG * LCAP];
     __shared__ LutEntry<F> lut_s[NB_MAX];
     __shared__ F win[WAVES][DET_WIN];
+    __shared__ F bnd_s[TG];
     for (int e = threadIdx.x; e < TG * LCAP; e += WG) lh[e] = 0;
     load_lut(lut_s, lut, P.nb);
+    if (threadIdx.x < TG) bnd_s[threadIdx.x] = S.s[threadIdx.x];
+    priv_zero<TG>(priv_slice<TG>(priv_base), nt, WG);
     __syncthreads();
+    uint32_t *priv = priv_slice<TG>(priv_base);
+    const Sink K{lh, priv, hist, nc};
 
     const int lane = threadIdx.x & (WAVE - 1);
     const int wave = wave_id();
@@ -347,8 +408,7 @@ __global__ __launch_bounds__(WG) void k_det(const F *__restrict__ x, int64_t nc,
 
         uint32_t run[TG];
 #pragma unroll
-        for (int t = 0; t < TG; t++) run[t] = 0;
-        uint32_t popen = rv ? 0xFFFFFFFFu : 0u;
+        for (int t = 0; t < TG; t++) run[t] = rv ? PRE : 0u;
 
         for (int64_t s0 = sb; s0 < se; s0 += BLK) {
             F s[BLK];
@@ -359,49 +419,31 @@ __global__ __launch_bounds__(WG) void k_det(const F *__restrict__ x, int64_t nc,
                 const int64_t base = s0 + r0 + (int64_t)k0 * tau;
                 const int span = BLK + WAVE - 1 + (k1 - 1 - k0) * tau;
                 for (int e = lane; e < span; e += WAVE) wn[e] = x[base + e];
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                wave_sync();
                 for (int k = k0; k < k1; k++) {
                     const F *xr = x + s0 + (int64_t)k * tau;
                     const F *wl = wn + lane + (k - k0) * tau;
 #pragma unroll
                     for (int u = 0; u < BLK; u++) s[u] = dim_step<NORM>(s[u], xr[u], wl[u]);
                 }
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                wave_sync();
             }
             uint32_t R[8];
-            pack_ranks(s, R, lut_s, P, S, nt);
+            pack_ranks<F, TG>(s, R, lut_s, P, bnd_s, nt);
             // bit u valid iff s0+u < dlen; invalid steps decide 0
             const int64_t nv64 = dlen - s0;
             const int nvalid = nv64 <= 0 ? 0 : (nv64 >= BLK ? BLK : (int)nv64);
             const uint32_t vmask = nvalid >= BLK ? 0xFFFFFFFFu : ((1u << nvalid) - 1u);
-#pragma unroll
-            for (int t = 0; t < TG; t++) {
-                if (t < nt) {
-                    const uint32_t w = swar_word(R, t) & vmask;
-                    consume_word(w, run[t], popen, t, cslot, cstride, lh + t * LCAP, hist, (uint32_t)nc);
-                }
-            }
+            scan_words<TG>(R, 0xFFFFFFFFu, ~vmask, run, nt, K, cslot, cstride);
         }
         // Steps past the strip's end decide 0 for every lane, so a segment cut
         // short by the end of the diagonals reports the same (prefix, suffix)
         // as one padded with zeros.
-        if (rv) {
-#pragma unroll
-            for (int t = 0; t < TG; t++)
-                if (t < nt) close_segment(&cslot[(size_t)opaque(t) * cstride], run[t], popen, 1u << t);
-        }
+        if (rv) close_segment<TG>(run, nt, cslot, cstride);
     }
 
     __syncthreads();
-    for (int e = threadIdx.x; e < TG * LCAP; e += WG) {
-        const uint32_t v = lh[e];
-        const int t = e / LCAP;
-        if (v && t < nt) glob_add(&hist[(size_t)t * (nc + 1) + (e - t * LCAP)], v);
-    }
+    flush_hist<TG>(lh, priv, nt, hist, nc, WG);
 }
 
 // ---------------------------------------------------------------- k_rr
@@ -414,11 +456,14 @@ __global__ __launch_bounds__(WG) void k_rr(const F *__restrict__ x, int64_t nc, 
                                            unsigned long long *__restrict__ counts) {
     __shared__ unsigned long long acc[TG];
     __shared__ LutEntry<F> lut_s[NB_MAX];
+    __shared__ F bnd_s[TG];
     for (int e = threadIdx.x; e < TG; e += WG) acc[e] = 0;
     load_lut(lut_s, lut, P.nb);
+    if (threadIdx.x < TG) bnd_s[threadIdx.x] = S.s[threadIdx.x];
     __syncthreads();
     const int lane = threadIdx.x & (WAVE - 1);
     const int wave = wave_id();
+    constexpr int B = rank_bits<TG>();
 
     for (int64_t item = (int64_t)blockIdx.x * WAVES + wave; item < nitems; item += (int64_t)gridDim.x * WAVES) {
         const int2 it = items[item];
@@ -446,7 +491,10 @@ __global__ __launch_bounds__(WG) void k_rr(const F *__restrict__ x, int64_t nc, 
                 xj = xjn;
             }
             uint32_t R[8];
-            pack_ranks(s, R, lut_s, P, S, nt);
+            pack_ranks<F, TG>(s, R, lut_s, P, bnd_s, nt);
+            uint32_t Pl[B];
+            rank_planes<B>(R, Pl);
+            const GeTree<B> G(Pl);
             // rows i0+u with u < nrow are in the segment; weight 2 for i < j, 1 for i == j
             const int64_t nr64 = ie - i0;
             const uint32_t mrow = nr64 >= BLK ? 0xFFFFFFFFu : ((1u << nr64) - 1u);
@@ -456,7 +504,7 @@ __global__ __launch_bounds__(WG) void k_rr(const F *__restrict__ x, int64_t nc, 
 #pragma unroll
             for (int t = 0; t < TG; t++) {
                 if (t < nt) {
-                    const uint32_t w = swar_word(R, t) & mrow;
+                    const uint32_t w = ~G.ge(t + 1) & mrow;
                     c2[t] += __popc(w & mup);
                     c1[t] += __popc(w & mdg);
                 }
@@ -486,11 +534,15 @@ __global__ __launch_bounds__(WG) void k_rr(const F *__restrict__ x, int64_t nc, 
 // product gamma_m (n_i+n_j)/2 doubled; the final add and fma 3u; 10u spare).
 // A byte table over the bits of s~ gives the rank r = #{t : B_t < s} exactly
 // for every bucket that has no bound within E_max = max E_ij of its value
-// range; pairs in the other buckets are re-decided in exact order on VALU.
-// Inputs whose norms are not finite or exceed 1e30 flag every bucket, so the
-// whole matrix is then decided exactly.
-constexpr int NBX = 32768;   // byte buckets of the MFMA rank table
-constexpr int MXW = 128;     // LDS operand window: 32 + (m-1)*tau samples at most
+// range; a bucket with one bound within reach names that bound, and its pairs
+// are decided against it with their own E_ij (second stage); pairs still
+// within E_ij of a bound are re-decided in exact order on VALU.  Inputs whose
+// norms are not finite or exceed 1e30 flag every bucket, so the whole matrix
+// is then decided exactly.
+constexpr int NBX = 16384;   // byte buckets of the MFMA rank table (LDS)
+constexpr int MXW = 128;     // LDS row window: 32 + (m-1)*tau samples at most
+constexpr int NBND = 64;     // bound slots reachable by a 6-bit table index
+constexpr int QCAP = 128;    // recheck queue entries per wave and round
 
 struct MxParams {
     uint32_t base;   // key bits of bucket 0 (half the smallest bound)
@@ -521,8 +573,8 @@ __global__ void k_norms(const float *__restrict__ x, int64_t nc, int tau, int m,
 }
 
 // Byte table.  Bucket without any bound within E_max of its value range:
-// bits 0-5 = rank (number of bounds below it), bit 7 clear.  Otherwise bit 7
-// set and bits 0-5 = index of the one bound within reach (bit 6 set if there
+// bits 0-5 = rank (number of bounds below it), bits 6-7 clear.  Otherwise bit
+// 7 set and bits 0-5 = index of the one bound within reach (bit 6 set if there
 // are several): such pairs are checked against that bound with their own
 // E_ij, and only the truly ambiguous ones are re-decided in exact order.
 template <int TG>
@@ -558,27 +610,21 @@ __global__ void k_mx_lut(Bounds<float, TG> S, int nt, MxParams P, const unsigned
     lut[b] = (uint8_t)e;
 }
 
-// Second stage for a flagged pair: rank if the one nearby bound is farther
-// than E_ij = cE * (n_i + n_j), else flag for exact re-decision.
-__device__ __forceinline__ uint32_t mx_resolve(uint32_t ent, float st, float nsum, const float *bnd, float cE,
-                                               uint32_t &amb) {
-    if (!(ent & 0x80u)) return ent & 0x3Fu;
-    if (ent & 0x40u) {
-        amb = 1u;
-        return 0u;
-    }
-    const uint32_t idx = ent & 0x3Fu;
-    const float d = st - bnd[idx];
-    const float E = fmaf(cE, nsum, 1e-30f) * 1.0001f;
-    if (fabsf(d) > E) return idx + (d > 0.0f ? 1u : 0u);
-    amb = 1u;
-    return 0u;
-}
-
 __device__ __forceinline__ uint32_t mx_key_bucket(float st, const MxParams &P) {
     const uint32_t key = __float_as_uint(st);   // st >= +0, finite under the norm guard
-    int b = (int)(key - P.base) >> P.sh;
-    return (uint32_t)(b < 0 ? 0 : (b > P.nb - 1 ? P.nb - 1 : b));
+    return (uint32_t)max(min((int)(key - P.base) >> P.sh, P.nb - 1), 0);
+}
+
+// Second stage of one flagged element (table byte with bit 7 set): the one
+// bound within reach decides unless it lies within E_ij of s~, or several
+// bounds were within reach; amb is set for those pairs.
+__device__ __forceinline__ uint32_t mx_second(uint32_t byte, float st, float sum, const float *bnd, float cE,
+                                              bool &amb) {
+    const uint32_t idx = byte & 0x3Fu;
+    const float d = st - bnd[idx];
+    const float E = fmaf(cE, sum, 1e-30f) * 1.0001f;
+    amb = (byte & 0x40u) != 0u || !(fabsf(d) > E);
+    return idx + (d > 0.0f ? 1u : 0u);
 }
 
 // Exact-order fp32 squared distance of one pair (the reference's sum).
@@ -605,13 +651,6 @@ __device__ __forceinline__ float exact_sqdist(const float *__restrict__ x, int64
     return s;
 }
 
-template <int TG>
-__device__ __forceinline__ uint32_t exact_rank(float s, const Bounds<float, TG> &S, int nt) {
-    return rank_direct<float, TG>(s, S, nt);
-}
-
-typedef float floatx16 __attribute__((ext_vector_type(16)));
-
 // Wave-wide exclusive prefix sum of a small per-lane count.
 __device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, uint32_t &total) {
     const int lane = threadIdx.x & (WAVE - 1);
@@ -625,238 +664,303 @@ __device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, uint32_t &total) 
     return inc - v;
 }
 
+// Exact re-decision of one wave's ambiguous elements.  Bit b of fm flags an
+// element of this lane; pair(owner lane, b, i, j) names its matrix pair and
+// put(owner lane, b, rank) stores the exact rank where the owner will read it
+// (LDS).  The flagged elements go through a queue so that every lane of the
+// wave takes a share of the exact sums.
+template <int TG, class Pair, class Put>
+__device__ __forceinline__ void mx_recheck(uint32_t fm, uint16_t *queue, const float *__restrict__ x, int tau, int m,
+                                           const float *bnd, int nt, Pair &&pair, Put &&put) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    uint32_t total;
+    const uint32_t off = wave_excl_scan(__popc(fm), total);
+    for (uint32_t base = 0; base < total; base += QCAP) {
+        uint32_t f = fm, o = off;
+        while (f) {
+            const uint32_t b = __builtin_ctz(f);
+            f &= f - 1u;
+            if (o >= base && o < base + QCAP) queue[o - base] = (uint16_t)((lane << 5) | b);
+            o++;
+        }
+        wave_sync();
+        const uint32_t nq = total - base < (uint32_t)QCAP ? total - base : (uint32_t)QCAP;
+        for (uint32_t qi = lane; qi < nq; qi += WAVE) {
+            const uint32_t it = queue[qi];
+            const uint32_t ow = it >> 5, b = it & 31u;
+            int64_t i, j;
+            pair(ow, b, i, j);
+            put(ow, b, rank_direct<float, TG>(exact_sqdist(x, i, j, tau, m), bnd, nt));
+        }
+        wave_sync();
+    }
+}
+
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+// Gram tile G[a][b] = x_(i0+a) . x_(c0+b), a, b < 32, on MFMA.  rw / cw hold
+// x[i0 ..] and x[c0 ..] (LDS windows); lane l supplies A[row l%32][k l/32]
+// and B[k l/32][col l%32], i.e. dimension 2 k2 + h.  For odd m the last step
+// of half 1 reads dimension m-1 again and zeroes its row operand.
+__device__ __forceinline__ floatx16 gram_tile(const float *rw, const float *cw, int ob, int omax, int m2, int tau) {
+    floatx16 acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    // eight steps at a time: all operand loads in flight before the MFMA chain
+    int k2 = 0;
+    for (; k2 + 8 <= m2; k2 += 8) {
+        float a[8], b[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int oraw = ob + 2 * (k2 + u) * tau;
+            const int o = min(oraw, omax);
+            const float av = rw[o];
+            a[u] = oraw <= omax ? av : 0.f;
+            b[u] = cw[o];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[u], b[u], acc, 0, 0, 0);
+    }
+    for (; k2 + 2 <= m2; k2 += 2) {
+        float a[2], b[2];
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            const int oraw = ob + 2 * (k2 + u) * tau;
+            const int o = min(oraw, omax);
+            const float av = rw[o];
+            a[u] = oraw <= omax ? av : 0.f;
+            b[u] = cw[o];
+        }
+        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[0], b[0], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[1], b[1], acc, 0, 0, 0);
+    }
+    if (k2 < m2) {
+        const int oraw = ob + 2 * k2 * tau;
+        const int o = min(oraw, omax);
+        const float av = rw[o];
+        const float a = oraw <= omax ? av : 0.f;
+        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, cw[o], acc, 0, 0, 0);
+    }
+    return acc;
+}
+
+// Ranks of one Gram tile's 16 elements of this lane, packed one per byte in
+// the layout of the tile (element e -> byte e>>2 of Rq[e&3]).  First stage:
+// table bytes, flags included.  Second stage, for flagged bytes only: the
+// bound within reach decides with the pair's own E_ij; fm gets the elements
+// that still need the exact sum.  ni[e]: norm of element e's row, nj: norm of
+// the lane's column.
+__device__ __forceinline__ void tile_ranks(const floatx16 &acc, const float (&ni)[16], float nj, const uint8_t *lut_s,
+                                           const MxParams &P, const float *bnd_s, uint32_t (&Rq)[4], uint32_t &fm) {
+#pragma unroll
+    for (int c = 0; c < 4; c++) Rq[c] = 0u;
+#pragma unroll
+    for (int e = 0; e < 16; e++) {
+        const float st = fmaxf(fmaf(-2.0f, acc[e], ni[e] + nj), 0.0f);
+        Rq[e & 3] |= (uint32_t)lut_s[mx_key_bucket(st, P)] << (8 * (e >> 2));
+    }
+    fm = 0u;
+    if (__any(((Rq[0] | Rq[1] | Rq[2] | Rq[3]) & 0x80808080u) != 0u)) {
+#pragma unroll
+        for (int e = 0; e < 16; e++) {
+            const int c = e & 3, sh = 8 * (e >> 2);
+            if (__any(((Rq[c] >> (sh + 7)) & 1u) != 0u)) {
+                const uint32_t byte = (Rq[c] >> sh) & 0xFFu;
+                const float sum = ni[e] + nj;
+                const float st = fmaxf(fmaf(-2.0f, acc[e], sum), 0.0f);
+                bool amb;
+                const uint32_t rk = mx_second(byte, st, sum, bnd_s, P.cE, amb);
+                if (byte & 0x80u) {
+                    Rq[c] = (Rq[c] & ~(0xFFu << sh)) | (rk << sh);
+                    if (amb) fm |= 1u << e;
+                }
+            }
+        }
+    }
+}
+
+// Norms of the rows of this lane's tile elements: i0 + 8(e>>2) + 4h + (e&3).
+__device__ __forceinline__ void tile_row_norms(const float *__restrict__ nrm, int64_t i0, int h, float (&ni)[16]) {
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+        const float4 v = *(const float4 *)(nrm + i0 + 8 * g + 4 * h);
+        ni[4 * g + 0] = v.x;
+        ni[4 * g + 1] = v.y;
+        ni[4 * g + 2] = v.z;
+        ni[4 * g + 3] = v.w;
+    }
+}
+
 // ---------------------------------------------------------------- k_lam_mx
-// LAM pass of the MFMA form.  Work item = (band of 32 columns, row segment).
-// Lane l owns column j = j0 + l%32; the 32x32 MFMA tile gives it 16 rows of
-// that column: rows 8g + 4h + r (g, r < 4, h = l/32), i.e. exactly the rank
-// registers R[4h .. 4h+3] of the byte packing.  The two halves swap their
-// four registers, then each half runs the words of every other threshold
-// (t = 2t' + h) over all 32 rows.
+// LAM pass of the MFMA form.  Work item = (band of 64 columns, row segment).
+// Per 32-row block two 32x32 Gram tiles (columns j0..j0+31, then j0+32..j0+63)
+// come from MFMA one after the other; element e of a lane in half h is row
+// 8(e>>2) + 4h + (e&3) of column l%32 of its tile, i.e. byte e>>2 of rank
+// register (e&3) + 4h of that column.  One cross-half swap per register then
+// leaves lane l (half 0) with column j0 + l and lane l (half 1) with column
+// j0 + l, all 32 rows, and the scan is k_lam's.
 template <int TG>
 __global__ __launch_bounds__(WG) void k_lam_mx(const float *__restrict__ x, const float *__restrict__ nrm, int64_t nc,
                                                int tau, int m, Bounds<float, TG> S, int nt,
                                                const uint8_t *__restrict__ lut, MxParams P, int seg_len, int nseg,
-                                               unsigned long long *__restrict__ hist, uint2 *__restrict__ carry) {
-    constexpr int TH = (TG + 1) / 2;
-    constexpr int QCAP = 256;                      // recheck queue entries per wave and round
+                                               unsigned long long *__restrict__ hist, uint2 *__restrict__ carry,
+                                               uint32_t *__restrict__ priv_base) {
     __shared__ uint32_t lh[TG * LCAP];
     __shared__ uint8_t lut_s[NBX];
-    __shared__ uint32_t rbuf[WAVES][WAVE][4];      // packed ranks, re-read after rechecks
+    __shared__ uint32_t rbuf[WAVES][8 * WAVE];     // packed ranks during rechecks
     __shared__ uint16_t queue[WAVES][QCAP];
-    __shared__ float bnd_s[TG];
-    __shared__ float rwin[WAVES][MXW];   // x[i0 .. i0 + 31 + (m-1)tau] of the current block
-    __shared__ float cwin[WAVES][MXW];   // x[j0 .. j0 + 31 + (m-1)tau] of the current item
+    __shared__ float bnd_s[NBND];
+    __shared__ float rwin[WAVES][MXW];        // x[i0 .. i0 + 31 + (m-1)tau] of the current block
+    __shared__ float cwin[WAVES][MXW + 32];   // x[j0 .. j0 + 63 + (m-1)tau] of the current item
     for (int e = threadIdx.x; e < TG * LCAP; e += WG) lh[e] = 0;
     for (int e = threadIdx.x; e < P.nb / 4; e += WG) ((uint32_t *)lut_s)[e] = ((const uint32_t *)lut)[e];
-    if (threadIdx.x < TG) bnd_s[threadIdx.x] = S.s[threadIdx.x];
+    if (threadIdx.x < NBND) bnd_s[threadIdx.x] = (int)threadIdx.x < nt ? S.s[threadIdx.x] : __int_as_float(0x7F800000);
+    priv_zero<TG>(priv_slice<TG>(priv_base), nt, WG);
     __syncthreads();
+    uint32_t *priv = priv_slice<TG>(priv_base);
+    const Sink K{lh, priv, hist, nc};
 
     const int lane = threadIdx.x & (WAVE - 1);
     const int wave = wave_id();
     const int col = lane & 31;
     const int h = lane >> 5;
-    const int span = 32 + (m - 1) * tau;   // <= MXW (checked by the host)
-    const int64_t nbands = (nc + 31) / 32;
+    const int rspan = 32 + (m - 1) * tau;   // <= MXW (checked by the host)
+    const int cspan = 64 + (m - 1) * tau;   // <= MXW + 32
+    const int64_t nbands = (nc + WAVE - 1) / WAVE;
     const int64_t nitems = nbands * nseg;
     const size_t cstride = (size_t)nc * nseg;
     const int m2 = (m + 1) / 2;
+    const int ob = col + h * tau, omax = col + (m - 1) * tau;   // operand offsets of dimension h, m-1
+    float *rw = rwin[wave], *cw = cwin[wave];
+    uint32_t *rb = rbuf[wave];
 
     for (int64_t item = (int64_t)blockIdx.x * WAVES + wave; item < nitems; item += (int64_t)gridDim.x * WAVES) {
         const int64_t band = item / nseg;
         const int q = (int)(item - band * nseg);
-        const int64_t j0 = band * 32;
-        const int64_t j = j0 + col;
-        const bool jv = j < nc;
+        const int64_t j0 = band * WAVE;
+        const int64_t jown = j0 + lane;   // the column this lane scans (after the swap)
+        const bool jv = jown < nc;
         const int64_t ib = (int64_t)q * seg_len;
         const int64_t ie = ib + seg_len < nc ? ib + seg_len : nc;
-        const float nj = nrm[j];
-        uint2 *cslot = carry + (size_t)(jv ? j : 0) * nseg + q;
+        uint2 *cslot = carry + (size_t)(jv ? jown : 0) * nseg + q;
 
-        uint32_t run[TH];
+        uint32_t run[TG];
 #pragma unroll
-        for (int t = 0; t < TH; t++) run[t] = 0;
-        uint32_t popen = jv ? 0xFFFFFFFFu : 0u;
+        for (int t = 0; t < TG; t++) run[t] = jv ? PRE : 0u;
 
         // column window once per item; row window of the first block
-        float *rw = rwin[wave], *cw = cwin[wave];
-        for (int e = lane; e < span; e += WAVE) cw[e] = x[j0 + e];
-        float pre0 = lane < span ? x[ib + lane] : 0.f;
-        float pre1 = lane + WAVE < span ? x[ib + WAVE + lane] : 0.f;
+        for (int e = lane; e < cspan; e += WAVE) cw[e] = x[j0 + e];
+        float pre0 = lane < rspan ? x[ib + lane] : 0.f;
+        float pre1 = lane + WAVE < rspan ? x[ib + WAVE + lane] : 0.f;
 
         for (int64_t i0 = ib; i0 < ie; i0 += BLK) {
             const int nb = ie - i0 < BLK ? (int)(ie - i0) : BLK;
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            if (lane < span) rw[lane] = pre0;
-            if (lane + WAVE < span) rw[WAVE + lane] = pre1;
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            wave_sync();
+            if (lane < rspan) rw[lane] = pre0;
+            if (lane + WAVE < rspan) rw[WAVE + lane] = pre1;
+            wave_sync();
             // next block's row window, in flight while this block computes
             // (only if there is a next block: its window ends by x[N + 30])
             const bool more = i0 + BLK < ie;
-            pre0 = more && lane < span ? x[i0 + BLK + lane] : 0.f;
-            pre1 = more && lane + WAVE < span ? x[i0 + BLK + WAVE + lane] : 0.f;
-            // Gram tile G[i0 + a][j0 + b] on MFMA: lane supplies A[row=col][k=h], B[k=h][col]
-            floatx16 acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-#pragma unroll 4
-            for (int k2 = 0; k2 < m2; k2++) {
-                const int dim = 2 * k2 + h;
-                const float a = dim < m ? rw[col + dim * tau] : 0.f;
-                const float b = dim < m ? cw[col + dim * tau] : 0.f;
-                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
-            }
-            // rows of element e: i0 + 8(e>>2) + 4h + (e&3)
+            pre0 = more && lane < rspan ? x[i0 + BLK + lane] : 0.f;
+            pre1 = more && lane + WAVE < rspan ? x[i0 + BLK + WAVE + lane] : 0.f;
             float ni[16];
+            tile_row_norms(nrm, i0, h, ni);
+            // real rows of this lane's elements (rows past the segment are padding)
+            uint32_t rowm = 0;
 #pragma unroll
-            for (int g = 0; g < 4; g++) {
-                const float4 v = *(const float4 *)(nrm + i0 + 8 * g + 4 * h);
-                ni[4 * g + 0] = v.x;
-                ni[4 * g + 1] = v.y;
-                ni[4 * g + 2] = v.z;
-                ni[4 * g + 3] = v.w;
+            for (int e = 0; e < 16; e++) rowm |= ((8 * (e >> 2) + 4 * h + (e & 3)) < nb ? 1u : 0u) << e;
+            uint32_t Rr[8], fm = 0u;
+#pragma unroll
+            for (int p = 0; p < 2; p++) {
+                const floatx16 acc = gram_tile(rw, cw + 32 * p, ob, omax, m2, tau);
+                uint32_t Rq[4], f;
+                tile_ranks(acc, ni, nrm[j0 + 32 * p + col], lut_s, P, bnd_s, Rq, f);
+#pragma unroll
+                for (int c = 0; c < 4; c++) Rr[4 * p + c] = Rq[c];
+                if (j0 + 32 * p + col < nc) fm |= (f & rowm) << (16 * p);
             }
-            uint32_t Rh[4] = {0u, 0u, 0u, 0u};   // R[4h + c]: rows 4h + c + 8g in byte g
-            uint32_t fm = 0;                      // elements to re-decide exactly
+            if (__any(fm != 0u)) {
 #pragma unroll
-            for (int e = 0; e < 16; e++) {
-                const float sum = ni[e] + nj;
-                float st = fmaf(-2.0f, acc[e], sum);
-                st = fmaxf(st, 0.0f);
-                const uint32_t ent = lut_s[mx_key_bucket(st, P)];
-                uint32_t amb = 0;
-                const uint32_t rk = mx_resolve(ent, st, sum, bnd_s, P.cE, amb);
-                Rh[e & 3] |= rk << (8 * (e >> 2));
-                fm |= amb << e;
+                for (int c = 0; c < 8; c++) rb[c * WAVE + lane] = Rr[c];
+                uint8_t *rb8 = (uint8_t *)rb;
+                mx_recheck<TG>(
+                    fm, queue[wave], x, tau, m, bnd_s, nt,
+                    [&](uint32_t ow, uint32_t b, int64_t &i, int64_t &j) {
+                        const uint32_t e = b & 15u;
+                        i = i0 + 8 * (e >> 2) + 4 * (ow >> 5) + (e & 3);
+                        j = j0 + 32 * (b >> 4) + (ow & 31);
+                    },
+                    [&](uint32_t ow, uint32_t b, uint32_t r) {
+                        // byte e>>2 of register 4p + (e&3) of lane ow
+                        const uint32_t e = b & 15u, c = 4 * (b >> 4) + (e & 3);
+                        rb8[(c * WAVE + ow) * 4 + (e >> 2)] = (uint8_t)r;
+                    });
+#pragma unroll
+                for (int c = 0; c < 8; c++) Rr[c] = rb[c * WAVE + lane];
+                wave_sync();
             }
-            if (!jv) fm = 0;
-            // rows past the segment are filled later; do not re-decide them
-            {
-                const int nvr = nb;   // valid rows are i0 .. i0+nb-1
-                uint32_t vm = 0;
-#pragma unroll
-                for (int e = 0; e < 16; e++) vm |= ((8 * (e >> 2) + 4 * h + (e & 3)) < nvr ? 1u : 0u) << e;
-                fm &= vm;
-            }
-            if (__any(fm != 0)) {
-                uint32_t total;
-                const uint32_t cnt = __popc(fm);
-                uint32_t off = wave_excl_scan(cnt, total);
-#pragma unroll
-                for (int c = 0; c < 4; c++) rbuf[wave][lane][c] = Rh[c];
-                uint8_t *rb = (uint8_t *)rbuf[wave];
-                for (uint32_t base = 0; base < total; base += QCAP) {
-                    uint32_t f = fm;
-                    uint32_t o = off;
-                    while (f) {
-                        const uint32_t e = __builtin_ctz(f);
-                        f &= f - 1;
-                        if (o >= base && o < base + QCAP) queue[wave][o - base] = (uint16_t)((lane << 4) | e);
-                        o++;
-                    }
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                    __builtin_amdgcn_wave_barrier();
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                    const uint32_t nq = total - base < (uint32_t)QCAP ? total - base : (uint32_t)QCAP;
-                    for (uint32_t qi = lane; qi < nq; qi += WAVE) {
-                        const uint32_t it = queue[wave][qi];
-                        const uint32_t ow = it >> 4, e = it & 15u;
-                        const int64_t i = i0 + 8 * (e >> 2) + 4 * (ow >> 5) + (e & 3);
-                        const int64_t jj = j0 + (ow & 31);
-                        const float sx = exact_sqdist(x, i, jj, tau, m);
-                        const uint32_t r = exact_rank<TG>(sx, S, nt);
-                        // byte (e>>2) of register (e&3) of lane ow
-                        rb[(ow * 4 + (e & 3)) * 4 + (e >> 2)] = (uint8_t)r;
-                    }
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                    __builtin_amdgcn_wave_barrier();
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                }
-#pragma unroll
-                for (int c = 0; c < 4; c++) Rh[c] = rbuf[wave][lane][c];
-            }
-            // full rank set of this column: R[0..3] from half 0, R[4..7] from half 1
+            // one swap per register: half 0 keeps tile 0 rows 8g+c and gets
+            // tile 0 rows 8g+4+c from its partner; half 1 the same for tile 1
             uint32_t R[8];
 #pragma unroll
             for (int c = 0; c < 4; c++) {
-                const uint32_t other = __shfl_xor(Rh[c], 32);
-                R[c] = h ? other : Rh[c];
-                R[4 + c] = h ? Rh[c] : other;
+                const auto sw = __builtin_amdgcn_permlane32_swap(Rr[c], Rr[4 + c], false, false);
+                R[c] = sw[0];
+                R[4 + c] = sw[1];
             }
             const uint32_t fill = nb == BLK ? 0u : (0xFFFFFFFFu << nb);
+            scan_words<TG>(R, jv ? ~fill : 0u, jv ? 0u : 0xFFFFFFFFu, run, nt, K, cslot, cstride);
+            if (fill && jv) {
 #pragma unroll
-            for (int tp = 0; tp < TH; tp++) {
-                const uint32_t t = 2 * tp + h;
-                if ((int)t < nt) {
-                    const uint32_t K = 0x80808080u + t * 0x01010101u;
-                    uint32_t w = 0;
-#pragma unroll
-                    for (int c = 0; c < 8; c++) w |= ((K - R[c]) >> (7 - c)) & (0x01010101u << c);
-                    w = jv ? (w | fill) : 0u;
-                    consume_word(w, run[tp], popen, (uint32_t)tp, cslot + (size_t)t * cstride - (size_t)tp * cstride,
-                                 cstride, lh + t * LCAP, hist + (size_t)t * (nc + 1) - (size_t)tp * (nc + 1),
-                                 (uint32_t)nc);
-                    if (fill && jv) run[tp] -= (uint32_t)(BLK - nb);
-                }
+                for (int t = 0; t < TG; t++) run[t] -= (uint32_t)(BLK - nb);
             }
         }
-        if (jv) {
-#pragma unroll
-            for (int tp = 0; tp < TH; tp++) {
-                const uint32_t t = 2 * tp + h;
-                if ((int)t < nt) close_segment(&cslot[(size_t)t * cstride], run[tp], popen, 1u << tp);
-            }
-        }
+        if (jv) close_segment<TG>(run, nt, cslot, cstride);
     }
 
     __syncthreads();
-    for (int e = threadIdx.x; e < TG * LCAP; e += WG) {
-        const uint32_t v = lh[e];
-        const int t = e / LCAP;
-        if (v && t < nt) glob_add(&hist[(size_t)t * (nc + 1) + (e - t * LCAP)], v);
-    }
+    flush_hist<TG>(lh, priv, nt, hist, nc, WG);
 }
 
 // ---------------------------------------------------------------- k_det_mx
 // DET pass of the MFMA form.  Work item = (strip of 64 diagonals, step
 // segment), as k_det.  A 32-step block needs G(s0+u, s0+r0+v) for v = u + l;
-// three 32x32 MFMA tiles cover v = 0..95.  Each tile element is written,
-// already combined as n_col - 2 g, into a skewed LDS buffer S[l][u] (l = v - u)
-// so that lane l then reads its own diagonal's 32 steps, adds n_row, and packs
-// the 32 ranks itself (no exchange).  Elements with v - u outside 0..63 are
-// not needed and not written.
-constexpr int NBX_DET = 16384;
-constexpr int SKS = 33;   // skew buffer row stride (floats): conflict-free writes and reads
+// three 32x32 MFMA tiles cover v = 0..95, one after the other.  Each needed
+// element's rank byte is stored at (diagonal l, step u) of an LDS byte matrix
+// laid out so that lane l then reads its diagonal's eight packed rank
+// registers as words: byte u of row l sits at l*SKB + 4(u&7) + (u>>3).
+constexpr int SKB = 36;   // bytes per diagonal row (9 words: conflict-free stores and loads)
 
 template <int TG>
 __global__ __launch_bounds__(WG) void k_det_mx(const float *__restrict__ x, const float *__restrict__ nrm, int64_t nc,
                                                int tau, int m, Bounds<float, TG> S, int nt,
                                                const uint8_t *__restrict__ lut, MxParams P, int seg_len, int nseg,
                                                const int2 *__restrict__ items, int64_t nitems,
-                                               unsigned long long *__restrict__ hist, uint2 *__restrict__ carry) {
-    constexpr int QCAP = 256;
+                                               unsigned long long *__restrict__ hist, uint2 *__restrict__ carry,
+                                               uint32_t *__restrict__ priv_base) {
     __shared__ uint32_t lh[TG * LCAP];
-    __shared__ uint8_t lut_s[NBX_DET];
-    __shared__ float skew[WAVES][WAVE * SKS];      // also holds the packed ranks during rechecks
+    __shared__ uint8_t lut_s[NBX];
+    __shared__ uint32_t skew[WAVES][WAVE * SKB / 4];
     __shared__ uint16_t queue[WAVES][QCAP];
-    __shared__ float nwin[WAVES][BLK + WAVE];      // n_j of the block's columns
-    __shared__ float bnd_s[TG];
+    __shared__ float bnd_s[NBND];
     __shared__ float rwin[WAVES][MXW];             // x[s0 .. s0 + 31 + (m-1)tau]
     __shared__ float cwin[WAVES][MXW + WAVE];      // x[c0 .. c0 + 95 + (m-1)tau]
     for (int e = threadIdx.x; e < TG * LCAP; e += WG) lh[e] = 0;
     for (int e = threadIdx.x; e < P.nb / 4; e += WG) ((uint32_t *)lut_s)[e] = ((const uint32_t *)lut)[e];
-    if (threadIdx.x < TG) bnd_s[threadIdx.x] = S.s[threadIdx.x];
+    if (threadIdx.x < NBND) bnd_s[threadIdx.x] = (int)threadIdx.x < nt ? S.s[threadIdx.x] : __int_as_float(0x7F800000);
+    priv_zero<TG>(priv_slice<TG>(priv_base), nt, WG);
     __syncthreads();
+    uint32_t *priv = priv_slice<TG>(priv_base);
+    const Sink K{lh, priv, hist, nc};
 
     const int lane = threadIdx.x & (WAVE - 1);
     const int wave = wave_id();
     const int col = lane & 31;
     const int h = lane >> 5;
-    float *sk = skew[wave];
+    uint8_t *sk8 = (uint8_t *)skew[wave];
     float *rw = rwin[wave], *cw = cwin[wave];
     const size_t cstride = (size_t)(nc - 1) * nseg;
     const int m2 = (m + 1) / 2;
+    const int ob = col + h * tau, omax = col + (m - 1) * tau;   // operand offsets of dimension h, m-1
     const int rspan = 32 + (m - 1) * tau, cspan = 96 + (m - 1) * tau;   // <= MXW, MXW + 64
 
     for (int64_t item = (int64_t)blockIdx.x * WAVES + wave; item < nitems; item += (int64_t)gridDim.x * WAVES) {
@@ -874,8 +978,7 @@ __global__ __launch_bounds__(WG) void k_det_mx(const float *__restrict__ x, cons
 
         uint32_t run[TG];
 #pragma unroll
-        for (int t = 0; t < TG; t++) run[t] = 0;
-        uint32_t popen = rv ? 0xFFFFFFFFu : 0u;
+        for (int t = 0; t < TG; t++) run[t] = rv ? PRE : 0u;
 
         // operand windows of the first block
         float pr[2], pc[3];
@@ -886,18 +989,14 @@ __global__ __launch_bounds__(WG) void k_det_mx(const float *__restrict__ x, cons
 
         for (int64_t s0 = sb; s0 < se; s0 += BLK) {
             const int64_t c0 = s0 + r0;   // first column of tile 0
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            wave_sync();
 #pragma unroll
             for (int c = 0; c < 2; c++)
                 if (lane + c * WAVE < rspan) rw[lane + c * WAVE] = pr[c];
 #pragma unroll
             for (int c = 0; c < 3; c++)
                 if (lane + c * WAVE < cspan) cw[lane + c * WAVE] = pc[c];
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            wave_sync();
             // next block's windows, in flight while this block computes (only
             // if there is a next block: its windows end by x[N + 94])
             const bool more = s0 + BLK < se;
@@ -907,148 +1006,102 @@ __global__ __launch_bounds__(WG) void k_det_mx(const float *__restrict__ x, cons
 #pragma unroll
             for (int c = 0; c < 3; c++)
                 pc[c] = more && lane + c * WAVE < cspan ? x[c0 + BLK + c * WAVE + lane] : 0.f;
-            floatx16 acc0 = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-            floatx16 acc1 = acc0, acc2 = acc0;
-#pragma unroll 2
-            for (int k2 = 0; k2 < m2; k2++) {
-                const int dim = 2 * k2 + h;
-                const bool dv = dim < m;
-                const int off = dim * tau;
-                const float a = dv ? rw[col + off] : 0.f;
-                const float b0 = dv ? cw[col + off] : 0.f;
-                const float b1 = dv ? cw[32 + col + off] : 0.f;
-                const float b2 = dv ? cw[64 + col + off] : 0.f;
-                acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b0, acc0, 0, 0, 0);
-                acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b1, acc1, 0, 0, 0);
-                acc2 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b2, acc2, 0, 0, 0);
-            }
+            float ni[16];
+            tile_row_norms(nrm, s0, h, ni);
             // element e of tile p: row u = 8(e>>2) + 4h + (e&3), column v = 32p + col, diagonal l = v - u
-            const float n0 = nrm[c0 + col], n1 = nrm[c0 + 32 + col], n2 = nrm[c0 + 64 + col];
-            nwin[wave][lane] = h ? n1 : n0;       // columns c0 .. c0+63
-            if (lane < 32) nwin[wave][64 + lane] = n2;   // columns c0+64 .. c0+95
 #pragma unroll
-            for (int e = 0; e < 16; e++) {
-                const int u = 8 * (e >> 2) + 4 * h + (e & 3);
-                const int l0 = col - u, l1 = 32 + col - u, l2 = 64 + col - u;
-                if (l0 >= 0) sk[l0 * SKS + u] = fmaf(-2.0f, acc0[e], n0);
-                sk[l1 * SKS + u] = fmaf(-2.0f, acc1[e], n1);          // 1 <= l1 <= 63 always
-                if (l2 < WAVE) sk[l2 * SKS + u] = fmaf(-2.0f, acc2[e], n2);
-            }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            const float *nrow = nrm + s0;   // uniform: scalar loads
-            uint32_t R[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
-            uint32_t fm = 0;
+            for (int p = 0; p < 3; p++) {
+                const floatx16 acc = gram_tile(rw, cw + 32 * p, ob, omax, m2, tau);
+                uint32_t Rq[4], fm;
+                tile_ranks(acc, ni, nrm[c0 + 32 * p + col], lut_s, P, bnd_s, Rq, fm);
+                uint32_t need = 0;
 #pragma unroll
-            for (int u = 0; u < BLK; u++) {
-                float st = sk[lane * SKS + u] + nrow[u];
-                st = fmaxf(st, 0.0f);
-                const uint32_t ent = lut_s[mx_key_bucket(st, P)];
-                uint32_t amb = 0, rk;
-                if (ent & 0x80u)
-                    rk = mx_resolve(ent, st, nrow[u] + nwin[wave][lane + u], bnd_s, P.cE, amb);
-                else
-                    rk = ent & 0x3Fu;
-                R[u & 7] |= rk << (8 * (u >> 3));
-                fm |= amb << u;
+                for (int e = 0; e < 16; e++) {
+                    const int u = 8 * (e >> 2) + 4 * h + (e & 3);
+                    const int l = 32 * p + col - u;
+                    if (l >= 0 && l < WAVE) {
+                        need |= 1u << e;
+                        sk8[l * SKB + 4 * (u & 7) + (u >> 3)] = (uint8_t)(Rq[e & 3] >> (8 * (e >> 2)));
+                    }
+                }
+                // pairs past the end of the matrix are not re-decided (the scan masks them)
+                fm &= c0 + 32 * p + col < nc ? need : 0u;
+                if (__any(fm != 0u)) {
+                    wave_sync();
+                    mx_recheck<TG>(
+                        fm, queue[wave], x, tau, m, bnd_s, nt,
+                        [&](uint32_t ow, uint32_t e, int64_t &i, int64_t &j) {
+                            const uint32_t u = 8 * (e >> 2) + 4 * (ow >> 5) + (e & 3);
+                            i = s0 + u;
+                            j = c0 + 32 * p + (ow & 31);
+                        },
+                        [&](uint32_t ow, uint32_t e, uint32_t rk) {
+                            const uint32_t u = 8 * (e >> 2) + 4 * (ow >> 5) + (e & 3);
+                            const uint32_t l = 32 * p + (ow & 31) - u;
+                            sk8[l * SKB + 4 * (u & 7) + (u >> 3)] = (uint8_t)rk;
+                        });
+                }
             }
+            wave_sync();
+            uint32_t R[8];
+#pragma unroll
+            for (int c = 0; c < 8; c++) R[c] = skew[wave][lane * (SKB / 4) + c];
             const int64_t nv64 = dlen - s0;
             const int nvalid = nv64 <= 0 ? 0 : (nv64 >= BLK ? BLK : (int)nv64);
             const uint32_t vmask = nvalid >= BLK ? 0xFFFFFFFFu : ((1u << nvalid) - 1u);
-            fm &= vmask;
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            if (__any(fm != 0)) {
-                uint32_t *rb32 = (uint32_t *)sk;   // 64 lanes x 8 registers, reused skew buffer
-#pragma unroll
-                for (int c = 0; c < 8; c++) rb32[lane * 8 + c] = R[c];
-                uint8_t *rb = (uint8_t *)sk;
-                uint32_t total;
-                const uint32_t off = wave_excl_scan(__popc(fm), total);
-                for (uint32_t base = 0; base < total; base += QCAP) {
-                    uint32_t f = fm, o = off;
-                    while (f) {
-                        const uint32_t u = __builtin_ctz(f);
-                        f &= f - 1;
-                        if (o >= base && o < base + QCAP) queue[wave][o - base] = (uint16_t)((lane << 5) | u);
-                        o++;
-                    }
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                    __builtin_amdgcn_wave_barrier();
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                    const uint32_t nq = total - base < (uint32_t)QCAP ? total - base : (uint32_t)QCAP;
-                    for (uint32_t qi = lane; qi < nq; qi += WAVE) {
-                        const uint32_t itq = queue[wave][qi];
-                        const uint32_t ow = itq >> 5, u = itq & 31u;
-                        const int64_t i = s0 + u;
-                        const int64_t jj = i + r0 + ow;
-                        const float sx = exact_sqdist(x, i, jj, tau, m);
-                        // byte (u>>3) of register (u&7) of lane ow
-                        rb[(ow * 8 + (u & 7)) * 4 + (u >> 3)] = (uint8_t)exact_rank<TG>(sx, S, nt);
-                    }
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                    __builtin_amdgcn_wave_barrier();
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                }
-#pragma unroll
-                for (int c = 0; c < 8; c++) R[c] = rb32[lane * 8 + c];
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            }
-#pragma unroll
-            for (int t = 0; t < TG; t++) {
-                if (t < nt) {
-                    const uint32_t w = swar_word(R, t) & vmask;
-                    consume_word(w, run[t], popen, t, cslot, cstride, lh + t * LCAP, hist, (uint32_t)nc);
-                }
-            }
+            scan_words<TG>(R, 0xFFFFFFFFu, ~vmask, run, nt, K, cslot, cstride);
         }
-        if (rv) {
-#pragma unroll
-            for (int t = 0; t < TG; t++)
-                if (t < nt) close_segment(&cslot[(size_t)opaque(t) * cstride], run[t], popen, 1u << t);
-        }
+        if (rv) close_segment<TG>(run, nt, cslot, cstride);
     }
 
     __syncthreads();
-    for (int e = threadIdx.x; e < TG * LCAP; e += WG) {
-        const uint32_t v = lh[e];
-        const int t = e / LCAP;
-        if (v && t < nt) glob_add(&hist[(size_t)t * (nc + 1) + (e - t * LCAP)], v);
-    }
+    flush_hist<TG>(lh, priv, nt, hist, nc, WG);
 }
 
 // ---------------------------------------------------------------- k_stitch
-// One thread per (threshold, sequence): joins the (prefix, suffix) runs of
-// the sequence's segments in order.  seq_len(sq) gives the sequence length;
-// segment q spans steps [q*seg_len, min((q+1)*seg_len, seq_len)).
-__global__ void k_stitch(const uint2 *__restrict__ carry, int nt, int64_t nseq, int64_t nc, int is_det, int seg_len,
-                         int nseg, unsigned long long *__restrict__ hist) {
-    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= (int64_t)nt * nseq) return;
-    const int t = (int)(gid / nseq);
-    const int64_t sq = gid - (int64_t)t * nseq;
-    // DET sequences are diagonals r = sq + 1 of length nc - r; LAM sequences are full columns.
-    const int64_t len = is_det ? nc - (sq + 1) : nc;
-    const int64_t nseg_sq = (len + seg_len - 1) / seg_len;
-    const uint2 *c = carry + ((size_t)t * nseq + sq) * nseg;
+// Joins the (prefix, suffix) runs of each sequence's segments in order.
+// Block (x, t): threshold t, sequences x*STT + tid, + gridDim.x*STT, ...
+// Joined runs shorter than SCAP are counted in an LDS histogram per block.
+// seq_len(sq): DET sequences are diagonals r = sq + 1 of length nc - r, LAM
+// sequences are full columns; segment q spans steps [q*seg_len, min((q+1)*seg_len, len)).
+constexpr int STT = 256;
+constexpr int SCAP = 4096;
+
+__global__ __launch_bounds__(STT) void k_stitch(const uint2 *__restrict__ carry, int nt, int64_t nseq, int64_t nc,
+                                                int is_det, int seg_len, int nseg,
+                                                unsigned long long *__restrict__ hist) {
+    __shared__ uint32_t sh[SCAP];
+    const int t = blockIdx.y;
+    for (int e = threadIdx.x; e < SCAP; e += STT) sh[e] = 0u;
+    __syncthreads();
     unsigned long long *h = hist + (size_t)t * (nc + 1);
-    uint64_t cur = 0;
-    for (int64_t q = 0; q < nseg_sq; q++) {
-        const uint2 pc = c[q];
-        const int64_t L = (q + 1) * seg_len <= len ? seg_len : len - q * seg_len;
-        if ((int64_t)pc.x == L) {   // whole segment recurrent
-            cur += (uint64_t)L;
-            continue;
+    auto count = [&](uint64_t len) {
+        if (len < (uint64_t)SCAP)
+            lds_inc(&sh[len]);
+        else if (len <= (uint64_t)nc)
+            glob_add(&h[len], 1ull);
+    };
+    for (int64_t sq = (int64_t)blockIdx.x * STT + threadIdx.x; sq < nseq; sq += (int64_t)gridDim.x * STT) {
+        const int64_t len = is_det ? nc - (sq + 1) : nc;
+        const int64_t nseg_sq = (len + seg_len - 1) / seg_len;
+        const uint2 *c = carry + ((size_t)t * nseq + sq) * nseg;
+        uint64_t cur = 0;
+        for (int64_t q = 0; q < nseg_sq; q++) {
+            const uint2 pc = c[q];
+            const int64_t L = (q + 1) * seg_len <= len ? seg_len : len - q * seg_len;
+            if ((int64_t)pc.x == L) {   // whole segment recurrent
+                cur += (uint64_t)L;
+                continue;
+            }
+            const uint64_t tot = cur + pc.x;
+            if (tot) count(tot);
+            cur = pc.y;
         }
-        const uint64_t tot = cur + pc.x;
-        if (tot && tot <= (uint64_t)nc) glob_add(&h[tot], 1ull);
-        cur = pc.y;
+        if (cur) count(cur);
     }
-    if (cur && cur <= (uint64_t)nc) glob_add(&h[cur], 1ull);
+    __syncthreads();
+    for (int e = threadIdx.x; e < SCAP; e += STT)
+        if (sh[e] && e <= nc) glob_add(&h[e], sh[e]);
 }
 
 // ---------------------------------------------------------------- k_finalize

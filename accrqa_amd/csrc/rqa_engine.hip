@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -47,7 +48,7 @@ struct Ctx {
     int ncu = 0;
     hipStream_t stream = nullptr;
     DevBuf x, hist, carry, metric, scan, items_rr, items_det, counts, perm, lmins, lut, nrm, nmax, lut8, out_rr,
-        out_det, out_lam;
+        out_det, out_lam, priv;
     int64_t items_nc = -1;     // (nc, seg_len) the item tables were built for
     int items_L = -1;
     int64_t n_items_rr = 0, n_items_det = 0;
@@ -308,17 +309,45 @@ void prof_collect(hipStream_t st) {
 
 constexpr int PAD = 128;   // zero samples after N (kernel windows read up to N + 95)
 
+// Segment length: enough (band, segment) items for about four per resident
+// wave of a full GPU (16384 items), in multiples of 32 steps, at least 256.
+// Every segment boundary costs a carry slot per (threshold, sequence) and a
+// join in k_stitch, so segments stay as long as the parallelism allows.
 int seg_len_for(int64_t nc) {
-    int64_t L = (nc + 15) / 16;
-    L = std::max<int64_t>(L, 1024);
+    const int64_t nbands = (nc + rqa::WAVE - 1) / rqa::WAVE;
+    const int64_t nseg = std::max<int64_t>(1, (16384 + nbands - 1) / nbands);
+    int64_t L = (nc + nseg - 1) / nseg;
+    L = std::max<int64_t>(L, 256);
     L = (L + rqa::BLK - 1) / rqa::BLK * rqa::BLK;
     return (int)std::min<int64_t>(L, 1 << 30);
 }
 
-unsigned grid_for(int64_t nitems, int ncu) {
-    int64_t wgs = (nitems + rqa::WAVES - 1) / rqa::WAVES;
-    int64_t cap = (int64_t)ncu * 8;
-    return (unsigned)std::max<int64_t>(1, std::min(wgs, cap));
+// Persistent grid: as many workgroups as can be resident at once (from the
+// kernel's register and LDS use), or fewer if there are fewer items.
+template <typename Kern>
+unsigned grid_for(Kern kern, int64_t nitems, int ncu) {
+    static std::map<const void *, int> per_cu;
+    const void *key = reinterpret_cast<const void *>(kern);
+    auto it = per_cu.find(key);
+    int nb = 0;
+    if (it == per_cu.end()) {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, key, rqa::WG, 0) != hipSuccess || nb < 1) nb = 1;
+        per_cu[key] = nb;
+    } else {
+        nb = it->second;
+    }
+    const int64_t wgs = (nitems + rqa::WAVES - 1) / rqa::WAVES;
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(wgs, (int64_t)nb * ncu));
+}
+
+// Workgroup-private long-run bins for a launch of `grid` workgroups.
+template <int TG>
+int ensure_priv(unsigned grid) {
+    return ensure(g_ctx.priv, (size_t)grid * TG * rqa::PBINS * sizeof(uint32_t));
+}
+
+unsigned stitch_blocks(int64_t nseq) {
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>((nseq + rqa::STT - 1) / rqa::STT, 64));
 }
 
 // Work-item tables (uploaded once per (N_c, segment length)):
@@ -390,7 +419,8 @@ int launch_group(const F *dx, int64_t nc, int tau, int m, const F *bounds, int n
         fill_zero(counts, 32, st);
         const int64_t ni = c.n_items_rr;
         prof_begin("k_rr", st);
-        hipLaunchKernelGGL((rqa::k_rr<F, NORM, TG>), dim3(grid_for(ni, c.ncu)), dim3(rqa::WG), 0, st, dx, nc, tau, m,
+        hipLaunchKernelGGL((rqa::k_rr<F, NORM, TG>), dim3(grid_for(rqa::k_rr<F, NORM, TG>, ni, c.ncu)), dim3(rqa::WG), 0,
+                           st, dx, nc, tau, m,
                            B, nt, d_lut, P, L, (const int2 *)c.items_rr.p, ni, counts);
         prof_end(st);
         HIP_TRY(hipGetLastError());
@@ -416,33 +446,38 @@ int launch_group(const F *dx, int64_t nc, int tau, int m, const F *bounds, int n
                 hipLaunchKernelGGL((rqa::k_mx_lut<TG>), dim3((MP.nb + 255) / 256), dim3(256), 0, st, Bf, nt, MP,
                                    (const unsigned int *)c.nmax.p, (uint8_t *)c.lut8.p);
                 HIP_TRY(hipGetLastError());
-                const int64_t ni = ((nc + 31) / 32) * nseg;
+                const int64_t ni = ((nc + rqa::WAVE - 1) / rqa::WAVE) * nseg;
+                const unsigned g = grid_for(rqa::k_lam_mx<TG>, ni, c.ncu);
+                if ((e = ensure_priv<TG>(g))) return e;
                 prof_begin("k_lam_mx", st);
-                hipLaunchKernelGGL((rqa::k_lam_mx<TG>), dim3(grid_for(ni, c.ncu)), dim3(rqa::WG), 0, st,
-                                   (const float *)dx, (const float *)c.nrm.p, nc, tau, m, Bf, nt,
-                                   (const uint8_t *)c.lut8.p, MP, L, nseg, hist, carry);
+                hipLaunchKernelGGL((rqa::k_lam_mx<TG>), dim3(g), dim3(rqa::WG), 0, st, (const float *)dx,
+                                   (const float *)c.nrm.p, nc, tau, m, Bf, nt, (const uint8_t *)c.lut8.p, MP, L, nseg,
+                                   hist, carry, (uint32_t *)c.priv.p);
                 prof_end(st);
                 HIP_TRY(hipGetLastError());
             } else {
                 const int64_t ni = ((nc + rqa::WAVE - 1) / rqa::WAVE) * nseg;
+                const unsigned g = grid_for(rqa::k_lam<F, NORM, TG>, ni, c.ncu);
+                if ((e = ensure_priv<TG>(g))) return e;
                 prof_begin("k_lam", st);
-                hipLaunchKernelGGL((rqa::k_lam<F, NORM, TG>), dim3(grid_for(ni, c.ncu)), dim3(rqa::WG), 0, st, dx,
-                                   nc, tau, m, B, nt, d_lut, P, L, nseg, hist, carry);
+                hipLaunchKernelGGL((rqa::k_lam<F, NORM, TG>), dim3(g), dim3(rqa::WG), 0, st, dx, nc, tau, m, B, nt,
+                                   d_lut, P, L, nseg, hist, carry, (uint32_t *)c.priv.p);
                 prof_end(st);
                 HIP_TRY(hipGetLastError());
             }
         } else {
             const int64_t ni = ((nc + rqa::WAVE - 1) / rqa::WAVE) * nseg;
+            const unsigned g = grid_for(rqa::k_lam<F, NORM, TG>, ni, c.ncu);
+            if ((e = ensure_priv<TG>(g))) return e;
             prof_begin("k_lam", st);
-            hipLaunchKernelGGL((rqa::k_lam<F, NORM, TG>), dim3(grid_for(ni, c.ncu)), dim3(rqa::WG), 0, st, dx, nc,
-                               tau, m, B, nt, d_lut, P, L, nseg, hist, carry);
+            hipLaunchKernelGGL((rqa::k_lam<F, NORM, TG>), dim3(g), dim3(rqa::WG), 0, st, dx, nc, tau, m, B, nt, d_lut,
+                               P, L, nseg, hist, carry, (uint32_t *)c.priv.p);
             prof_end(st);
             HIP_TRY(hipGetLastError());
         }
-        const int64_t nthr = (int64_t)nt * nc;
         prof_begin("k_stitch", st);
-        hipLaunchKernelGGL(rqa::k_stitch, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, st,
-                           (const uint2 *)carry, nt, nc, nc, 0, L, nseg, hist);
+        hipLaunchKernelGGL(rqa::k_stitch, dim3(stitch_blocks(nc), nt), dim3(rqa::STT), 0, st, (const uint2 *)carry,
+                           nt, nc, nc, 0, L, nseg, hist);
         prof_end(st);
         HIP_TRY(hipGetLastError());
         // The LAM histogram covers every recurrent pair of the full matrix once,
@@ -468,32 +503,35 @@ int launch_group(const F *dx, int64_t nc, int tau, int m, const F *bounds, int n
                 if (use_mfma<F>(1, m, tau)) {
                     rqa::Bounds<float, TG> Bf;
                     for (int t = 0; t < TG; t++) Bf.s[t] = (float)B.s[t];
-                    const rqa::MxParams MP = mx_params((const float *)bounds, nt, m, rqa::NBX_DET);
+                    const rqa::MxParams MP = mx_params((const float *)bounds, nt, m, rqa::NBX);
                     if ((e = ensure(c.lut8, rqa::NBX))) return e;
                     uint8_t *lut8d = (uint8_t *)c.lut8.p;
                     hipLaunchKernelGGL((rqa::k_mx_lut<TG>), dim3((MP.nb + 255) / 256), dim3(256), 0, st, Bf, nt, MP,
                                        (const unsigned int *)c.nmax.p, lut8d);
                     HIP_TRY(hipGetLastError());
+                    const unsigned g = grid_for(rqa::k_det_mx<TG>, ni, c.ncu);
+                    if ((e = ensure_priv<TG>(g))) return e;
                     prof_begin("k_det_mx", st);
-                    hipLaunchKernelGGL((rqa::k_det_mx<TG>), dim3(grid_for(ni, c.ncu)), dim3(rqa::WG), 0, st,
-                                       (const float *)dx, (const float *)c.nrm.p, nc, tau, m, Bf, nt,
-                                       (const uint8_t *)lut8d, MP, L, nseg, (const int2 *)c.items_det.p, ni, hist,
-                                       carry);
+                    hipLaunchKernelGGL((rqa::k_det_mx<TG>), dim3(g), dim3(rqa::WG), 0, st, (const float *)dx,
+                                       (const float *)c.nrm.p, nc, tau, m, Bf, nt, (const uint8_t *)lut8d, MP, L, nseg,
+                                       (const int2 *)c.items_det.p, ni, hist, carry, (uint32_t *)c.priv.p);
                     prof_end(st);
                     HIP_TRY(hipGetLastError());
                     done = true;
                 }
             }
             if (!done) {
+                const unsigned g = grid_for(rqa::k_det<F, NORM, TG>, ni, c.ncu);
+                if ((e = ensure_priv<TG>(g))) return e;
                 prof_begin("k_det", st);
-                hipLaunchKernelGGL((rqa::k_det<F, NORM, TG>), dim3(grid_for(ni, c.ncu)), dim3(rqa::WG), 0, st, dx,
-                                   nc, tau, m, B, nt, d_lut, P, L, nseg, (const int2 *)c.items_det.p, ni, hist, carry);
+                hipLaunchKernelGGL((rqa::k_det<F, NORM, TG>), dim3(g), dim3(rqa::WG), 0, st, dx, nc, tau, m, B, nt,
+                                   d_lut, P, L, nseg, (const int2 *)c.items_det.p, ni, hist, carry,
+                                   (uint32_t *)c.priv.p);
                 prof_end(st);
                 HIP_TRY(hipGetLastError());
             }
-            const int64_t nthr = (int64_t)nt * (nc - 1);
             prof_begin("k_stitch", st);
-            hipLaunchKernelGGL(rqa::k_stitch, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, st,
+            hipLaunchKernelGGL(rqa::k_stitch, dim3(stitch_blocks(nc - 1), nt), dim3(rqa::STT), 0, st,
                                (const uint2 *)carry, nt, nc - 1, nc, 1, L, nseg, hist);
             prof_end(st);
             HIP_TRY(hipGetLastError());

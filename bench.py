@@ -108,6 +108,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--form", default="both", choices=["both", "exact", "mfma"],
                     help="Euclidean distance form(s) to time; 'both' reports the MFMA form as value")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="run one device-to-device copy of this many MiB first (calibrates HBM byte counters)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -132,6 +134,13 @@ def main():
     d_lam = torch.zeros(nl * T * 5, dtype=torch.float32, device=dev)
     stream = torch.cuda.current_stream(dev)
     L = accrqa_amd.load()
+    if args.copy_probe_mib > 0:
+        src = torch.ones(args.copy_probe_mib << 18, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     def step():
         accrqa_amd.rqa_device(dx.data_ptr(), x.size, tau, emb, thr, lmin, dist, d_rr.data_ptr(), d_det.data_ptr(),
