@@ -670,24 +670,25 @@ __device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, uint32_t &total) 
 // (LDS).  The flagged elements go through a queue so that every lane of the
 // wave takes a share of the exact sums.
 template <int TG, class Pair, class Put>
-__device__ __forceinline__ void mx_recheck(uint32_t fm, uint16_t *queue, const float *__restrict__ x, int tau, int m,
+__device__ __forceinline__ void mx_recheck(uint64_t fm, uint16_t *queue, const float *__restrict__ x, int tau, int m,
                                            const float *bnd, int nt, Pair &&pair, Put &&put) {
     const int lane = threadIdx.x & (WAVE - 1);
     uint32_t total;
-    const uint32_t off = wave_excl_scan(__popc(fm), total);
+    const uint32_t off = wave_excl_scan((uint32_t)__popcll(fm), total);
     for (uint32_t base = 0; base < total; base += QCAP) {
-        uint32_t f = fm, o = off;
+        uint64_t f = fm;
+        uint32_t o = off;
         while (f) {
-            const uint32_t b = __builtin_ctz(f);
+            const uint32_t b = (uint32_t)__builtin_ctzll(f);
             f &= f - 1u;
-            if (o >= base && o < base + QCAP) queue[o - base] = (uint16_t)((lane << 5) | b);
+            if (o >= base && o < base + QCAP) queue[o - base] = (uint16_t)((lane << 6) | b);
             o++;
         }
         wave_sync();
         const uint32_t nq = total - base < (uint32_t)QCAP ? total - base : (uint32_t)QCAP;
         for (uint32_t qi = lane; qi < nq; qi += WAVE) {
             const uint32_t it = queue[qi];
-            const uint32_t ow = it >> 5, b = it & 31u;
+            const uint32_t ow = it >> 6, b = it & 63u;
             int64_t i, j;
             pair(ow, b, i, j);
             put(ow, b, rank_direct<float, TG>(exact_sqdist(x, i, j, tau, m), bnd, nt));
@@ -704,9 +705,9 @@ typedef float floatx16 __attribute__((ext_vector_type(16)));
 // of half 1 reads dimension m-1 again and zeroes its row operand.
 __device__ __forceinline__ floatx16 gram_tile(const float *rw, const float *cw, int ob, int omax, int m2, int tau) {
     floatx16 acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    // eight steps at a time: all operand loads in flight before the MFMA chain
-    int k2 = 0;
-    for (; k2 + 8 <= m2; k2 += 8) {
+    // chunks of eight steps, all operand loads in flight before the MFMA
+    // chain; steps past m2 (and dimension m for odd m) get a zero row operand
+    for (int k2 = 0; k2 < m2; k2 += 8) {
         float a[8], b[8];
 #pragma unroll
         for (int u = 0; u < 8; u++) {
@@ -718,26 +719,6 @@ __device__ __forceinline__ floatx16 gram_tile(const float *rw, const float *cw, 
         }
 #pragma unroll
         for (int u = 0; u < 8; u++) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[u], b[u], acc, 0, 0, 0);
-    }
-    for (; k2 + 2 <= m2; k2 += 2) {
-        float a[2], b[2];
-#pragma unroll
-        for (int u = 0; u < 2; u++) {
-            const int oraw = ob + 2 * (k2 + u) * tau;
-            const int o = min(oraw, omax);
-            const float av = rw[o];
-            a[u] = oraw <= omax ? av : 0.f;
-            b[u] = cw[o];
-        }
-        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[0], b[0], acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[1], b[1], acc, 0, 0, 0);
-    }
-    if (k2 < m2) {
-        const int oraw = ob + 2 * k2 * tau;
-        const int o = min(oraw, omax);
-        const float av = rw[o];
-        const float a = oraw <= omax ? av : 0.f;
-        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, cw[o], acc, 0, 0, 0);
     }
     return acc;
 }
@@ -883,7 +864,7 @@ __global__ __launch_bounds__(WG) void k_lam_mx(const float *__restrict__ x, cons
                 for (int c = 0; c < 8; c++) rb[c * WAVE + lane] = Rr[c];
                 uint8_t *rb8 = (uint8_t *)rb;
                 mx_recheck<TG>(
-                    fm, queue[wave], x, tau, m, bnd_s, nt,
+                    (uint64_t)fm, queue[wave], x, tau, m, bnd_s, nt,
                     [&](uint32_t ow, uint32_t b, int64_t &i, int64_t &j) {
                         const uint32_t e = b & 15u;
                         i = i0 + 8 * (e >> 2) + 4 * (ow >> 5) + (e & 3);
@@ -1008,7 +989,14 @@ __global__ __launch_bounds__(WG) void k_det_mx(const float *__restrict__ x, cons
                 pc[c] = more && lane + c * WAVE < cspan ? x[c0 + BLK + c * WAVE + lane] : 0.f;
             float ni[16];
             tile_row_norms(nrm, s0, h, ni);
-            // element e of tile p: row u = 8(e>>2) + 4h + (e&3), column v = 32p + col, diagonal l = v - u
+            // element e of tile p: row u = 8(e>>2) + 4h + (e&3), column v = 32p + col,
+            // diagonal l = v - u = 32p + colh - u0 with u0 = u - 4h, colh = col - 4h.
+            // Its byte goes to l*SKB + 4(u&7) + (u>>3) = skb + SKB*(32p - u0) + 4(u0&7) + (u0>>3),
+            // skb = SKB*colh + 16h (u0&7 <= 3, so adding 4h changes only the low three bits of u).
+            // Tile 1 is needed whole; tile 0 where colh >= u0, tile 2 where colh < u0.
+            const int colh = col - 4 * h;
+            const int skb = SKB * colh + 16 * h;   // may be negative; every stored offset is not
+            uint64_t fma = 0;
 #pragma unroll
             for (int p = 0; p < 3; p++) {
                 const floatx16 acc = gram_tile(rw, cw + 32 * p, ob, omax, m2, tau);
@@ -1017,30 +1005,31 @@ __global__ __launch_bounds__(WG) void k_det_mx(const float *__restrict__ x, cons
                 uint32_t need = 0;
 #pragma unroll
                 for (int e = 0; e < 16; e++) {
-                    const int u = 8 * (e >> 2) + 4 * h + (e & 3);
-                    const int l = 32 * p + col - u;
-                    if (l >= 0 && l < WAVE) {
+                    const int u0 = 8 * (e >> 2) + (e & 3);
+                    const bool nd = p == 1 ? true : (p == 0 ? colh >= u0 : colh < u0);
+                    if (nd) {
                         need |= 1u << e;
-                        sk8[l * SKB + 4 * (u & 7) + (u >> 3)] = (uint8_t)(Rq[e & 3] >> (8 * (e >> 2)));
+                        sk8[skb + SKB * (32 * p - u0) + 4 * (u0 & 7) + (u0 >> 3)] = (uint8_t)(Rq[e & 3] >> (8 * (e >> 2)));
                     }
                 }
                 // pairs past the end of the matrix are not re-decided (the scan masks them)
-                fm &= c0 + 32 * p + col < nc ? need : 0u;
-                if (__any(fm != 0u)) {
-                    wave_sync();
-                    mx_recheck<TG>(
-                        fm, queue[wave], x, tau, m, bnd_s, nt,
-                        [&](uint32_t ow, uint32_t e, int64_t &i, int64_t &j) {
-                            const uint32_t u = 8 * (e >> 2) + 4 * (ow >> 5) + (e & 3);
-                            i = s0 + u;
-                            j = c0 + 32 * p + (ow & 31);
-                        },
-                        [&](uint32_t ow, uint32_t e, uint32_t rk) {
-                            const uint32_t u = 8 * (e >> 2) + 4 * (ow >> 5) + (e & 3);
-                            const uint32_t l = 32 * p + (ow & 31) - u;
-                            sk8[l * SKB + 4 * (u & 7) + (u >> 3)] = (uint8_t)rk;
-                        });
-                }
+                fma |= (uint64_t)(c0 + 32 * p + col < nc ? (fm & need) : 0u) << (16 * p);
+            }
+            if (__any(fma != 0ull)) {
+                wave_sync();
+                mx_recheck<TG>(
+                    fma, queue[wave], x, tau, m, bnd_s, nt,
+                    [&](uint32_t ow, uint32_t b, int64_t &i, int64_t &j) {
+                        const uint32_t e = b & 15u;
+                        i = s0 + 8 * (e >> 2) + 4 * (ow >> 5) + (e & 3);
+                        j = c0 + 32 * (b >> 4) + (ow & 31);
+                    },
+                    [&](uint32_t ow, uint32_t b, uint32_t rk) {
+                        const uint32_t e = b & 15u;
+                        const uint32_t u = 8 * (e >> 2) + 4 * (ow >> 5) + (e & 3);
+                        const uint32_t l = 32 * (b >> 4) + (ow & 31) - u;
+                        sk8[l * SKB + 4 * (u & 7) + (u >> 3)] = (uint8_t)rk;
+                    });
             }
             wave_sync();
             uint32_t R[8];
