@@ -25,6 +25,9 @@ HIPCC_FLAGS = [
     "-shared",
     "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # MFMA results in ordinary VGPRs: the element stage reads every result once,
+    # and accumulator registers would cost a copy per element
+    "-mllvm", "-amdgpu-mfma-vgpr-form=1",
     "-Wno-unused-command-line-argument",
 ]
 

@@ -542,7 +542,6 @@ __global__ __launch_bounds__(WG) void k_rr(const F *__restrict__ x, int64_t nc, 
 constexpr int NBX = 16384;   // byte buckets of the MFMA rank table (LDS)
 constexpr int MXW = 128;     // LDS row window: 32 + (m-1)*tau samples at most
 constexpr int NBND = 64;     // bound slots reachable by a 6-bit table index
-constexpr int QCAP = 128;    // recheck queue entries per wave and round
 
 struct MxParams {
     uint32_t base;   // key bits of bucket 0 (half the smallest bound)
@@ -610,9 +609,13 @@ __global__ void k_mx_lut(Bounds<float, TG> S, int nt, MxParams P, const unsigned
     lut[b] = (uint8_t)e;
 }
 
+// Bucket of s~.  A slightly negative s~ (rounding near 0) has a negative bit
+// pattern as an integer and lands in bucket 0, like +0; keys below the table
+// saturate to bucket 0 and keys above it to the last bucket.
 __device__ __forceinline__ uint32_t mx_key_bucket(float st, const MxParams &P) {
-    const uint32_t key = __float_as_uint(st);   // st >= +0, finite under the norm guard
-    return (uint32_t)max(min((int)(key - P.base) >> P.sh, P.nb - 1), 0);
+    const uint32_t key = (uint32_t)max(__float_as_int(st), 0);
+    const uint32_t d = __builtin_elementwise_sub_sat(key, P.base);
+    return min(d >> P.sh, (uint32_t)(P.nb - 1));
 }
 
 // Second stage of one flagged element (table byte with bit 7 set): the one
@@ -628,19 +631,19 @@ __device__ __forceinline__ uint32_t mx_second(uint32_t byte, float st, float sum
 }
 
 // Exact-order fp32 squared distance of one pair (the reference's sum).
-// Loads go out eight dimensions at a time; the sum keeps the order k = 0..m-1.
+// Loads go out four dimensions at a time; the sum keeps the order k = 0..m-1.
 __device__ __forceinline__ float exact_sqdist(const float *__restrict__ x, int64_t i, int64_t j, int tau, int m) {
     float s = 0.f;
-    for (int k0 = 0; k0 < m; k0 += 8) {
-        float a[8], b[8];
+    for (int k0 = 0; k0 < m; k0 += 4) {
+        float a[4], b[4];
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
+        for (int k = 0; k < 4; k++) {
             const int64_t o = (int64_t)(k0 + k < m ? k0 + k : m - 1) * tau;
             a[k] = x[i + o];
             b[k] = x[j + o];
         }
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
+        for (int k = 0; k < 4; k++) {
             if (k0 + k < m) {
                 const float d = a[k] - b[k];
                 const float q = d * d;
@@ -662,39 +665,6 @@ __device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, uint32_t &total) 
     }
     total = __shfl(inc, WAVE - 1);
     return inc - v;
-}
-
-// Exact re-decision of one wave's ambiguous elements.  Bit b of fm flags an
-// element of this lane; pair(owner lane, b, i, j) names its matrix pair and
-// put(owner lane, b, rank) stores the exact rank where the owner will read it
-// (LDS).  The flagged elements go through a queue so that every lane of the
-// wave takes a share of the exact sums.
-template <int TG, class Pair, class Put>
-__device__ __forceinline__ void mx_recheck(uint64_t fm, uint16_t *queue, const float *__restrict__ x, int tau, int m,
-                                           const float *bnd, int nt, Pair &&pair, Put &&put) {
-    const int lane = threadIdx.x & (WAVE - 1);
-    uint32_t total;
-    const uint32_t off = wave_excl_scan((uint32_t)__popcll(fm), total);
-    for (uint32_t base = 0; base < total; base += QCAP) {
-        uint64_t f = fm;
-        uint32_t o = off;
-        while (f) {
-            const uint32_t b = (uint32_t)__builtin_ctzll(f);
-            f &= f - 1u;
-            if (o >= base && o < base + QCAP) queue[o - base] = (uint16_t)((lane << 6) | b);
-            o++;
-        }
-        wave_sync();
-        const uint32_t nq = total - base < (uint32_t)QCAP ? total - base : (uint32_t)QCAP;
-        for (uint32_t qi = lane; qi < nq; qi += WAVE) {
-            const uint32_t it = queue[qi];
-            const uint32_t ow = it >> 6, b = it & 63u;
-            int64_t i, j;
-            pair(ow, b, i, j);
-            put(ow, b, rank_direct<float, TG>(exact_sqdist(x, i, j, tau, m), bnd, nt));
-        }
-        wave_sync();
-    }
 }
 
 typedef float floatx16 __attribute__((ext_vector_type(16)));
@@ -727,15 +697,16 @@ __device__ __forceinline__ floatx16 gram_tile(const float *rw, const float *cw, 
 // the layout of the tile (element e -> byte e>>2 of Rq[e&3]).  First stage:
 // table bytes, flags included.  Second stage, for flagged bytes only: the
 // bound within reach decides with the pair's own E_ij; fm gets the elements
-// that still need the exact sum.  ni[e]: norm of element e's row, nj: norm of
-// the lane's column.
+// still ambiguous after that (a few in 10^5 pairs), which the caller
+// re-decides in exact order.  ni[e]: norm of element e's row, nj: norm of the
+// lane's column.
 __device__ __forceinline__ void tile_ranks(const floatx16 &acc, const float (&ni)[16], float nj, const uint8_t *lut_s,
                                            const MxParams &P, const float *bnd_s, uint32_t (&Rq)[4], uint32_t &fm) {
 #pragma unroll
     for (int c = 0; c < 4; c++) Rq[c] = 0u;
 #pragma unroll
     for (int e = 0; e < 16; e++) {
-        const float st = fmaxf(fmaf(-2.0f, acc[e], ni[e] + nj), 0.0f);
+        const float st = fmaf(-2.0f, acc[e], ni[e] + nj);
         Rq[e & 3] |= (uint32_t)lut_s[mx_key_bucket(st, P)] << (8 * (e >> 2));
     }
     fm = 0u;
@@ -786,8 +757,6 @@ __global__ __launch_bounds__(WG) void k_lam_mx(const float *__restrict__ x, cons
                                                uint32_t *__restrict__ priv_base) {
     __shared__ uint32_t lh[TG * LCAP];
     __shared__ uint8_t lut_s[NBX];
-    __shared__ uint32_t rbuf[WAVES][8 * WAVE];     // packed ranks during rechecks
-    __shared__ uint16_t queue[WAVES][QCAP];
     __shared__ float bnd_s[NBND];
     __shared__ float rwin[WAVES][MXW];        // x[i0 .. i0 + 31 + (m-1)tau] of the current block
     __shared__ float cwin[WAVES][MXW + 32];   // x[j0 .. j0 + 63 + (m-1)tau] of the current item
@@ -811,7 +780,6 @@ __global__ __launch_bounds__(WG) void k_lam_mx(const float *__restrict__ x, cons
     const int m2 = (m + 1) / 2;
     const int ob = col + h * tau, omax = col + (m - 1) * tau;   // operand offsets of dimension h, m-1
     float *rw = rwin[wave], *cw = cwin[wave];
-    uint32_t *rb = rbuf[wave];
 
     for (int64_t item = (int64_t)blockIdx.x * WAVES + wave; item < nitems; item += (int64_t)gridDim.x * WAVES) {
         const int64_t band = item / nseg;
@@ -859,25 +827,17 @@ __global__ __launch_bounds__(WG) void k_lam_mx(const float *__restrict__ x, cons
                 for (int c = 0; c < 4; c++) Rr[4 * p + c] = Rq[c];
                 if (j0 + 32 * p + col < nc) fm |= (f & rowm) << (16 * p);
             }
-            if (__any(fm != 0u)) {
+            // exact re-decision of the ambiguous elements by their own lane
+            while (fm) {
+                const uint32_t b = __builtin_ctz(fm);
+                fm &= fm - 1u;
+                const uint32_t e = b & 15u, p = b >> 4;
+                const float sx = exact_sqdist(x, i0 + 4 * h + 8 * (e >> 2) + (e & 3), j0 + 32 * p + col, tau, m);
+                const uint32_t rk = rank_direct<float, TG>(sx, bnd_s, nt);
+                const uint32_t c = 4 * p + (e & 3), sh = 8 * (e >> 2), keep = ~(0xFFu << sh), val = rk << sh;
 #pragma unroll
-                for (int c = 0; c < 8; c++) rb[c * WAVE + lane] = Rr[c];
-                uint8_t *rb8 = (uint8_t *)rb;
-                mx_recheck<TG>(
-                    (uint64_t)fm, queue[wave], x, tau, m, bnd_s, nt,
-                    [&](uint32_t ow, uint32_t b, int64_t &i, int64_t &j) {
-                        const uint32_t e = b & 15u;
-                        i = i0 + 8 * (e >> 2) + 4 * (ow >> 5) + (e & 3);
-                        j = j0 + 32 * (b >> 4) + (ow & 31);
-                    },
-                    [&](uint32_t ow, uint32_t b, uint32_t r) {
-                        // byte e>>2 of register 4p + (e&3) of lane ow
-                        const uint32_t e = b & 15u, c = 4 * (b >> 4) + (e & 3);
-                        rb8[(c * WAVE + ow) * 4 + (e >> 2)] = (uint8_t)r;
-                    });
-#pragma unroll
-                for (int c = 0; c < 8; c++) Rr[c] = rb[c * WAVE + lane];
-                wave_sync();
+                for (int cc = 0; cc < 8; cc++)
+                    if (c == (uint32_t)cc) Rr[cc] = (Rr[cc] & keep) | val;
             }
             // one swap per register: half 0 keeps tile 0 rows 8g+c and gets
             // tile 0 rows 8g+4+c from its partner; half 1 the same for tile 1
@@ -921,7 +881,6 @@ __global__ __launch_bounds__(WG) void k_det_mx(const float *__restrict__ x, cons
     __shared__ uint32_t lh[TG * LCAP];
     __shared__ uint8_t lut_s[NBX];
     __shared__ uint32_t skew[WAVES][WAVE * SKB / 4];
-    __shared__ uint16_t queue[WAVES][QCAP];
     __shared__ float bnd_s[NBND];
     __shared__ float rwin[WAVES][MXW];             // x[s0 .. s0 + 31 + (m-1)tau]
     __shared__ float cwin[WAVES][MXW + WAVE];      // x[c0 .. c0 + 95 + (m-1)tau]
@@ -1015,21 +974,15 @@ __global__ __launch_bounds__(WG) void k_det_mx(const float *__restrict__ x, cons
                 // pairs past the end of the matrix are not re-decided (the scan masks them)
                 fma |= (uint64_t)(c0 + 32 * p + col < nc ? (fm & need) : 0u) << (16 * p);
             }
-            if (__any(fma != 0ull)) {
-                wave_sync();
-                mx_recheck<TG>(
-                    fma, queue[wave], x, tau, m, bnd_s, nt,
-                    [&](uint32_t ow, uint32_t b, int64_t &i, int64_t &j) {
-                        const uint32_t e = b & 15u;
-                        i = s0 + 8 * (e >> 2) + 4 * (ow >> 5) + (e & 3);
-                        j = c0 + 32 * (b >> 4) + (ow & 31);
-                    },
-                    [&](uint32_t ow, uint32_t b, uint32_t rk) {
-                        const uint32_t e = b & 15u;
-                        const uint32_t u = 8 * (e >> 2) + 4 * (ow >> 5) + (e & 3);
-                        const uint32_t l = 32 * (b >> 4) + (ow & 31) - u;
-                        sk8[l * SKB + 4 * (u & 7) + (u >> 3)] = (uint8_t)rk;
-                    });
+            // exact re-decision of the ambiguous elements by their own lane
+            while (fma) {
+                const uint32_t b = (uint32_t)__builtin_ctzll(fma);
+                fma &= fma - 1u;
+                const uint32_t e = b & 15u, p = b >> 4;
+                const uint32_t u = 8 * (e >> 2) + 4 * h + (e & 3);
+                const float sx = exact_sqdist(x, s0 + u, c0 + 32 * p + col, tau, m);
+                const uint32_t l = 32 * p + col - u;
+                sk8[l * SKB + 4 * (u & 7) + (u >> 3)] = (uint8_t)rank_direct<float, TG>(sx, bnd_s, nt);
             }
             wave_sync();
             uint32_t R[8];
