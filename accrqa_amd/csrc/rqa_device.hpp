@@ -280,6 +280,28 @@ __device__ __forceinline__ void load_lut(LutEntry<F> *dst, const LutEntry<F> *sr
     for (int e = threadIdx.x; e < nb; e += WG) dst[e] = src[e];
 }
 
+// ---------------------------------------------------------------- rank tiles
+// When LAM and DET are computed together, the LAM pass keeps its packed ranks
+// for k_det_scan.  Tile (row block I, band J) covers rows 32I .. 32I+31 and
+// columns 64J .. 64J+63; lane l's 32 bytes (the eight rank registers of
+// column 64J + l, register q byte g = row 32I + 8g + q) sit at tile*2048 +
+// 32l.  Only tiles with J >= I/2 are kept -- every pair (i, j > i) lies in
+// one of them -- packed row block by row block: row block I starts at tile
+// I*nbands - floor(I/2)*floor((I-1)/2).
+constexpr int TILE_BYTES = 2048;
+
+__device__ __forceinline__ int64_t tile_index(int64_t I, int64_t J, int64_t nbands) {
+    return I * nbands - (I / 2) * ((I - 1) / 2) + (J - I / 2);
+}
+
+__device__ __forceinline__ void store_tile(uint8_t *tiles, int64_t I, int64_t J, int64_t nbands,
+                                           const uint32_t (&R)[8]) {
+    if (J < I / 2) return;
+    uint4 *dst = (uint4 *)(tiles + tile_index(I, J, nbands) * TILE_BYTES + (threadIdx.x & (WAVE - 1)) * 32);
+    dst[0] = make_uint4(R[0], R[1], R[2], R[3]);
+    dst[1] = make_uint4(R[4], R[5], R[6], R[7]);
+}
+
 // ---------------------------------------------------------------- k_lam
 // Work item = (band of 64 columns, row segment q).  x is padded with >= 128
 // zeros past N so that the unguarded window reads stay inside the buffer.
@@ -289,7 +311,8 @@ template <typename F, int NORM, int TG>
 __global__ __launch_bounds__(WG) RQA_WPE_LAM void k_lam(const F *__restrict__ x, int64_t nc, int tau, int m, Bounds<F, TG> S,
                                             int nt, const LutEntry<F> *__restrict__ lut, LutParams P, int seg_len,
                                             int nseg, unsigned long long *__restrict__ hist,
-                                            uint2 *__restrict__ carry, uint32_t *__restrict__ priv_base) {
+                                            uint2 *__restrict__ carry, uint32_t *__restrict__ priv_base,
+                                            uint8_t *__restrict__ tiles) {
     __shared__ uint32_t lh[TG * LCAP];
     __shared__ LutEntry<F> lut_s[NB_MAX];
     __shared__ F bnd_s[TG];
@@ -346,6 +369,7 @@ __global__ __launch_bounds__(WG) RQA_WPE_LAM void k_lam(const F *__restrict__ x,
             }
             uint32_t R[8];
             pack_ranks<F, TG>(s, R, lut_s, P, bnd_s, nt);
+            if (tiles) store_tile(tiles, i0 / BLK, band, nbands, R);
             // rows past the segment count as recurrent padding, taken back below
             const uint32_t fill = nb == BLK ? 0u : (0xFFFFFFFFu << nb);
             scan_words<TG>(R, jv ? ~fill : 0u, jv ? 0u : 0xFFFFFFFFu, run, nt, K, cslot, cstride);
@@ -446,6 +470,82 @@ __global__ __launch_bounds__(WG) void k_det(const F *__restrict__ x, int64_t nc,
     flush_hist<TG>(lh, priv, nt, hist, nc, WG);
 }
 
+// ---------------------------------------------------------------- k_det_scan
+// DET pass over the rank tiles the LAM pass kept.  Work items as k_det.  The
+// pairs of a block, (s0+u, s0+r0+l+u) for lane l and step u < 32, lie in row
+// block s0/32 and columns c0 .. c0+94 (c0 = s0 + r0): at most three
+// consecutive tiles, which the wave copies to LDS (one 36-byte record per
+// tile lane: conflict-free gathers) before each lane collects its diagonal.
+constexpr int TREC = 9;   // LDS words per tile lane (8 + 1 pad)
+
+template <int TG>
+__global__ __launch_bounds__(WG) void k_det_scan(const uint8_t *__restrict__ tiles, int64_t nbands, int64_t nc, int nt,
+                                                 int seg_len, int nseg, const int2 *__restrict__ items,
+                                                 int64_t nitems, unsigned long long *__restrict__ hist,
+                                                 uint2 *__restrict__ carry, uint32_t *__restrict__ priv_base) {
+    __shared__ uint32_t lh[TG * LCAP];
+    __shared__ uint32_t tl[WAVES][3 * WAVE * TREC];
+    for (int e = threadIdx.x; e < TG * LCAP; e += WG) lh[e] = 0;
+    priv_zero<TG>(priv_slice<TG>(priv_base), nt, WG);
+    __syncthreads();
+    uint32_t *priv = priv_slice<TG>(priv_base);
+    const Sink K{lh, priv, hist, nc};
+
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wave = wave_id();
+    uint32_t *tw = tl[wave];
+    const uint8_t *tb = (const uint8_t *)tw;
+    const size_t cstride = (size_t)(nc - 1) * nseg;
+
+    for (int64_t item = (int64_t)blockIdx.x * WAVES + wave; item < nitems; item += (int64_t)gridDim.x * WAVES) {
+        const int2 it = items[item];
+        const int64_t r0 = 1 + (int64_t)it.x * WAVE;
+        const int q = it.y;
+        const int64_t r = r0 + lane;
+        const bool rv = r < nc;
+        const int64_t dlen = rv ? nc - r : 0;
+        const int64_t sb = (int64_t)q * seg_len;
+        const int64_t se_nom = sb + seg_len;
+        const int64_t smax = nc - r0;
+        const int64_t se = se_nom < smax ? se_nom : smax;
+        uint2 *cslot = carry + (size_t)(rv ? r - 1 : 0) * nseg + q;
+
+        uint32_t run[TG];
+#pragma unroll
+        for (int t = 0; t < TG; t++) run[t] = rv ? PRE : 0u;
+
+        for (int64_t s0 = sb; s0 < se; s0 += BLK) {
+            const int64_t I = s0 / BLK, c0 = s0 + r0, J0 = c0 / WAVE;
+            wave_sync();   // the previous block's gathers are done
+#pragma unroll
+            for (int t = 0; t < 3; t++) {
+                if (J0 + t < nbands) {
+                    const uint4 *src = (const uint4 *)(tiles + tile_index(I, J0 + t, nbands) * TILE_BYTES) + 2 * lane;
+                    const uint4 a = src[0], b = src[1];
+                    uint32_t *d = tw + (t * WAVE + lane) * TREC;
+                    d[0] = a.x; d[1] = a.y; d[2] = a.z; d[3] = a.w;
+                    d[4] = b.x; d[5] = b.y; d[6] = b.z; d[7] = b.w;
+                }
+            }
+            wave_sync();
+            // step u of lane l: column c0 + l + u, i.e. record (c0 - 64 J0) + l + u
+            const int base = (int)(c0 - J0 * WAVE) + lane;
+            uint32_t R[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
+#pragma unroll
+            for (int u = 0; u < BLK; u++)
+                R[u & 7] |= (uint32_t)tb[(base + u) * (4 * TREC) + 4 * (u & 7) + (u >> 3)] << (8 * (u >> 3));
+            const int64_t nv64 = dlen - s0;
+            const int nvalid = nv64 <= 0 ? 0 : (nv64 >= BLK ? BLK : (int)nv64);
+            const uint32_t vmask = nvalid >= BLK ? 0xFFFFFFFFu : ((1u << nvalid) - 1u);
+            scan_words<TG>(R, 0xFFFFFFFFu, ~vmask, run, nt, K, cslot, cstride);
+        }
+        if (rv) close_segment<TG>(run, nt, cslot, cstride);
+    }
+
+    __syncthreads();
+    flush_hist<TG>(lh, priv, nt, hist, nc, WG);
+}
+
 // ---------------------------------------------------------------- k_rr
 // Work item = (band of 64 columns, row segment q) over rows i <= max column
 // of the band.  Pair weight: 2 if i < j, 1 if i == j, 0 otherwise.
@@ -539,7 +639,8 @@ __global__ __launch_bounds__(WG) void k_rr(const F *__restrict__ x, int64_t nc, 
 // within E_ij of a bound are re-decided in exact order on VALU.  Inputs whose
 // norms are not finite or exceed 1e30 flag every bucket, so the whole matrix
 // is then decided exactly.
-constexpr int NBX = 16384;   // byte buckets of the MFMA rank table (LDS)
+constexpr int NBX = 16384;       // byte buckets of the MFMA rank table (LDS), DET pass
+constexpr int NBX_LAM = 32768;   // the same for the LAM pass (which has the LDS to spare)
 constexpr int MXW = 128;     // LDS row window: 32 + (m-1)*tau samples at most
 constexpr int NBND = 64;     // bound slots reachable by a 6-bit table index
 
@@ -754,9 +855,9 @@ __global__ __launch_bounds__(WG) void k_lam_mx(const float *__restrict__ x, cons
                                                int tau, int m, Bounds<float, TG> S, int nt,
                                                const uint8_t *__restrict__ lut, MxParams P, int seg_len, int nseg,
                                                unsigned long long *__restrict__ hist, uint2 *__restrict__ carry,
-                                               uint32_t *__restrict__ priv_base) {
+                                               uint32_t *__restrict__ priv_base, uint8_t *__restrict__ tiles) {
     __shared__ uint32_t lh[TG * LCAP];
-    __shared__ uint8_t lut_s[NBX];
+    __shared__ uint8_t lut_s[NBX_LAM];
     __shared__ float bnd_s[NBND];
     __shared__ float rwin[WAVES][MXW];        // x[i0 .. i0 + 31 + (m-1)tau] of the current block
     __shared__ float cwin[WAVES][MXW + 32];   // x[j0 .. j0 + 63 + (m-1)tau] of the current item
@@ -848,6 +949,7 @@ __global__ __launch_bounds__(WG) void k_lam_mx(const float *__restrict__ x, cons
                 R[c] = sw[0];
                 R[4 + c] = sw[1];
             }
+            if (tiles) store_tile(tiles, i0 / BLK, band, nbands, R);
             const uint32_t fill = nb == BLK ? 0u : (0xFFFFFFFFu << nb);
             scan_words<TG>(R, jv ? ~fill : 0u, jv ? 0u : 0xFFFFFFFFu, run, nt, K, cslot, cstride);
             if (fill && jv) {

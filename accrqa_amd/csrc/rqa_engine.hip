@@ -48,7 +48,7 @@ struct Ctx {
     int ncu = 0;
     hipStream_t stream = nullptr;
     DevBuf x, hist, carry, metric, scan, items_rr, items_det, counts, perm, lmins, lut, nrm, nmax, lut8, out_rr,
-        out_det, out_lam, priv;
+        out_det, out_lam, priv, tiles;
     int64_t items_nc = -1;     // (nc, seg_len) the item tables were built for
     int items_L = -1;
     int64_t n_items_rr = 0, n_items_det = 0;
@@ -346,6 +346,25 @@ int ensure_priv(unsigned grid) {
     return ensure(g_ctx.priv, (size_t)grid * TG * rqa::PBINS * sizeof(uint32_t));
 }
 
+// Bytes of the rank tiles the LAM pass keeps for k_det_scan: row blocks
+// I < ceil(nc/32), bands J >= I/2 (see rqa::tile_index).
+size_t tile_store_bytes(int64_t nc) {
+    const int64_t nrb = (nc + rqa::BLK - 1) / rqa::BLK;
+    const int64_t nbands = (nc + rqa::WAVE - 1) / rqa::WAVE;
+    const int64_t ntiles = nrb * nbands - (nrb / 2) * ((nrb - 1) / 2);
+    return (size_t)ntiles * rqa::TILE_BYTES;
+}
+
+// The tile store is used when it fits in well under half of the free device
+// memory; otherwise DET recomputes its pairs (k_det / k_det_mx).
+bool tile_store_fits(int64_t nc) {
+    const size_t need = tile_store_bytes(nc);
+    if (g_ctx.tiles.cap >= need) return true;
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) return false;
+    return need + g_ctx.tiles.cap < fr / 2;
+}
+
 unsigned stitch_blocks(int64_t nseq) {
     return (unsigned)std::max<int64_t>(1, std::min<int64_t>((nseq + rqa::STT - 1) / rqa::STT, 64));
 }
@@ -414,6 +433,12 @@ int launch_group(const F *dx, int64_t nc, int tau, int m, const F *bounds, int n
     const size_t hs = (size_t)nc + 1;
     int e;
     unsigned long long *counts = (unsigned long long *)c.counts.p;
+    // LAM keeps its rank tiles for DET when both are wanted
+    uint8_t *tiles = nullptr;
+    if (want_lam && want_det && nc > 1 && tile_store_fits(nc)) {
+        if ((e = ensure(c.tiles, tile_store_bytes(nc)))) return e;
+        tiles = (uint8_t *)c.tiles.p;
+    }
 
     if (want_rr && !want_lam) {
         fill_zero(counts, 32, st);
@@ -441,8 +466,8 @@ int launch_group(const F *dx, int64_t nc, int tau, int m, const F *bounds, int n
             if (use_mfma<F>(1, m, tau)) {
                 rqa::Bounds<float, TG> Bf;
                 for (int t = 0; t < TG; t++) Bf.s[t] = (float)B.s[t];
-                const rqa::MxParams MP = mx_params((const float *)bounds, nt, m, rqa::NBX);
-                if ((e = ensure(c.lut8, rqa::NBX))) return e;
+                const rqa::MxParams MP = mx_params((const float *)bounds, nt, m, rqa::NBX_LAM);
+                if ((e = ensure(c.lut8, rqa::NBX_LAM))) return e;
                 hipLaunchKernelGGL((rqa::k_mx_lut<TG>), dim3((MP.nb + 255) / 256), dim3(256), 0, st, Bf, nt, MP,
                                    (const unsigned int *)c.nmax.p, (uint8_t *)c.lut8.p);
                 HIP_TRY(hipGetLastError());
@@ -452,7 +477,7 @@ int launch_group(const F *dx, int64_t nc, int tau, int m, const F *bounds, int n
                 prof_begin("k_lam_mx", st);
                 hipLaunchKernelGGL((rqa::k_lam_mx<TG>), dim3(g), dim3(rqa::WG), 0, st, (const float *)dx,
                                    (const float *)c.nrm.p, nc, tau, m, Bf, nt, (const uint8_t *)c.lut8.p, MP, L, nseg,
-                                   hist, carry, (uint32_t *)c.priv.p);
+                                   hist, carry, (uint32_t *)c.priv.p, tiles);
                 prof_end(st);
                 HIP_TRY(hipGetLastError());
             } else {
@@ -461,7 +486,7 @@ int launch_group(const F *dx, int64_t nc, int tau, int m, const F *bounds, int n
                 if ((e = ensure_priv<TG>(g))) return e;
                 prof_begin("k_lam", st);
                 hipLaunchKernelGGL((rqa::k_lam<F, NORM, TG>), dim3(g), dim3(rqa::WG), 0, st, dx, nc, tau, m, B, nt,
-                                   d_lut, P, L, nseg, hist, carry, (uint32_t *)c.priv.p);
+                                   d_lut, P, L, nseg, hist, carry, (uint32_t *)c.priv.p, tiles);
                 prof_end(st);
                 HIP_TRY(hipGetLastError());
             }
@@ -471,7 +496,7 @@ int launch_group(const F *dx, int64_t nc, int tau, int m, const F *bounds, int n
             if ((e = ensure_priv<TG>(g))) return e;
             prof_begin("k_lam", st);
             hipLaunchKernelGGL((rqa::k_lam<F, NORM, TG>), dim3(g), dim3(rqa::WG), 0, st, dx, nc, tau, m, B, nt, d_lut,
-                               P, L, nseg, hist, carry, (uint32_t *)c.priv.p);
+                               P, L, nseg, hist, carry, (uint32_t *)c.priv.p, tiles);
             prof_end(st);
             HIP_TRY(hipGetLastError());
         }
@@ -499,8 +524,19 @@ int launch_group(const F *dx, int64_t nc, int tau, int m, const F *bounds, int n
         if (nc > 1) {
             const int64_t ni = c.n_items_det;
             bool done = false;
+            if (tiles) {
+                const unsigned g = grid_for(rqa::k_det_scan<TG>, ni, c.ncu);
+                if ((e = ensure_priv<TG>(g))) return e;
+                prof_begin("k_det_scan", st);
+                hipLaunchKernelGGL((rqa::k_det_scan<TG>), dim3(g), dim3(rqa::WG), 0, st, (const uint8_t *)tiles,
+                                   (int64_t)((nc + rqa::WAVE - 1) / rqa::WAVE), nc, nt, L, nseg,
+                                   (const int2 *)c.items_det.p, ni, hist, carry, (uint32_t *)c.priv.p);
+                prof_end(st);
+                HIP_TRY(hipGetLastError());
+                done = true;
+            }
             if constexpr (sizeof(F) == 4 && NORM == 1) {
-                if (use_mfma<F>(1, m, tau)) {
+                if (!done && use_mfma<F>(1, m, tau)) {
                     rqa::Bounds<float, TG> Bf;
                     for (int t = 0; t < TG; t++) Bf.s[t] = (float)B.s[t];
                     const rqa::MxParams MP = mx_params((const float *)bounds, nt, m, rqa::NBX);
