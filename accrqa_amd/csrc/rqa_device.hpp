@@ -302,6 +302,19 @@ __device__ __forceinline__ void store_tile(uint8_t *tiles, int64_t I, int64_t J,
     dst[1] = make_uint4(R[4], R[5], R[6], R[7]);
 }
 
+// LAM work of the column kernels.  Without a table: every (band, segment)
+// over full columns, carry slot q of nseg per column.  With a table (the
+// split form, rank tiles kept): (band J, segment q) over rows [0, V_J) only,
+// V_J = min(64(J+1), N_c) -- the upper triangle and the band's diagonal
+// tiles; k_lam_hscan covers rows >= V_J from the tiles, in carry slots
+// nseg .. 2 nseg-1 of the column.
+struct LamWork {
+    const int2 *items;
+    int64_t nitems;
+    int nseg;   // segments of a full column
+    int cseg;   // carry slots per column
+};
+
 // ---------------------------------------------------------------- k_lam
 // Work item = (band of 64 columns, row segment q).  x is padded with >= 128
 // zeros past N so that the unguarded window reads stay inside the buffer.
@@ -310,7 +323,7 @@ __device__ __forceinline__ void store_tile(uint8_t *tiles, int64_t I, int64_t J,
 template <typename F, int NORM, int TG>
 __global__ __launch_bounds__(WG) RQA_WPE_LAM void k_lam(const F *__restrict__ x, int64_t nc, int tau, int m, Bounds<F, TG> S,
                                             int nt, const LutEntry<F> *__restrict__ lut, LutParams P, int seg_len,
-                                            int nseg, unsigned long long *__restrict__ hist,
+                                            LamWork W, unsigned long long *__restrict__ hist,
                                             uint2 *__restrict__ carry, uint32_t *__restrict__ priv_base,
                                             uint8_t *__restrict__ tiles) {
     __shared__ uint32_t lh[TG * LCAP];
@@ -327,18 +340,27 @@ __global__ __launch_bounds__(WG) RQA_WPE_LAM void k_lam(const F *__restrict__ x,
     const int lane = threadIdx.x & (WAVE - 1);
     const int wave = wave_id();
     const int64_t nbands = (nc + WAVE - 1) / WAVE;
-    const int64_t nitems = nbands * nseg;
-    const size_t cstride = (size_t)nc * nseg;   // carry stride between thresholds
+    const int64_t nitems = W.items ? W.nitems : nbands * W.nseg;
+    const size_t cstride = (size_t)nc * W.cseg;   // carry stride between thresholds
     constexpr int MJ = sizeof(F) == 4 ? 16 : 8;  // column samples kept in registers
 
     for (int64_t item = (int64_t)blockIdx.x * WAVES + wave; item < nitems; item += (int64_t)gridDim.x * WAVES) {
-        const int64_t band = item / nseg;
-        const int q = (int)(item - band * nseg);
+        int64_t band;
+        int q;
+        if (W.items) {
+            const int2 it = W.items[item];
+            band = it.x;
+            q = it.y;
+        } else {
+            band = item / W.nseg;
+            q = (int)(item - band * W.nseg);
+        }
+        const int64_t vend = W.items ? min((band + 1) * WAVE, nc) : nc;
         const int64_t j = band * WAVE + lane;
         const bool jv = j < nc;
         const int64_t ib = (int64_t)q * seg_len;
-        const int64_t ie = ib + seg_len < nc ? ib + seg_len : nc;
-        uint2 *cslot = carry + (size_t)(jv ? j : 0) * nseg + q;
+        const int64_t ie = ib + seg_len < vend ? ib + seg_len : vend;
+        uint2 *cslot = carry + (size_t)(jv ? j : 0) * W.cseg + q;
 
         uint32_t run[TG];
 #pragma unroll
@@ -540,6 +562,84 @@ __global__ __launch_bounds__(WG) void k_det_scan(const uint8_t *__restrict__ til
             scan_words<TG>(R, 0xFFFFFFFFu, ~vmask, run, nt, K, cslot, cstride);
         }
         if (rv) close_segment<TG>(run, nt, cslot, cstride);
+    }
+
+    __syncthreads();
+    flush_hist<TG>(lh, priv, nt, hist, nc, WG);
+}
+
+// ---------------------------------------------------------------- k_lam_hscan
+// The rest of each LAM column in the split form: rows i >= V_J of column j
+// (band J) are, R being symmetric, columns i >= V_J of row j, and those pairs
+// are in the rank tiles.  Work item = (row band J' = rows 64J' .. 64J'+63,
+// segment q of the columns [V_J', N_c)); lane l walks row 64J' + l in blocks
+// of 32 columns.  A block's bytes are records 32h .. 32h+31 (h = which half
+// of column band Jc) of tiles (2J', Jc) and (2J'+1, Jc); the wave copies them
+// to LDS and lane l reads byte (l%32) of each record of its row block.
+template <int TG>
+__global__ __launch_bounds__(WG) void k_lam_hscan(const uint8_t *__restrict__ tiles, int64_t nbands, int64_t nc, int nt,
+                                                  int seg_len, int nseg, const int2 *__restrict__ items,
+                                                  int64_t nitems, unsigned long long *__restrict__ hist,
+                                                  uint2 *__restrict__ carry, uint32_t *__restrict__ priv_base) {
+    __shared__ uint32_t lh[TG * LCAP];
+    __shared__ uint32_t tl[WAVES][WAVE * TREC];
+    for (int e = threadIdx.x; e < TG * LCAP; e += WG) lh[e] = 0;
+    priv_zero<TG>(priv_slice<TG>(priv_base), nt, WG);
+    __syncthreads();
+    uint32_t *priv = priv_slice<TG>(priv_base);
+    const Sink K{lh, priv, hist, nc};
+
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wave = wave_id();
+    uint32_t *tw = tl[wave];
+    const uint8_t *tb = (const uint8_t *)tw;
+    const int a = lane >> 5, rr = lane & 31;
+    // lane's byte in a record of its row block: register rr&7, byte rr>>3
+    const int gofs = (a * 32) * (4 * TREC) + 4 * (rr & 7) + (rr >> 3);
+    const size_t cstride = (size_t)nc * 2 * nseg;
+
+    for (int64_t item = (int64_t)blockIdx.x * WAVES + wave; item < nitems; item += (int64_t)gridDim.x * WAVES) {
+        const int2 it = items[item];
+        const int64_t Jr = it.x;
+        const int q = it.y;
+        const int64_t j = Jr * WAVE + lane;
+        const bool jv = j < nc;
+        const int64_t hb = min((Jr + 1) * WAVE, nc);   // first column of the part
+        const int64_t cb = hb + (int64_t)q * seg_len;
+        const int64_t ce = cb + seg_len < nc ? cb + seg_len : nc;
+        const int64_t I = 2 * Jr + a;                    // row block of this lane's row
+        uint2 *cslot = carry + (size_t)(jv ? j : 0) * 2 * nseg + nseg + q;
+
+        uint32_t run[TG];
+#pragma unroll
+        for (int t = 0; t < TG; t++) run[t] = jv ? PRE : 0u;
+
+        for (int64_t c0 = cb; c0 < ce; c0 += BLK) {
+            const int nb = ce - c0 < BLK ? (int)(ce - c0) : BLK;
+            const int64_t Jc = c0 / WAVE;
+            const int half = (int)((c0 / BLK) & 1);
+            wave_sync();   // the previous block's gathers are done
+            if (BLK * I < nc) {   // row blocks past the matrix hold no tile (their lanes are idle)
+                const uint4 *src = (const uint4 *)(tiles + tile_index(I, Jc, nbands) * TILE_BYTES +
+                                                   (32 * half + rr) * 32);
+                const uint4 v0 = src[0], v1 = src[1];
+                uint32_t *d = tw + lane * TREC;
+                d[0] = v0.x; d[1] = v0.y; d[2] = v0.z; d[3] = v0.w;
+                d[4] = v1.x; d[5] = v1.y; d[6] = v1.z; d[7] = v1.w;
+            }
+            wave_sync();
+            uint32_t R[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
+#pragma unroll
+            for (int u = 0; u < BLK; u++) R[u & 7] |= (uint32_t)tb[gofs + u * (4 * TREC)] << (8 * (u >> 3));
+            // columns past the part count as recurrent padding, taken back below
+            const uint32_t fill = nb == BLK ? 0u : (0xFFFFFFFFu << nb);
+            scan_words<TG>(R, jv ? ~fill : 0u, jv ? 0u : 0xFFFFFFFFu, run, nt, K, cslot, cstride);
+            if (fill && jv) {
+#pragma unroll
+                for (int t = 0; t < TG; t++) run[t] -= (uint32_t)(BLK - nb);
+            }
+        }
+        if (jv) close_segment<TG>(run, nt, cslot, cstride);
     }
 
     __syncthreads();
@@ -853,7 +953,7 @@ __device__ __forceinline__ void tile_row_norms(const float *__restrict__ nrm, in
 template <int TG>
 __global__ __launch_bounds__(WG) void k_lam_mx(const float *__restrict__ x, const float *__restrict__ nrm, int64_t nc,
                                                int tau, int m, Bounds<float, TG> S, int nt,
-                                               const uint8_t *__restrict__ lut, MxParams P, int seg_len, int nseg,
+                                               const uint8_t *__restrict__ lut, MxParams P, int seg_len, LamWork W,
                                                unsigned long long *__restrict__ hist, uint2 *__restrict__ carry,
                                                uint32_t *__restrict__ priv_base, uint8_t *__restrict__ tiles) {
     __shared__ uint32_t lh[TG * LCAP];
@@ -876,21 +976,30 @@ __global__ __launch_bounds__(WG) void k_lam_mx(const float *__restrict__ x, cons
     const int rspan = 32 + (m - 1) * tau;   // <= MXW (checked by the host)
     const int cspan = 64 + (m - 1) * tau;   // <= MXW + 32
     const int64_t nbands = (nc + WAVE - 1) / WAVE;
-    const int64_t nitems = nbands * nseg;
-    const size_t cstride = (size_t)nc * nseg;
+    const int64_t nitems = W.items ? W.nitems : nbands * W.nseg;
+    const size_t cstride = (size_t)nc * W.cseg;
     const int m2 = (m + 1) / 2;
     const int ob = col + h * tau, omax = col + (m - 1) * tau;   // operand offsets of dimension h, m-1
     float *rw = rwin[wave], *cw = cwin[wave];
 
     for (int64_t item = (int64_t)blockIdx.x * WAVES + wave; item < nitems; item += (int64_t)gridDim.x * WAVES) {
-        const int64_t band = item / nseg;
-        const int q = (int)(item - band * nseg);
+        int64_t band;
+        int q;
+        if (W.items) {
+            const int2 it = W.items[item];
+            band = it.x;
+            q = it.y;
+        } else {
+            band = item / W.nseg;
+            q = (int)(item - band * W.nseg);
+        }
+        const int64_t vend = W.items ? min((band + 1) * WAVE, nc) : nc;
         const int64_t j0 = band * WAVE;
         const int64_t jown = j0 + lane;   // the column this lane scans (after the swap)
         const bool jv = jown < nc;
         const int64_t ib = (int64_t)q * seg_len;
-        const int64_t ie = ib + seg_len < nc ? ib + seg_len : nc;
-        uint2 *cslot = carry + (size_t)(jv ? jown : 0) * nseg + q;
+        const int64_t ie = ib + seg_len < vend ? ib + seg_len : vend;
+        uint2 *cslot = carry + (size_t)(jv ? jown : 0) * W.cseg + q;
 
         uint32_t run[TG];
 #pragma unroll
@@ -1106,13 +1215,16 @@ __global__ __launch_bounds__(WG) void k_det_mx(const float *__restrict__ x, cons
 // Joins the (prefix, suffix) runs of each sequence's segments in order.
 // Block (x, t): threshold t, sequences x*STT + tid, + gridDim.x*STT, ...
 // Joined runs shorter than SCAP are counted in an LDS histogram per block.
-// seq_len(sq): DET sequences are diagonals r = sq + 1 of length nc - r, LAM
-// sequences are full columns; segment q spans steps [q*seg_len, min((q+1)*seg_len, len)).
+// A sequence has cseg carry slots.  DET sequences are diagonals r = sq + 1 of
+// length nc - r; LAM sequences are full columns, either one part (slots
+// 0 ..) or, split (split != 0), rows [0, V) in slots 0 .. and rows [V, nc)
+// in slots nseg .., V = min(64 (sq/64 + 1), nc).  Part segment q covers
+// [q*seg_len, min((q+1)*seg_len, part length)).
 constexpr int STT = 256;
 constexpr int SCAP = 4096;
 
 __global__ __launch_bounds__(STT) void k_stitch(const uint2 *__restrict__ carry, int nt, int64_t nseq, int64_t nc,
-                                                int is_det, int seg_len, int nseg,
+                                                int is_det, int split, int seg_len, int nseg, int cseg,
                                                 unsigned long long *__restrict__ hist) {
     __shared__ uint32_t sh[SCAP];
     const int t = blockIdx.y;
@@ -1126,20 +1238,30 @@ __global__ __launch_bounds__(STT) void k_stitch(const uint2 *__restrict__ carry,
             glob_add(&h[len], 1ull);
     };
     for (int64_t sq = (int64_t)blockIdx.x * STT + threadIdx.x; sq < nseq; sq += (int64_t)gridDim.x * STT) {
-        const int64_t len = is_det ? nc - (sq + 1) : nc;
-        const int64_t nseg_sq = (len + seg_len - 1) / seg_len;
-        const uint2 *c = carry + ((size_t)t * nseq + sq) * nseg;
+        const uint2 *c = carry + ((size_t)t * nseq + sq) * cseg;
         uint64_t cur = 0;
-        for (int64_t q = 0; q < nseg_sq; q++) {
-            const uint2 pc = c[q];
-            const int64_t L = (q + 1) * seg_len <= len ? seg_len : len - q * seg_len;
-            if ((int64_t)pc.x == L) {   // whole segment recurrent
-                cur += (uint64_t)L;
-                continue;
+        auto part = [&](int64_t plen, const uint2 *pc0) {
+            const int64_t np = (plen + seg_len - 1) / seg_len;
+            for (int64_t q = 0; q < np; q++) {
+                const uint2 pc = pc0[q];
+                const int64_t L = (q + 1) * seg_len <= plen ? seg_len : plen - q * seg_len;
+                if ((int64_t)pc.x == L) {   // whole segment recurrent
+                    cur += (uint64_t)L;
+                    continue;
+                }
+                const uint64_t tot = cur + pc.x;
+                if (tot) count(tot);
+                cur = pc.y;
             }
-            const uint64_t tot = cur + pc.x;
-            if (tot) count(tot);
-            cur = pc.y;
+        };
+        if (is_det) {
+            part(nc - (sq + 1), c);
+        } else if (split) {
+            const int64_t V = min((sq / WAVE + 1) * WAVE, nc);
+            part(V, c);
+            part(nc - V, c + nseg);
+        } else {
+            part(nc, c);
         }
         if (cur) count(cur);
     }

@@ -47,11 +47,11 @@ struct Ctx {
     int device = -1;
     int ncu = 0;
     hipStream_t stream = nullptr;
-    DevBuf x, hist, carry, metric, scan, items_rr, items_det, counts, perm, lmins, lut, nrm, nmax, lut8, out_rr,
+    DevBuf x, hist, carry, metric, scan, items_rr, items_det, items_lamv, items_lamh, counts, perm, lmins, lut, nrm, nmax, lut8, out_rr,
         out_det, out_lam, priv, tiles;
     int64_t items_nc = -1;     // (nc, seg_len) the item tables were built for
     int items_L = -1;
-    int64_t n_items_rr = 0, n_items_det = 0;
+    int64_t n_items_rr = 0, n_items_det = 0, n_items_lamv = 0, n_items_lamh = 0;
 };
 
 std::mutex g_mu;
@@ -309,13 +309,15 @@ void prof_collect(hipStream_t st) {
 
 constexpr int PAD = 128;   // zero samples after N (kernel windows read up to N + 95)
 
-// Segment length: enough (band, segment) items for about four per resident
-// wave of a full GPU (16384 items), in multiples of 32 steps, at least 256.
+// Segment length: enough full-column (band, segment) items for about sixteen
+// per resident wave of a full GPU (65536 items; the split LAM and DET passes
+// have items of very different sizes, so they get many), in multiples of 32
+// steps, at least 256.
 // Every segment boundary costs a carry slot per (threshold, sequence) and a
 // join in k_stitch, so segments stay as long as the parallelism allows.
 int seg_len_for(int64_t nc) {
     const int64_t nbands = (nc + rqa::WAVE - 1) / rqa::WAVE;
-    const int64_t nseg = std::max<int64_t>(1, (16384 + nbands - 1) / nbands);
+    const int64_t nseg = std::max<int64_t>(1, (65536 + nbands - 1) / nbands);
     int64_t L = (nc + nseg - 1) / nseg;
     L = std::max<int64_t>(L, 256);
     L = (L + rqa::BLK - 1) / rqa::BLK * rqa::BLK;
@@ -376,27 +378,38 @@ int build_items(int64_t nc, hipStream_t st) {
     Ctx &c = g_ctx;
     const int L = seg_len_for(nc);
     if (c.items_nc == nc && c.items_L == L) return E_OK;
-    std::vector<int2> rr, det;
+    // (item, steps) lists; each table is sorted by decreasing size so that the
+    // persistent waves take the long items first
+    std::vector<std::pair<int64_t, int2>> rr, det, lamv, lamh;
     const int64_t nb = (nc + rqa::WAVE - 1) / rqa::WAVE;
+    auto seg = [&](std::vector<std::pair<int64_t, int2>> &v, int64_t b, int64_t len) {
+        for (int64_t q = 0; q * L < len; q++)
+            v.push_back({std::min<int64_t>(L, len - q * L), make_int2((int)b, (int)q)});
+    };
     for (int64_t b = 0; b < nb; b++) {
-        int64_t rows = std::min<int64_t>((b + 1) * rqa::WAVE, nc);
-        for (int64_t q = 0; q * L < rows; q++) rr.push_back(make_int2((int)b, (int)q));
+        const int64_t V = std::min<int64_t>((b + 1) * rqa::WAVE, nc);
+        seg(rr, b, V);          // rows that can be <= a column of the band
+        seg(lamv, b, V);        // split LAM: rows [0, V) of the band's columns
+        seg(lamh, b, nc - V);   // split LAM: columns [V, nc) of the band's rows
     }
     const int64_t nstrips = (nc - 1 + rqa::WAVE - 1) / rqa::WAVE;
-    for (int64_t b = 0; b < nstrips; b++) {
-        int64_t steps = nc - (1 + b * rqa::WAVE);
-        for (int64_t q = 0; q * L < steps; q++) det.push_back(make_int2((int)b, (int)q));
-    }
+    for (int64_t b = 0; b < nstrips; b++) seg(det, b, nc - (1 + b * rqa::WAVE));
     int e;
-    if ((e = ensure(c.items_rr, rr.size() * sizeof(int2)))) return e;
-    if ((e = ensure(c.items_det, det.size() * sizeof(int2)))) return e;
-    if (!rr.empty())
-        HIP_TRY(hipMemcpyAsync(c.items_rr.p, rr.data(), rr.size() * sizeof(int2), hipMemcpyHostToDevice, st));
-    if (!det.empty())
-        HIP_TRY(hipMemcpyAsync(c.items_det.p, det.data(), det.size() * sizeof(int2), hipMemcpyHostToDevice, st));
-    HIP_TRY(hipStreamSynchronize(st));   // the host vectors die here
-    c.n_items_rr = (int64_t)rr.size();
-    c.n_items_det = (int64_t)det.size();
+    auto upload = [&](std::vector<std::pair<int64_t, int2>> &v, DevBuf &buf, int64_t &n) -> int {
+        std::stable_sort(v.begin(), v.end(), [](const auto &x, const auto &y) { return x.first > y.first; });
+        std::vector<int2> it(v.size());
+        for (size_t i = 0; i < v.size(); i++) it[i] = v[i].second;
+        int e2;
+        if ((e2 = ensure(buf, std::max<size_t>(it.size(), 1) * sizeof(int2)))) return e2;
+        if (!it.empty()) HIP_TRY(hipMemcpy(buf.p, it.data(), it.size() * sizeof(int2), hipMemcpyHostToDevice));
+        n = (int64_t)it.size();
+        return E_OK;
+    };
+    HIP_TRY(hipStreamSynchronize(st));
+    if ((e = upload(rr, c.items_rr, c.n_items_rr))) return e;
+    if ((e = upload(det, c.items_det, c.n_items_det))) return e;
+    if ((e = upload(lamv, c.items_lamv, c.n_items_lamv))) return e;
+    if ((e = upload(lamh, c.items_lamh, c.n_items_lamh))) return e;
     c.items_nc = nc;
     c.items_L = L;
     return E_OK;
@@ -433,12 +446,18 @@ int launch_group(const F *dx, int64_t nc, int tau, int m, const F *bounds, int n
     const size_t hs = (size_t)nc + 1;
     int e;
     unsigned long long *counts = (unsigned long long *)c.counts.p;
-    // LAM keeps its rank tiles for DET when both are wanted
+    // Split form (rank tiles fit in memory): the LAM column kernel ranks the
+    // pairs of the upper triangle and the diagonal tiles once, keeps them, and
+    // k_lam_hscan / k_det_scan read the rest of LAM and all of DET from them.
     uint8_t *tiles = nullptr;
-    if (want_lam && want_det && nc > 1 && tile_store_fits(nc)) {
+    if (want_lam && nc > 1 && tile_store_fits(nc)) {
         if ((e = ensure(c.tiles, tile_store_bytes(nc)))) return e;
         tiles = (uint8_t *)c.tiles.p;
     }
+    const int64_t nbands = (nc + rqa::WAVE - 1) / rqa::WAVE;
+    const rqa::LamWork LW = tiles ? rqa::LamWork{(const int2 *)c.items_lamv.p, c.n_items_lamv, nseg, 2 * nseg}
+                                  : rqa::LamWork{nullptr, nbands * nseg, nseg, nseg};
+    const int cseg_lam = LW.cseg;
 
     if (want_rr && !want_lam) {
         fill_zero(counts, 32, st);
@@ -455,7 +474,7 @@ int launch_group(const F *dx, int64_t nc, int tau, int m, const F *bounds, int n
         if ((e = ensure(c.hist, (size_t)nt * hs * 8))) return e;
         if ((e = ensure(c.metric, (size_t)nt * hs * 8))) return e;
         if ((e = ensure(c.scan, (size_t)nt * hs * 8))) return e;
-        if ((e = ensure(c.carry, (size_t)nt * (size_t)nc * nseg * sizeof(uint2)))) return e;
+        if ((e = ensure(c.carry, (size_t)nt * (size_t)nc * cseg_lam * sizeof(uint2)))) return e;
     }
     unsigned long long *hist = (unsigned long long *)c.hist.p;
     uint2 *carry = (uint2 *)c.carry.p;
@@ -471,38 +490,45 @@ int launch_group(const F *dx, int64_t nc, int tau, int m, const F *bounds, int n
                 hipLaunchKernelGGL((rqa::k_mx_lut<TG>), dim3((MP.nb + 255) / 256), dim3(256), 0, st, Bf, nt, MP,
                                    (const unsigned int *)c.nmax.p, (uint8_t *)c.lut8.p);
                 HIP_TRY(hipGetLastError());
-                const int64_t ni = ((nc + rqa::WAVE - 1) / rqa::WAVE) * nseg;
-                const unsigned g = grid_for(rqa::k_lam_mx<TG>, ni, c.ncu);
+                const unsigned g = grid_for(rqa::k_lam_mx<TG>, LW.nitems, c.ncu);
                 if ((e = ensure_priv<TG>(g))) return e;
                 prof_begin("k_lam_mx", st);
                 hipLaunchKernelGGL((rqa::k_lam_mx<TG>), dim3(g), dim3(rqa::WG), 0, st, (const float *)dx,
-                                   (const float *)c.nrm.p, nc, tau, m, Bf, nt, (const uint8_t *)c.lut8.p, MP, L, nseg,
+                                   (const float *)c.nrm.p, nc, tau, m, Bf, nt, (const uint8_t *)c.lut8.p, MP, L, LW,
                                    hist, carry, (uint32_t *)c.priv.p, tiles);
                 prof_end(st);
                 HIP_TRY(hipGetLastError());
             } else {
-                const int64_t ni = ((nc + rqa::WAVE - 1) / rqa::WAVE) * nseg;
-                const unsigned g = grid_for(rqa::k_lam<F, NORM, TG>, ni, c.ncu);
+                const unsigned g = grid_for(rqa::k_lam<F, NORM, TG>, LW.nitems, c.ncu);
                 if ((e = ensure_priv<TG>(g))) return e;
                 prof_begin("k_lam", st);
                 hipLaunchKernelGGL((rqa::k_lam<F, NORM, TG>), dim3(g), dim3(rqa::WG), 0, st, dx, nc, tau, m, B, nt,
-                                   d_lut, P, L, nseg, hist, carry, (uint32_t *)c.priv.p, tiles);
+                                   d_lut, P, L, LW, hist, carry, (uint32_t *)c.priv.p, tiles);
                 prof_end(st);
                 HIP_TRY(hipGetLastError());
             }
         } else {
-            const int64_t ni = ((nc + rqa::WAVE - 1) / rqa::WAVE) * nseg;
-            const unsigned g = grid_for(rqa::k_lam<F, NORM, TG>, ni, c.ncu);
+            const unsigned g = grid_for(rqa::k_lam<F, NORM, TG>, LW.nitems, c.ncu);
             if ((e = ensure_priv<TG>(g))) return e;
             prof_begin("k_lam", st);
             hipLaunchKernelGGL((rqa::k_lam<F, NORM, TG>), dim3(g), dim3(rqa::WG), 0, st, dx, nc, tau, m, B, nt, d_lut,
-                               P, L, nseg, hist, carry, (uint32_t *)c.priv.p, tiles);
+                               P, L, LW, hist, carry, (uint32_t *)c.priv.p, tiles);
+            prof_end(st);
+            HIP_TRY(hipGetLastError());
+        }
+        if (tiles && c.n_items_lamh > 0) {
+            const unsigned g = grid_for(rqa::k_lam_hscan<TG>, c.n_items_lamh, c.ncu);
+            if ((e = ensure_priv<TG>(g))) return e;
+            prof_begin("k_lam_hscan", st);
+            hipLaunchKernelGGL((rqa::k_lam_hscan<TG>), dim3(g), dim3(rqa::WG), 0, st, (const uint8_t *)tiles, nbands,
+                               nc, nt, L, nseg, (const int2 *)c.items_lamh.p, c.n_items_lamh, hist, carry,
+                               (uint32_t *)c.priv.p);
             prof_end(st);
             HIP_TRY(hipGetLastError());
         }
         prof_begin("k_stitch", st);
         hipLaunchKernelGGL(rqa::k_stitch, dim3(stitch_blocks(nc), nt), dim3(rqa::STT), 0, st, (const uint2 *)carry,
-                           nt, nc, nc, 0, L, nseg, hist);
+                           nt, nc, nc, 0, tiles ? 1 : 0, L, nseg, cseg_lam, hist);
         prof_end(st);
         HIP_TRY(hipGetLastError());
         // The LAM histogram covers every recurrent pair of the full matrix once,
@@ -529,7 +555,7 @@ int launch_group(const F *dx, int64_t nc, int tau, int m, const F *bounds, int n
                 if ((e = ensure_priv<TG>(g))) return e;
                 prof_begin("k_det_scan", st);
                 hipLaunchKernelGGL((rqa::k_det_scan<TG>), dim3(g), dim3(rqa::WG), 0, st, (const uint8_t *)tiles,
-                                   (int64_t)((nc + rqa::WAVE - 1) / rqa::WAVE), nc, nt, L, nseg,
+                                   nbands, nc, nt, L, nseg,
                                    (const int2 *)c.items_det.p, ni, hist, carry, (uint32_t *)c.priv.p);
                 prof_end(st);
                 HIP_TRY(hipGetLastError());
@@ -568,7 +594,7 @@ int launch_group(const F *dx, int64_t nc, int tau, int m, const F *bounds, int n
             }
             prof_begin("k_stitch", st);
             hipLaunchKernelGGL(rqa::k_stitch, dim3(stitch_blocks(nc - 1), nt), dim3(rqa::STT), 0, st,
-                               (const uint2 *)carry, nt, nc - 1, nc, 1, L, nseg, hist);
+                               (const uint2 *)carry, nt, nc - 1, nc, 1, 0, L, nseg, nseg, hist);
             prof_end(st);
             HIP_TRY(hipGetLastError());
         }

@@ -168,7 +168,7 @@ def main():
         L.accrqa_amd_profile(0)
         ms = max(ev0.elapsed_time(ev1), wall * 1e3) / args.steps
         kern = {}
-        for k in ("k_lam", "k_lam_mx", "k_det", "k_det_mx", "k_det_scan", "k_rr", "k_stitch", "k_finalize"):
+        for k in ("k_lam", "k_lam_mx", "k_lam_hscan", "k_det", "k_det_mx", "k_det_scan", "k_rr", "k_stitch", "k_finalize"):
             t, n = C.c_double(0), C.c_longlong(0)
             L.accrqa_amd_profile_read(k.encode(), C.byref(t), C.byref(n))
             if n.value:
