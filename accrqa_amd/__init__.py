@@ -16,8 +16,8 @@ import os
 
 import numpy as np
 
-__all__ = ["RR", "DET", "LAM", "histograms", "rqa_device", "set_form", "AccrqaError", "library_path", "load",
-           "PLT_AMD_GPU", "FORM_AUTO", "FORM_EXACT", "FORM_MFMA",
+__all__ = ["RR", "DET", "LAM", "histograms", "rqa_device", "set_form", "set_split", "AccrqaError", "library_path",
+           "load", "PLT_AMD_GPU", "FORM_AUTO", "FORM_EXACT", "FORM_MFMA", "SPLIT_AUTO", "SPLIT_NEVER",
            "DST_EUCLIDEAN", "DST_MAXIMAL"]
 
 PLT_AMD_GPU = 2048
@@ -88,6 +88,8 @@ def load():
         f.restype = I
     L.accrqa_amd_set_form.argtypes = [I]
     L.accrqa_amd_set_form.restype = I
+    L.accrqa_amd_set_split.argtypes = [I]
+    L.accrqa_amd_set_split.restype = I
     L.accrqa_amd_last_error.restype = C.c_char_p
     L.accrqa_amd_version.restype = C.c_char_p
     _lib = L
@@ -104,6 +106,19 @@ def set_form(form):
     e = load().accrqa_amd_set_form(int(form))
     if e:
         raise AccrqaError(e, f"form {form}")
+
+
+SPLIT_AUTO, SPLIT_NEVER = 0, 1
+
+
+def set_split(mode):
+    """LAM/DET work split: SPLIT_AUTO (rank the upper triangle once, keep the
+    tiles and read the rest of LAM and all of DET from them, when they fit in
+    device memory) or SPLIT_NEVER (LAM ranks full columns, DET its own pairs).
+    Results are identical; only speed and memory differ."""
+    e = load().accrqa_amd_set_split(int(mode))
+    if e:
+        raise AccrqaError(e, f"split mode {mode}")
 
 
 def _dist(d):

@@ -3,11 +3,14 @@
 //
 // One call handles one (tau, emb) block for all thresholds: thresholds are
 // turned into exact comparison bounds, sorted, split into groups of 4/8/16/32
-// (one pass over the pairs per group), and each group runs
-//   k_rr   -> k_rr_out                      (RR)
-//   k_lam  -> k_stitch -> k_finalize        (LAM, TT, TTmax, ENTR, RR)
-//   k_det  -> k_stitch -> k_finalize        (DET, L, Lmax, ENTR, RR)
-// on one HIP stream.  Device buffers live in a per-process context that grows
+// (one pass over the pairs per group), and each group runs on one HIP stream
+//   RR alone:   k_rr -> k_rr_out
+//   LAM:        k_lam | k_lam_mx (column pass; keeps the rank tiles when they
+//               fit) [-> k_lam_hscan] -> k_stitch -> k_finalize, RR from its
+//               histogram -> k_rr_out
+//   DET:        k_det_scan (from the tiles) | k_det | k_det_mx -> k_stitch
+//               -> k_finalize
+// (k_lam_mx / k_det_mx: MFMA Gram form, with k_norms and k_mx_lut first).  Device buffers live in a per-process context that grows
 // on demand and is reused across calls (the reference allocates and frees per
 // call, src/AccRQA-GPU-HST.cu:1855-1945).
 #include <hip/hip_runtime.h>
@@ -61,6 +64,9 @@ Ctx g_ctx;
 // input with m >= 8, exact VALU sums otherwise), 1 = exact VALU sums only,
 // 2 = MFMA + recheck wherever it applies (fp32 input, Euclidean norm).
 int g_form = 0;
+
+// LAM/DET split through kept rank tiles: 0 = when the tiles fit, 1 = never.
+int g_split = 0;
 
 template <typename F>
 bool use_mfma(int dist, int m, int tau) {
@@ -450,7 +456,7 @@ int launch_group(const F *dx, int64_t nc, int tau, int m, const F *bounds, int n
     // pairs of the upper triangle and the diagonal tiles once, keeps them, and
     // k_lam_hscan / k_det_scan read the rest of LAM and all of DET from them.
     uint8_t *tiles = nullptr;
-    if (want_lam && nc > 1 && tile_store_fits(nc)) {
+    if (want_lam && nc > 1 && g_split == 0 && tile_store_fits(nc)) {
         if ((e = ensure(c.tiles, tile_store_bytes(nc)))) return e;
         tiles = (uint8_t *)c.tiles.p;
     }
@@ -965,6 +971,13 @@ int accrqa_amd_set_form(int form) {
     if (form < 0 || form > 2) return E_INVALID;
     std::lock_guard<std::mutex> lk(g_mu);
     g_form = form;
+    return E_OK;
+}
+
+int accrqa_amd_set_split(int mode) {
+    if (mode < 0 || mode > 1) return E_INVALID;
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_split = mode;
     return E_OK;
 }
 

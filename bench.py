@@ -13,8 +13,11 @@ N = 262,144, z-normalised, fp32, tau = 1, m = 16, 16 thresholds
 linspace(0.1, 1.6) * sqrt(m), Euclidean norm, l_min = 2.
 
 The JSON line also carries
-  roofline     -- the dominant kernel (k_lam) timed with HIP events on its own
-                  stream during the timed steps, against the FP32 VALU issue rate;
+  roofline     -- the dominant kernel of the reported form, timed with HIP
+                  events on its own stream during the timed steps, against the
+                  peak of the unit that bounds its algorithmic work (MFMA for
+                  the Gram of k_lam_mx, HBM for the tile scans); per-kernel
+                  figures of every form under forms.*.rooflines;
   cpu_baseline -- the reference CPU path (oracle/_ref, OpenMP on this host's
                   cores) on a bounded prefix of the same series.
 """
@@ -55,6 +58,62 @@ def workload(name):
     else:
         raise SystemExit(f"unknown workload {name}")
     return c
+
+
+def tile_store_bytes(nc):
+    """Bytes of the upper-triangle rank tiles (32 x 64 pairs, one byte each)."""
+    nrb, nb = -(-nc // 32), -(-nc // 64)
+    return (nrb * nb - (nrb // 2) * ((nrb - 1) // 2)) * 2048
+
+
+# HBM bytes per launch of the kernels on the default workload, measured with
+# rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE over one step of the MFMA
+# form and scaled by the same counters' readings for a 1 GiB device copy
+# (FETCH_SIZE reads 0.5 GiB for it: x2.0; WRITE_SIZE 1 GiB: x1.0).
+MEASURED_TRAFFIC = {
+    ("ar1_m16", "k_lam_mx"): 2.0 * 1.101e9 + 41.06e9,
+    ("ar1_m16", "k_det_scan"): 2.0 * 49.05e9 + 5.43e9,
+    ("ar1_m16", "k_lam_hscan"): 2.0 * 17.29e9 + 6.84e9,
+}
+
+
+def kernel_rooflines(kern, nc, emb, dist, workload):
+    """Roofline figures of the main kernels from their average launch times.
+    Algorithmic work per launch:
+      k_lam_mx      Gram dot products of the N_c(N_c+1)/2 distinct pairs it
+                    ranks (upper triangle and diagonal), 2m FLOP each, on MFMA;
+      k_lam         the same pairs' exact-order distances, 3m (Euclidean) or 2m
+                    (maximal) VALU FLOP each;
+      k_lam_hscan,  one read of the rank-tile store (one byte per pair of the
+      k_det_scan    upper triangle), from HBM."""
+    pairs = nc * (nc + 1) / 2.0
+    out = {}
+    for k, v in kern.items():
+        t = v["ms_total"] / max(v["launches"], 1) / 1e3
+        if t <= 0:
+            continue
+        if k == "k_lam_mx":
+            w = pairs * 2 * emb
+            out[k] = {"bound": "mfma", "achieved": w / t / 1e12, "peak": PUBLISHED_FP32_TFLOPS, "unit": "TFLOP/s",
+                      "work_per_launch": w, "work_unit": "FLOP (2m per distinct pair)"}
+        elif k == "k_lam":
+            w = pairs * (3 * emb if dist == 1 else 2 * emb)
+            out[k] = {"bound": "valu", "achieved": w / t / 1e12, "peak": VALU_PEAK_TOPS, "unit": "TFLOP/s",
+                      "work_per_launch": w, "work_unit": "FLOP (exact-order sum per distinct pair)",
+                      "peak_note": "FP32 VALU non-FMA issue rate (sub/mul/add in order, no FMA)"}
+        elif k in ("k_lam_hscan", "k_det_scan"):
+            w = float(tile_store_bytes(nc))
+            out[k] = {"bound": "hbm", "achieved": w / t / 1e9, "peak": 8000.0, "unit": "GB/s",
+                      "work_per_launch": w, "work_unit": "bytes (one read of the rank-tile store)"}
+        else:
+            continue
+        r = out[k]
+        r["frac"] = r["achieved"] / r["peak"]
+        r["avg_ms"] = t * 1e3
+        r["traffic"] = MEASURED_TRAFFIC.get((workload, k))
+        if r["traffic"] is not None:
+            r["traffic_note"] = "HBM bytes per launch, rocprofv3 FETCH_SIZE+WRITE_SIZE, copy-calibrated"
+    return out
 
 
 def cpu_baseline(x, tau, emb, thr, lmin, dist, budget_s):
@@ -190,25 +249,17 @@ def main():
         mismatches = int(sum(int((u.view(np.uint32) != v.view(np.uint32)).sum()) for u, v in zip(a, b)))
 
     value = float(nc) * nc * T / (ms_step / 1e3)
-    # Roofline of the exact form's dominant kernel (k_lam): every pair of the
-    # full matrix, 3m FP ops (m sub, m mul, m add) for the Euclidean distance or
-    # 2m (sub, max) for the maximal norm, plus one compare per threshold.
-    ms_x, kern_x, _ = res.get("exact", res[head])
-    lam = kern_x.get("k_lam", {"ms_total": 0.0, "launches": 1})
-    lam_avg_ms = lam["ms_total"] / max(lam["launches"], 1)
-    per_pair = (3 * emb if dist == 1 else 2 * emb) + T / max(lam["launches"] // args.steps, 1)
-    ops = float(nc) * nc * per_pair
-    achieved = ops / (lam_avg_ms / 1e3) / 1e12 if lam_avg_ms > 0 else 0.0
     form_report = {}
     for f, (ms_f, kern_f, _) in res.items():
         form_report[f] = {"value": float(nc) * nc * T / (ms_f / 1e3), "ms_per_step": ms_f,
                           "kernels_ms_per_step": {k: v["ms_total"] / args.steps for k, v in kern_f.items()}}
-    mx = kern.get("k_lam_mx")
-    if mx:
-        mx_ms = mx["ms_total"] / max(mx["launches"], 1)
-        gram_tflops = float(nc) * nc * 2 * emb / (mx_ms / 1e3) / 1e12
-        form_report["mfma"]["k_lam_mx_gram_tflops"] = gram_tflops
-        form_report["mfma"]["k_lam_mx_gram_frac_of_fp32_mfma_peak"] = gram_tflops / PUBLISHED_FP32_TFLOPS
+    rooflines = {f: kernel_rooflines(res[f][1], nc, emb, dist, args.workload) for f in res}
+    for f in res:
+        form_report[f]["rooflines"] = rooflines[f]
+    # the dominant kernel of the reported form: the largest share of the step
+    dom = max(kern, key=lambda k: kern[k]["ms_total"])
+    roofline = dict(rooflines[head].get(dom, {}))
+    roofline["kernel"] = dom
 
     rr = d_rr.cpu().numpy()
     out = {
@@ -227,10 +278,7 @@ def main():
         "config": {"workload": c["desc"], "N": int(x.size), "N_c": int(nc), "thresholds": T,
                    "form": {"mfma": "MFMA Gram (fp32) + exact-order recheck near thresholds",
                             "exact": "exact-order FP32 sums on VALU"}[head]},
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TOPS, "unit": "TFLOP/s",
-                     "frac": achieved / VALU_PEAK_TOPS, "traffic": None, "kernel": "k_lam",
-                     "kernel_avg_ms": lam_avg_ms, "ops_per_launch": ops,
-                     "peak_note": f"FP32 VALU non-FMA issue rate; published FMA/packed peak {PUBLISHED_FP32_TFLOPS}"},
+        "roofline": roofline,
         "kernels_ms_per_step": {k: v["ms_total"] / args.steps for k, v in kern.items()},
         "forms": form_report,
         "mfma_vs_exact_speedup": (form_report["exact"]["ms_per_step"] / form_report["mfma"]["ms_per_step"]

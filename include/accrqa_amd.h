@@ -120,10 +120,20 @@ int accrqa_amd_rqa_device_f64(const double *d_input, size_t data_size, int tau, 
  * Process-wide; returns 0, or 2 for an unknown form. */
 int accrqa_amd_set_form(int form);
 
+/* LAM/DET work split (no counterpart in the reference).  0 = automatic: when
+ * LAM is computed and N_c^2/2 bytes of rank tiles fit in under half of the
+ * free device memory, the column pass ranks only the upper triangle and the
+ * diagonal tiles and keeps them; the rest of each LAM column (row runs, R
+ * being symmetric) and all of DET are then read from the tiles.  1 = never:
+ * LAM ranks full columns and DET recomputes its pairs.  Results are identical
+ * either way.  Process-wide; returns 0, or 2 for an unknown mode. */
+int accrqa_amd_set_split(int mode);
+
 /* Kernel timing (tracing aid, no counterpart in the reference): when enabled,
  * every kernel launch is bracketed by HIP events on its own stream and the
- * elapsed times are summed per kernel name ("k_lam", "k_det", "k_rr",
- * "k_stitch", "k_finalize").  Adds a stream synchronisation per call. */
+ * elapsed times are summed per kernel name ("k_lam", "k_lam_mx",
+ * "k_lam_hscan", "k_det", "k_det_mx", "k_det_scan", "k_rr", "k_stitch",
+ * "k_finalize").  Adds a stream synchronisation per call. */
 void accrqa_amd_profile(int enable);
 void accrqa_amd_profile_reset(void);
 /* Returns 0 and fills the totals if the kernel was seen, else 1 and zeros. */

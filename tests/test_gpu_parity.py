@@ -324,3 +324,36 @@ def test_mfma_form_bit_exact(mfma_form, oracle):
         for k, t in enumerate(thrs):
             assert np.array_equal(got["lam"][k], oracle.hist("lam", x, np.float32(t), taus[0], embs[0], 1)), (name, t)
             assert np.array_equal(got["det"][k], oracle.hist("det", x, np.float32(t), taus[0], embs[0], 1)), (name, t)
+
+
+@pytest.mark.gpu
+def test_split_and_direct_forms(amd, oracle):
+    """The split form (upper-triangle rank tiles kept, LAM row runs and DET
+    read from them) and the direct form (full-column LAM, DET recomputed)
+    give the oracle's histograms bit for bit: both norms, fp32 and fp64,
+    sequence lengths around the 64-column band and 32-step block sizes, and
+    series long enough for many segments per column."""
+    rng = np.random.default_rng(2024)
+    cases = []
+    for n, m, tau in ((40, 3, 1), (64 + 2, 3, 1), (129, 2, 2), (700, 4, 3), (3001, 5, 1), (4200, 10, 2)):
+        w = np.cumsum(rng.standard_normal(n))
+        cases.append((f"walk{n}", (w - w.mean()) / w.std(), m, tau))
+    try:
+        for name, x64, m, tau in cases:
+            for dt in (np.float32, np.float64):
+                x = x64.astype(dt)
+                thr = np.array([0.05, 0.3, 0.9, 2.0], dt)
+                for dist in (1, 2):
+                    say("split", name, dt.__name__, dist)
+                    amd.set_split(amd.SPLIT_AUTO)
+                    a = amd.histograms(x, tau, m, thr, dist, which=7)
+                    amd.set_split(amd.SPLIT_NEVER)
+                    b = amd.histograms(x, tau, m, thr, dist, which=7)
+                    for key in ("rr", "det", "lam"):
+                        assert np.array_equal(a[key], b[key]), (name, dt, dist, key)
+                    for k in (0, 2):
+                        for kind in ("lam", "det"):
+                            want = oracle.hist(kind, x, dt(thr[k]), tau, m, dist)
+                            assert np.array_equal(a[kind][k], want), (name, dt, dist, kind, thr[k])
+    finally:
+        amd.set_split(amd.SPLIT_AUTO)
