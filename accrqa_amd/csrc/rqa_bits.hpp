@@ -94,9 +94,11 @@ struct GeTree {
 
 // One 32-step word of one threshold.  nw: bit u set = step u not recurrent
 // (bit 0 = oldest).  run: recurrent steps carried in from earlier words, with
-// PRE added while the segment's first run is open.  emit(len) counts a closed
-// run of length 1 <= len < LCAP; slow(first) takes the run that closes at the
-// word's first non-recurrent step when it is >= LCAP long or is the
+// PRE added while the segment's first run is open.  emit(len, ok) counts a
+// closed run of length 1 <= len < LCAP when ok, and does nothing countable
+// otherwise (the device form always issues its add, to a dummy bin when !ok,
+// so the common case needs no branch); slow(first) takes the run that closes
+// at the word's first non-recurrent step when it is >= LCAP long or is the
 // segment's first run (first >= PRE: its length is first - PRE, 0 included).
 template <class Emit, class Slow>
 RQA_HD void consume(uint32_t nw, uint32_t &run, Emit &&emit, Slow &&slow) {
@@ -104,14 +106,14 @@ RQA_HD void consume(uint32_t nw, uint32_t &run, Emit &&emit, Slow &&slow) {
     const uint32_t tail = ffbh(nw);     // recurrent steps after the last non-recurrent one
     const uint32_t first = run + lead;  // the carried run, closed by that first step
     const bool closes = nw != 0u;
-    if (closes && first - 1u < (uint32_t)(LCAP - 1)) emit(first);
+    emit(first, closes && first - 1u < (uint32_t)(LCAP - 1));
     if (closes && first >= (uint32_t)LCAP) slow(first);
     // runs strictly inside the word: recurrent steps between the first and
     // last non-recurrent ones (for nw == 0 both shifts are by 31 and mid is 0)
     uint32_t mid = ~nw & (0xFFFFFFFFu << (lead & 31u)) & (0xFFFFFFFFu >> (tail & 31u));
     while (mid) {
         const uint32_t lo = ffbl(mid);
-        emit(ffbl(nw >> lo));           // ends at the next non-recurrent step
+        emit(ffbl(nw >> lo), true);     // ends at the next non-recurrent step
         mid &= mid + (1u << lo);        // clear that run
     }
     run = closes ? tail : run + 32u;

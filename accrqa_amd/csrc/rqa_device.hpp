@@ -108,6 +108,7 @@ constexpr int PBINS = PCAP - LCAP;   // private bins per threshold
 
 struct Sink {
     uint32_t *lh;                  // LDS: row t at t * LCAP
+    uint32_t *dummy;               // LDS: this lane's bin for adds that count nothing
     uint32_t *priv;                // workgroup slice: row t at t * PBINS, bin len - LCAP
     unsigned long long *hist;      // result: row t at t * (nc + 1)
     int64_t nc;
@@ -182,7 +183,7 @@ __device__ __forceinline__ void scan_words(const uint32_t (&R)[8], uint32_t keep
             const uint32_t nw = (G.ge(t + 1) & keep) | force;
             uint32_t *lht = K.lh + t * LCAP;
             consume(
-                nw, run[t], [&](uint32_t len) { lds_inc(&lht[len]); },
+                nw, run[t], [&](uint32_t len, bool ok) { lds_inc(ok ? &lht[ok ? len : 0u] : K.dummy); },
                 [&](uint32_t first) {
                     const uint32_t tt = opaque(t);
                     slow_run(first, cslot + (size_t)tt * cstride, K, tt);
@@ -327,6 +328,7 @@ __global__ __launch_bounds__(WG) RQA_WPE_LAM void k_lam(const F *__restrict__ x,
                                             uint2 *__restrict__ carry, uint32_t *__restrict__ priv_base,
                                             uint8_t *__restrict__ tiles) {
     __shared__ uint32_t lh[TG * LCAP];
+    __shared__ uint32_t dummy_bins[WAVE];
     __shared__ LutEntry<F> lut_s[NB_MAX];
     __shared__ F bnd_s[TG];
     for (int e = threadIdx.x; e < TG * LCAP; e += WG) lh[e] = 0;
@@ -335,7 +337,7 @@ __global__ __launch_bounds__(WG) RQA_WPE_LAM void k_lam(const F *__restrict__ x,
     priv_zero<TG>(priv_slice<TG>(priv_base), nt, WG);
     __syncthreads();
     uint32_t *priv = priv_slice<TG>(priv_base);
-    const Sink K{lh, priv, hist, nc};
+    const Sink K{lh, dummy_bins + (threadIdx.x & (WAVE - 1)), priv, hist, nc};
 
     const int lane = threadIdx.x & (WAVE - 1);
     const int wave = wave_id();
@@ -420,6 +422,7 @@ __global__ __launch_bounds__(WG) void k_det(const F *__restrict__ x, int64_t nc,
                                             unsigned long long *__restrict__ hist, uint2 *__restrict__ carry,
                                             uint32_t *__restrict__ priv_base) {
     __shared__ uint32_t lh[TG * LCAP];
+    __shared__ uint32_t dummy_bins[WAVE];
     __shared__ LutEntry<F> lut_s[NB_MAX];
     __shared__ F win[WAVES][DET_WIN];
     __shared__ F bnd_s[TG];
@@ -429,7 +432,7 @@ __global__ __launch_bounds__(WG) void k_det(const F *__restrict__ x, int64_t nc,
     priv_zero<TG>(priv_slice<TG>(priv_base), nt, WG);
     __syncthreads();
     uint32_t *priv = priv_slice<TG>(priv_base);
-    const Sink K{lh, priv, hist, nc};
+    const Sink K{lh, dummy_bins + (threadIdx.x & (WAVE - 1)), priv, hist, nc};
 
     const int lane = threadIdx.x & (WAVE - 1);
     const int wave = wave_id();
@@ -506,12 +509,13 @@ __global__ __launch_bounds__(WG) void k_det_scan(const uint8_t *__restrict__ til
                                                  int64_t nitems, unsigned long long *__restrict__ hist,
                                                  uint2 *__restrict__ carry, uint32_t *__restrict__ priv_base) {
     __shared__ uint32_t lh[TG * LCAP];
+    __shared__ uint32_t dummy_bins[WAVE];
     __shared__ uint32_t tl[WAVES][3 * WAVE * TREC];
     for (int e = threadIdx.x; e < TG * LCAP; e += WG) lh[e] = 0;
     priv_zero<TG>(priv_slice<TG>(priv_base), nt, WG);
     __syncthreads();
     uint32_t *priv = priv_slice<TG>(priv_base);
-    const Sink K{lh, priv, hist, nc};
+    const Sink K{lh, dummy_bins + (threadIdx.x & (WAVE - 1)), priv, hist, nc};
 
     const int lane = threadIdx.x & (WAVE - 1);
     const int wave = wave_id();
@@ -582,12 +586,13 @@ __global__ __launch_bounds__(WG) void k_lam_hscan(const uint8_t *__restrict__ ti
                                                   int64_t nitems, unsigned long long *__restrict__ hist,
                                                   uint2 *__restrict__ carry, uint32_t *__restrict__ priv_base) {
     __shared__ uint32_t lh[TG * LCAP];
+    __shared__ uint32_t dummy_bins[WAVE];
     __shared__ uint32_t tl[WAVES][WAVE * TREC];
     for (int e = threadIdx.x; e < TG * LCAP; e += WG) lh[e] = 0;
     priv_zero<TG>(priv_slice<TG>(priv_base), nt, WG);
     __syncthreads();
     uint32_t *priv = priv_slice<TG>(priv_base);
-    const Sink K{lh, priv, hist, nc};
+    const Sink K{lh, dummy_bins + (threadIdx.x & (WAVE - 1)), priv, hist, nc};
 
     const int lane = threadIdx.x & (WAVE - 1);
     const int wave = wave_id();
@@ -957,6 +962,7 @@ __global__ __launch_bounds__(WG) void k_lam_mx(const float *__restrict__ x, cons
                                                unsigned long long *__restrict__ hist, uint2 *__restrict__ carry,
                                                uint32_t *__restrict__ priv_base, uint8_t *__restrict__ tiles) {
     __shared__ uint32_t lh[TG * LCAP];
+    __shared__ uint32_t dummy_bins[WAVE];
     __shared__ uint8_t lut_s[NBX_LAM];
     __shared__ float bnd_s[NBND];
     __shared__ float rwin[WAVES][MXW];        // x[i0 .. i0 + 31 + (m-1)tau] of the current block
@@ -967,7 +973,7 @@ __global__ __launch_bounds__(WG) void k_lam_mx(const float *__restrict__ x, cons
     priv_zero<TG>(priv_slice<TG>(priv_base), nt, WG);
     __syncthreads();
     uint32_t *priv = priv_slice<TG>(priv_base);
-    const Sink K{lh, priv, hist, nc};
+    const Sink K{lh, dummy_bins + (threadIdx.x & (WAVE - 1)), priv, hist, nc};
 
     const int lane = threadIdx.x & (WAVE - 1);
     const int wave = wave_id();
@@ -1090,6 +1096,7 @@ __global__ __launch_bounds__(WG) void k_det_mx(const float *__restrict__ x, cons
                                                unsigned long long *__restrict__ hist, uint2 *__restrict__ carry,
                                                uint32_t *__restrict__ priv_base) {
     __shared__ uint32_t lh[TG * LCAP];
+    __shared__ uint32_t dummy_bins[WAVE];
     __shared__ uint8_t lut_s[NBX];
     __shared__ uint32_t skew[WAVES][WAVE * SKB / 4];
     __shared__ float bnd_s[NBND];
@@ -1101,7 +1108,7 @@ __global__ __launch_bounds__(WG) void k_det_mx(const float *__restrict__ x, cons
     priv_zero<TG>(priv_slice<TG>(priv_base), nt, WG);
     __syncthreads();
     uint32_t *priv = priv_slice<TG>(priv_base);
-    const Sink K{lh, priv, hist, nc};
+    const Sink K{lh, dummy_bins + (threadIdx.x & (WAVE - 1)), priv, hist, nc};
 
     const int lane = threadIdx.x & (WAVE - 1);
     const int wave = wave_id();

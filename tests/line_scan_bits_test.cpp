@@ -93,7 +93,10 @@ static Result scanned(const std::vector<int> &b, uint32_t start_run) {
             if (!b[w0 + u]) nw |= 1u << u;
         // steps past the end are filled as recurrent and taken back afterwards
         consume(
-            nw, run, [&](uint32_t len) { r.hist[len]++; },
+            nw, run,
+            [&](uint32_t len, bool ok) {
+                if (ok) r.hist[len]++;
+            },
             [&](uint32_t first) {
                 if (first >= PRE) {
                     CHECK(pre < 0, "prefix written twice\n");
