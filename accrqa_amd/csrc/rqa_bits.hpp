@@ -24,9 +24,34 @@ constexpr int LCAP = 128;              // run lengths < LCAP go to the workgroup
 constexpr uint32_t PRE = 0x80000000u;  // run counter flag: the segment's first run has not closed yet
 
 // Position of the lowest / highest set bit (as the hardware's find-first-bit
-// instructions): 0xFFFFFFFF for a zero word.
-RQA_HD uint32_t ffbl(uint32_t v) { return v ? (uint32_t)__builtin_ctz(v) : 0xFFFFFFFFu; }
-RQA_HD uint32_t ffbh(uint32_t v) { return v ? (uint32_t)__builtin_clz(v) : 0xFFFFFFFFu; }
+// instructions): 0xFFFFFFFF for a zero word.  On the device they are the
+// instructions themselves, so the compiler does not split the zero case off
+// into a select at every use.
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ __forceinline__ uint32_t ffbl(uint32_t v) {
+    uint32_t r;
+    asm("v_ffbl_b32 %0, %1" : "=v"(r) : "v"(v));
+    return r;
+}
+__device__ __forceinline__ uint32_t ffbh(uint32_t v) {
+    uint32_t r;
+    asm("v_ffbh_u32 %0, %1" : "=v"(r) : "v"(v));
+    return r;
+}
+#else
+inline uint32_t ffbl(uint32_t v) { return v ? (uint32_t)__builtin_ctz(v) : 0xFFFFFFFFu; }
+inline uint32_t ffbh(uint32_t v) { return v ? (uint32_t)__builtin_clz(v) : 0xFFFFFFFFu; }
+#endif
+
+// All-ones shifted right by (s mod 32), the shift count taken as the
+// hardware's funnel shift takes it (a word of ffbh(0) = 0xFFFFFFFF gives 1).
+RQA_HD uint32_t ones_shr(uint32_t s) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_alignbit(0u, 0xFFFFFFFFu, s);
+#else
+    return 0xFFFFFFFFu >> (s & 31u);
+#endif
+}
 
 // Bits of a rank in [0, TG].
 template <int TG>
@@ -47,9 +72,13 @@ RQA_HD void bit_swap(uint32_t &a, uint32_t &b, int s, uint32_t m) {
 // Bit planes of the packed ranks: P[k] bit (q + 8g) = bit k of the rank of
 // step q + 8g, i.e. bit u of P[k] is bit k of step u's rank.  Each byte
 // column is an 8x8 bit transpose (rows = registers, columns = rank bits),
-// done in three exchange stages.
+// done in three exchange stages.  Then steps outside `keep` get rank 0
+// (recurrent for every threshold) and steps in `force` the largest rank
+// (recurrent for none): B operations for the whole word instead of one per
+// threshold.
 template <int B>
-RQA_HD void rank_planes(const uint32_t (&R)[8], uint32_t (&P)[B]) {
+RQA_HD void rank_planes(const uint32_t (&R)[8], uint32_t (&P)[B], uint32_t keep = 0xFFFFFFFFu,
+                        uint32_t force = 0u) {
     uint32_t r[8];
 #pragma unroll
     for (int q = 0; q < 8; q++) r[q] = R[q];
@@ -63,7 +92,7 @@ RQA_HD void rank_planes(const uint32_t (&R)[8], uint32_t (&P)[B]) {
 #pragma unroll
     for (int q = 0; q < 8; q += 2) bit_swap(r[q], r[q + 1], 1, 0x55555555u);
 #pragma unroll
-    for (int k = 0; k < B; k++) P[k] = r[k];
+    for (int k = 0; k < B; k++) P[k] = (r[k] & keep) | force;
 }
 
 // Comparator tree over the planes: ge(c) = mask of steps with rank >= c, so
@@ -110,11 +139,13 @@ RQA_HD void consume(uint32_t nw, uint32_t &run, Emit &&emit, Slow &&slow) {
     if (closes && first >= (uint32_t)LCAP) slow(first);
     // runs strictly inside the word: recurrent steps between the first and
     // last non-recurrent ones (for nw == 0 both shifts are by 31 and mid is 0)
-    uint32_t mid = ~nw & (0xFFFFFFFFu << (lead & 31u)) & (0xFFFFFFFFu >> (tail & 31u));
+    uint32_t mid = ~nw & (0xFFFFFFFFu << (lead & 31u)) & ones_shr(tail);
     while (mid) {
-        const uint32_t lo = ffbl(mid);
-        emit(ffbl(nw >> lo), true);     // ends at the next non-recurrent step
-        mid &= mid + (1u << lo);        // clear that run
+        // both operands are non-zero: mid by the loop test, nw >> lo because
+        // an inner run ends before the word's last non-recurrent step
+        const uint32_t lo = (uint32_t)__builtin_ctz(mid);
+        emit((uint32_t)__builtin_ctz(nw >> lo), true);   // ends at the next non-recurrent step
+        mid &= mid + (1u << lo);                         // clear that run
     }
     run = closes ? tail : run + 32u;
 }

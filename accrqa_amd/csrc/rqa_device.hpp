@@ -168,19 +168,32 @@ __device__ __forceinline__ void close_run(uint2 *slot, uint32_t run) {
 
 // All thresholds of one 32-step block of one lane.  nw(t) = (ge(t+1) & keep)
 // | force: keep clears steps that count as recurrent padding, force sets
-// steps that decide 0.  Histogram row t starts at hist + t*(nc+1), the
-// segment's carry slot of threshold t at cslot + t*cstride.
+// steps that decide 0 (both applied to the rank planes).  Histogram row t
+// starts at hist + t*(nc+1), the segment's carry slot of threshold t at
+// cslot + t*cstride.  Called by all lanes of the wave together.
+//
+// Thresholds are sorted, so a step that does not recur at t does not recur at
+// any lower threshold, and a run open at t is open at every higher one: the
+// thresholds at which every lane's word is all non-recurrent and no lane has
+// a run open form a prefix.  consume() would change nothing there (no run
+// closes or opens), so the wave skips them, testing up to the first threshold
+// that fails.
 template <int TG>
 __device__ __forceinline__ void scan_words(const uint32_t (&R)[8], uint32_t keep, uint32_t force, uint32_t (&run)[TG],
                                            int nt, const Sink &K, uint2 *cslot, size_t cstride) {
     constexpr int B = rank_bits<TG>();
     uint32_t Pl[B];
-    rank_planes<B>(R, Pl);
+    rank_planes<B>(R, Pl, keep, force);
     const GeTree<B> G(Pl);
+    bool lead_in = true;
 #pragma unroll
     for (int t = 0; t < TG; t++) {
         if (t < nt) {
-            const uint32_t nw = (G.ge(t + 1) & keep) | force;
+            const uint32_t nw = G.ge(t + 1);
+            if (lead_in) {
+                if (__builtin_amdgcn_ballot_w64((~nw | run[t]) != 0u) == 0ull) continue;
+                lead_in = false;
+            }
             uint32_t *lht = K.lh + t * LCAP;
             consume(
                 nw, run[t], [&](uint32_t len, bool ok) { lds_inc(ok ? &lht[ok ? len : 0u] : K.dummy); },
@@ -540,19 +553,34 @@ __global__ __launch_bounds__(WG) void k_det_scan(const uint8_t *__restrict__ til
 #pragma unroll
         for (int t = 0; t < TG; t++) run[t] = rv ? PRE : 0u;
 
-        for (int64_t s0 = sb; s0 < se; s0 += BLK) {
-            const int64_t I = s0 / BLK, c0 = s0 + r0, J0 = c0 / WAVE;
-            wave_sync();   // the previous block's gathers are done
+        // Tile records travel from HBM one block ahead: pf holds block s0 +
+        // BLK's records while block s0 is gathered and scanned.
+        uint4 pf[6];
+        auto fetch = [&](int64_t s) {
+            const int64_t I = s / BLK, J0 = (s + r0) / WAVE;
 #pragma unroll
             for (int t = 0; t < 3; t++) {
                 if (J0 + t < nbands) {
                     const uint4 *src = (const uint4 *)(tiles + tile_index(I, J0 + t, nbands) * TILE_BYTES) + 2 * lane;
-                    const uint4 a = src[0], b = src[1];
+                    pf[2 * t] = src[0];
+                    pf[2 * t + 1] = src[1];
+                }
+            }
+        };
+        if (sb < se) fetch(sb);
+        for (int64_t s0 = sb; s0 < se; s0 += BLK) {
+            const int64_t c0 = s0 + r0, J0 = c0 / WAVE;
+            wave_sync();   // the previous block's gathers are done
+#pragma unroll
+            for (int t = 0; t < 3; t++) {
+                if (J0 + t < nbands) {
+                    const uint4 a = pf[2 * t], b = pf[2 * t + 1];
                     uint32_t *d = tw + (t * WAVE + lane) * TREC;
                     d[0] = a.x; d[1] = a.y; d[2] = a.z; d[3] = a.w;
                     d[4] = b.x; d[5] = b.y; d[6] = b.z; d[7] = b.w;
                 }
             }
+            if (s0 + BLK < se) fetch(s0 + BLK);
             wave_sync();
             // step u of lane l: column c0 + l + u, i.e. record (c0 - 64 J0) + l + u
             const int base = (int)(c0 - J0 * WAVE) + lane;
@@ -619,18 +647,26 @@ __global__ __launch_bounds__(WG) void k_lam_hscan(const uint8_t *__restrict__ ti
 #pragma unroll
         for (int t = 0; t < TG; t++) run[t] = jv ? PRE : 0u;
 
+        // Row blocks past the matrix hold no tile (their lanes are idle).  The
+        // lane's record travels from HBM one block ahead: v0, v1 hold block
+        // c0 + BLK's while block c0 is gathered and scanned.
+        const bool has_rows = BLK * I < nc;
+        uint4 v0 = make_uint4(0u, 0u, 0u, 0u), v1 = v0;
+        auto fetch = [&](int64_t c) {
+            const uint4 *src = (const uint4 *)(tiles + tile_index(I, c / WAVE, nbands) * TILE_BYTES +
+                                               (32 * ((c / BLK) & 1) + rr) * 32);
+            v0 = src[0];
+            v1 = src[1];
+        };
+        if (has_rows && cb < ce) fetch(cb);
         for (int64_t c0 = cb; c0 < ce; c0 += BLK) {
             const int nb = ce - c0 < BLK ? (int)(ce - c0) : BLK;
-            const int64_t Jc = c0 / WAVE;
-            const int half = (int)((c0 / BLK) & 1);
             wave_sync();   // the previous block's gathers are done
-            if (BLK * I < nc) {   // row blocks past the matrix hold no tile (their lanes are idle)
-                const uint4 *src = (const uint4 *)(tiles + tile_index(I, Jc, nbands) * TILE_BYTES +
-                                                   (32 * half + rr) * 32);
-                const uint4 v0 = src[0], v1 = src[1];
+            if (has_rows) {
                 uint32_t *d = tw + lane * TREC;
                 d[0] = v0.x; d[1] = v0.y; d[2] = v0.z; d[3] = v0.w;
                 d[4] = v1.x; d[5] = v1.y; d[6] = v1.z; d[7] = v1.w;
+                if (c0 + BLK < ce) fetch(c0 + BLK);
             }
             wave_sync();
             uint32_t R[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};

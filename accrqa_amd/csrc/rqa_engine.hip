@@ -51,7 +51,11 @@ struct Ctx {
     int ncu = 0;
     hipStream_t stream = nullptr;
     DevBuf x, hist, carry, metric, scan, items_rr, items_det, items_lamv, items_lamh, counts, perm, lmins, lut, nrm, nmax, lut8, out_rr,
-        out_det, out_lam, priv, tiles;
+        out_det, out_lam, priv, tiles, dump;
+    // Raw-result sinks of accrqa_amd_histograms_*: when set, launch_group
+    // copies each group's counts / h_D rows / h_V rows there, indexed by the
+    // caller's threshold order.  Null for every other entry point.
+    unsigned long long *dump_rr = nullptr, *dump_det = nullptr, *dump_lam = nullptr;
     int64_t items_nc = -1;     // (nc, seg_len) the item tables were built for
     int items_L = -1;
     int64_t n_items_rr = 0, n_items_det = 0, n_items_lamv = 0, n_items_lamh = 0;
@@ -119,10 +123,12 @@ int init_ctx() {
         if (g_ctx.device >= 0) {
             int cur = dev;
             (void)hipSetDevice(g_ctx.device);
-            DevBuf *all[] = {&g_ctx.x, &g_ctx.hist, &g_ctx.carry, &g_ctx.metric, &g_ctx.scan, &g_ctx.items_rr,
-                             &g_ctx.items_det, &g_ctx.counts, &g_ctx.perm, &g_ctx.lmins, &g_ctx.lut, &g_ctx.nrm, &g_ctx.nmax,
-                             &g_ctx.lut8, &g_ctx.out_rr,
-                             &g_ctx.out_det, &g_ctx.out_lam};
+            DevBuf *all[] = {&g_ctx.x,       &g_ctx.hist,       &g_ctx.carry,      &g_ctx.metric,
+                             &g_ctx.scan,    &g_ctx.items_rr,   &g_ctx.items_det,  &g_ctx.items_lamv,
+                             &g_ctx.items_lamh, &g_ctx.counts,  &g_ctx.perm,       &g_ctx.lmins,
+                             &g_ctx.lut,     &g_ctx.nrm,        &g_ctx.nmax,       &g_ctx.lut8,
+                             &g_ctx.out_rr,  &g_ctx.out_det,    &g_ctx.out_lam,    &g_ctx.priv,
+                             &g_ctx.tiles,   &g_ctx.dump};
             g_ctx.items_nc = -1;
             for (DevBuf *b : all) {
                 if (b->p) (void)hipFree(b->p);
@@ -248,6 +254,25 @@ void fill_zero(unsigned long long *p, size_t n, hipStream_t st) {
     if (!n) return;
     unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
     hipLaunchKernelGGL(k_fill_u64, dim3(blocks), dim3(256), 0, st, p, n);
+}
+
+// Row t of a group's raw results (hs words) -> out + perm[t] * hs.
+__global__ void k_dump_rows(const unsigned long long *__restrict__ src, size_t hs, const int *__restrict__ perm,
+                            unsigned long long *__restrict__ out) {
+    const int t = blockIdx.y;
+    const unsigned long long *s = src + (size_t)t * hs;
+    unsigned long long *o = out + (size_t)perm[t] * hs;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < hs; i += (size_t)gridDim.x * blockDim.x)
+        o[i] = s[i];
+}
+
+int dump_rows(const unsigned long long *src, int nt, size_t hs, const int *d_perm, unsigned long long *out,
+              hipStream_t st) {
+    if (!out || nt <= 0) return E_OK;
+    const unsigned bx = (unsigned)std::min<size_t>((hs + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_dump_rows, dim3(bx, nt), dim3(256), 0, st, src, hs, d_perm, out);
+    HIP_TRY(hipGetLastError());
+    return E_OK;
 }
 
 // ---------------------------------------------------------------- profiling
@@ -537,6 +562,7 @@ int launch_group(const F *dx, int64_t nc, int tau, int m, const F *bounds, int n
                            nt, nc, nc, 0, tiles ? 1 : 0, L, nseg, cseg_lam, hist);
         prof_end(st);
         HIP_TRY(hipGetLastError());
+        if ((e = dump_rows(hist, nt, hs, d_perm, c.dump_lam, st))) return e;
         // The LAM histogram covers every recurrent pair of the full matrix once,
         // so its sum of l*h[l] is the RR count as well.
         prof_begin("k_finalize", st);
@@ -547,6 +573,7 @@ int launch_group(const F *dx, int64_t nc, int tau, int m, const F *bounds, int n
         HIP_TRY(hipGetLastError());
     }
     if (want_rr) {
+        if ((e = dump_rows(counts, nt, 1, d_perm, c.dump_rr, st))) return e;
         hipLaunchKernelGGL((rqa::k_rr_out<F>), dim3(1), dim3(64), 0, st, (const unsigned long long *)counts, nt, nc,
                            d_perm, d_rr);
         HIP_TRY(hipGetLastError());
@@ -610,6 +637,8 @@ int launch_group(const F *dx, int64_t nc, int tau, int m, const F *bounds, int n
                            (unsigned long long *)nullptr);
         prof_end(st);
         HIP_TRY(hipGetLastError());
+        // finalize leaves h_D doubled with the line-of-identity bin
+        if ((e = dump_rows(hist, nt, hs, d_perm, c.dump_det, st))) return e;
     }
     return E_OK;
 }
@@ -624,6 +653,7 @@ int launch_never(int64_t nc, int nt, bool want_rr, bool want_det, bool want_lam,
     if ((e = ensure(c.hist, (size_t)nt * hs * 8))) return e;
     if ((e = ensure(c.metric, (size_t)nt * hs * 8))) return e;
     if ((e = ensure(c.scan, (size_t)nt * hs * 8))) return e;
+    if ((e = ensure(c.counts, std::max<size_t>(64, (size_t)nt) * sizeof(unsigned long long)))) return e;
     unsigned long long *hist = (unsigned long long *)c.hist.p;
     unsigned long long *counts = (unsigned long long *)c.counts.p;
     if (want_rr) {
@@ -631,6 +661,7 @@ int launch_never(int64_t nc, int nt, bool want_rr, bool want_det, bool want_lam,
         hipLaunchKernelGGL((rqa::k_rr_out<F>), dim3((nt + 63) / 64), dim3(64), 0, st,
                            (const unsigned long long *)counts, nt, nc, d_perm, d_rr);
         HIP_TRY(hipGetLastError());
+        if ((e = dump_rows(counts, nt, 1, d_perm, c.dump_rr, st))) return e;
     }
     for (int kind = 0; kind < 2; kind++) {
         const bool det = kind == 1;
@@ -640,6 +671,7 @@ int launch_never(int64_t nc, int nt, bool want_rr, bool want_det, bool want_lam,
                            d_lmins, nl, (unsigned long long *)c.metric.p, (unsigned long long *)c.scan.p,
                            det ? d_det : d_lam, det ? 1 : 0, d_perm, T, (unsigned long long *)nullptr);
         HIP_TRY(hipGetLastError());
+        if ((e = dump_rows(hist, nt, hs, d_perm, det ? c.dump_det : c.dump_lam, st))) return e;
     }
     return E_OK;
 }
@@ -843,33 +875,42 @@ void host_hist(const F *x, size_t n, int tau, int emb, const F *thr, int T, int 
     hipStream_t st = c.stream;
     const int64_t nc = (int64_t)n - (int64_t)(emb - 1) * tau;
     const size_t hs = (size_t)nc + 1;
-    // Histograms come from the same kernels, one threshold at a time so the
-    // group buffer holds exactly the threshold asked for; the finalize output
-    // is discarded (l_min = 0 is always valid).
+    // The histograms come from the same kernels and launches as RR/DET/LAM,
+    // up to HCH thresholds per run_block (one threshold group), each group's
+    // raw rows copied to the dump sinks in the caller's order; the measures
+    // finalize writes are discarded (l_min = 0 is always valid).
+    constexpr int HCH = 32;
     int lm0 = 0;
-    if ((e = ensure(c.out_rr, 64 * sizeof(F))) || (e = ensure(c.out_det, 64 * sizeof(F))) ||
-        (e = ensure(c.out_lam, 64 * sizeof(F)))) {
+    const size_t rows = (size_t)HCH * hs;
+    const size_t words = HCH + ((which & 2) ? rows : 0) + ((which & 4) ? rows : 0);
+    if ((e = ensure(c.out_rr, HCH * sizeof(F))) || (e = ensure(c.out_det, HCH * 5 * sizeof(F))) ||
+        (e = ensure(c.out_lam, HCH * 5 * sizeof(F))) || (e = ensure(c.dump, words * 8))) {
         *error = e;
         return;
     }
-    for (int t = 0; t < T; t++) {
-        for (int k = 0; k < 3; k++) {
-            if (!(which & (1 << k))) continue;
-            e = run_block<F>((const F *)c.x.p, n, tau, emb, thr + t, 1, &lm0, 1, dist,
-                             k == 0 ? (F *)c.out_rr.p : nullptr, k == 1 ? (F *)c.out_det.p : nullptr,
-                             k == 2 ? (F *)c.out_lam.p : nullptr, st);
-            hipError_t he = hipSuccess;
-            if (!e && k == 0) he = hipMemcpyAsync(rr + t, c.counts.p, 8, hipMemcpyDeviceToHost, st);
-            if (!e && k == 1) he = hipMemcpyAsync(hd + (size_t)t * hs, c.hist.p, hs * 8, hipMemcpyDeviceToHost, st);
-            if (!e && k == 2) he = hipMemcpyAsync(hv + (size_t)t * hs, c.hist.p, hs * 8, hipMemcpyDeviceToHost, st);
-            if (!e && he == hipSuccess) he = hipStreamSynchronize(st);
-            if (!e && he != hipSuccess) e = map_err(he);
-            if (e) {
-                *error = e;
-                return;
-            }
-        }
+    unsigned long long *d_rr = (unsigned long long *)c.dump.p;
+    unsigned long long *d_hd = d_rr + HCH;
+    unsigned long long *d_hv = d_hd + ((which & 2) ? rows : 0);
+    for (int t0 = 0; t0 < T && !e; t0 += HCH) {
+        const int nt = std::min(HCH, T - t0);
+        fill_zero(d_rr, words, st);
+        c.dump_rr = (which & 1) ? d_rr : nullptr;
+        c.dump_det = (which & 2) ? d_hd : nullptr;
+        c.dump_lam = (which & 4) ? d_hv : nullptr;
+        e = run_block<F>((const F *)c.x.p, n, tau, emb, thr + t0, nt, &lm0, 1, dist,
+                         (which & 1) ? (F *)c.out_rr.p : nullptr, (which & 2) ? (F *)c.out_det.p : nullptr,
+                         (which & 4) ? (F *)c.out_lam.p : nullptr, st);
+        c.dump_rr = c.dump_det = c.dump_lam = nullptr;
+        hipError_t he = hipSuccess;
+        if (!e && (which & 1)) he = hipMemcpyAsync(rr + t0, d_rr, (size_t)nt * 8, hipMemcpyDeviceToHost, st);
+        if (!e && he == hipSuccess && (which & 2))
+            he = hipMemcpyAsync(hd + (size_t)t0 * hs, d_hd, (size_t)nt * hs * 8, hipMemcpyDeviceToHost, st);
+        if (!e && he == hipSuccess && (which & 4))
+            he = hipMemcpyAsync(hv + (size_t)t0 * hs, d_hv, (size_t)nt * hs * 8, hipMemcpyDeviceToHost, st);
+        if (!e && he == hipSuccess) he = hipStreamSynchronize(st);
+        if (!e && he != hipSuccess) e = map_err(he);
     }
+    *error = e;
 }
 
 template <typename F>

@@ -209,6 +209,27 @@ def test_tie_thresholds_exact(amd, oracle):
             check_case(amd, oracle, x, [1, 4], [3, 6], [2], thrs, dist, f"ties {dt} {dist}")
 
 
+def test_many_thresholds_with_never_recurring(amd, oracle):
+    """More thresholds than one group holds (100: three recurring groups) and
+    more never-recurring thresholds (70, zero or negative) than the count
+    buffer's default size, shuffled: measures and raw histograms per threshold,
+    in the caller's order."""
+    rng = np.random.default_rng(21)
+    for dt in (np.float32, np.float64):
+        x = rng.standard_normal(700).astype(dt)
+        thrs = np.concatenate([-rng.random(69), [0.0], np.linspace(0.05, 2.5, 30)]).astype(dt)
+        thrs = thrs[rng.permutation(thrs.size)]
+        thrs = np.concatenate([thrs, thrs[:5]])   # duplicates across groups
+        for dist in (1, 2):
+            say("many thresholds", dt, dist)
+            check_case(amd, oracle, x, [2], [3], [2], list(thrs), dist, f"many {dt} {dist}")
+            h = amd.histograms(x, 2, 3, thrs, dist, which=7)
+            for k, t in enumerate(thrs):
+                assert int(h["rr"][k]) == oracle.rr_count(x, t, 2, 3, dist), (dt, dist, k, t)
+                assert np.array_equal(h["det"][k], oracle.hist("det", x, t, 2, 3, dist)), (dt, dist, k, t)
+                assert np.array_equal(h["lam"][k], oracle.hist("lam", x, t, 2, 3, dist)), (dt, dist, k, t)
+
+
 def test_argument_errors_on_gpu(amd):
     x = np.ones(10, dtype=np.float32)
     with pytest.raises(amd.AccrqaError) as ei:
