@@ -47,8 +47,10 @@ constexpr int WAVES = WG / WAVE;
 #ifndef RQA_WPE_LAM
 #define RQA_WPE_LAM
 #endif
+// k_lam_mx: three waves per SIMD (168 VGPRs for 16 thresholds, no spills);
+// left to itself the compiler takes a few registers more and halves that.
 #ifndef RQA_WPE_MX
-#define RQA_WPE_MX
+#define RQA_WPE_MX __attribute__((amdgpu_waves_per_eu(3)))
 #endif
 constexpr int DET_WIN = 992;       // LDS samples per wave for the k_det lane window
 
@@ -915,19 +917,65 @@ typedef float floatx16 __attribute__((ext_vector_type(16)));
 // x[i0 ..] and x[c0 ..] (LDS windows); lane l supplies A[row l%32][k l/32]
 // and B[k l/32][col l%32], i.e. dimension 2 k2 + h.  For odd m the last step
 // of half 1 reads dimension m-1 again and zeroes its row operand.
-__device__ __forceinline__ floatx16 gram_tile(const float *rw, const float *cw, int ob, int omax, int m2, int tau) {
+// Per-wave operand windows of the MFMA kernels: rw holds the series values
+// of the block's rows, cw those of its NT column tiles (tile p at +32).  Both
+// are read at one per-lane offset o (cw tile p at o + 32p), so one LDS
+// address per operand step serves the row operand and every column tile,
+// the tiles being immediate offsets from it.  Slot ZS of rw and slots
+// ZS + 32p of cw hold 0: the operands of dimensions k >= m (both factors 0,
+// so the product is 0 whatever else the windows hold).
+template <int NT>
+struct MxWin {
+    static constexpr int ZS = MXW + 32 * (NT - 1);   // past the content of both windows
+    static constexpr int CWO = ZS + 1;                // cw[0] relative to rw[0], in floats
+    float rw[ZS + 1];
+    float cw[ZS + 32 * (NT - 1) + 1];
+};
+
+template <int NT>
+__device__ __forceinline__ void mx_win_init(MxWin<NT> &W, int lane) {
+    if (lane == 0) W.rw[MxWin<NT>::ZS] = 0.f;
+    if (lane < NT) W.cw[MxWin<NT>::ZS + 32 * lane] = 0.f;
+}
+
+// Operand k of the Gram tiles: lane (col, h) supplies dimension k = 2u + h of
+// step u of a chunk, at window offset col + k*tau (the zero slot for k >= m).
+// The first chunk's eight addresses depend only on the lane, m and tau.
+template <int NT>
+__device__ __forceinline__ void gram_addrs(const MxWin<NT> &W, int col, int h, int m, int tau, const float *(&pa)[8]) {
+    const float *wb = W.rw;
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+        const int k = 2 * u + h;
+        pa[u] = wb + (k < m ? col + k * tau : MxWin<NT>::ZS);
+    }
+}
+
+// One 32x32 Gram tile (column tile p): G[row][32p + col] over the m dimensions.
+template <int NT, int p>
+__device__ __forceinline__ floatx16 gram_tile(const MxWin<NT> &W, const float *const (&pa)[8], int col, int h, int m,
+                                              int tau) {
+    constexpr int CW = MxWin<NT>::CWO + 32 * p;
     floatx16 acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    // chunks of eight steps, all operand loads in flight before the MFMA
-    // chain; steps past m2 (and dimension m for odd m) get a zero row operand
-    for (int k2 = 0; k2 < m2; k2 += 8) {
+    {
         float a[8], b[8];
 #pragma unroll
         for (int u = 0; u < 8; u++) {
-            const int oraw = ob + 2 * (k2 + u) * tau;
-            const int o = min(oraw, omax);
-            const float av = rw[o];
-            a[u] = oraw <= omax ? av : 0.f;
-            b[u] = cw[o];
+            a[u] = pa[u][0];
+            b[u] = pa[u][CW];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[u], b[u], acc, 0, 0, 0);
+    }
+    // dimensions 16 and up (m > 16), addresses formed per chunk
+    for (int k0 = 16; k0 < m; k0 += 16) {
+        float a[8], b[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int k = k0 + 2 * u + h;
+            const float *q = W.rw + (k < m ? col + k * tau : MxWin<NT>::ZS);
+            a[u] = q[0];
+            b[u] = q[CW];
         }
 #pragma unroll
         for (int u = 0; u < 8; u++) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[u], b[u], acc, 0, 0, 0);
@@ -992,7 +1040,7 @@ __device__ __forceinline__ void tile_row_norms(const float *__restrict__ nrm, in
 // leaves lane l (half 0) with column j0 + l and lane l (half 1) with column
 // j0 + l, all 32 rows, and the scan is k_lam's.
 template <int TG>
-__global__ __launch_bounds__(WG) void k_lam_mx(const float *__restrict__ x, const float *__restrict__ nrm, int64_t nc,
+__global__ __launch_bounds__(WG) RQA_WPE_MX void k_lam_mx(const float *__restrict__ x, const float *__restrict__ nrm, int64_t nc,
                                                int tau, int m, Bounds<float, TG> S, int nt,
                                                const uint8_t *__restrict__ lut, MxParams P, int seg_len, LamWork W,
                                                unsigned long long *__restrict__ hist, uint2 *__restrict__ carry,
@@ -1001,8 +1049,8 @@ __global__ __launch_bounds__(WG) void k_lam_mx(const float *__restrict__ x, cons
     __shared__ uint32_t dummy_bins[WAVE];
     __shared__ uint8_t lut_s[NBX_LAM];
     __shared__ float bnd_s[NBND];
-    __shared__ float rwin[WAVES][MXW];        // x[i0 .. i0 + 31 + (m-1)tau] of the current block
-    __shared__ float cwin[WAVES][MXW + 32];   // x[j0 .. j0 + 63 + (m-1)tau] of the current item
+    // rw: x[i0 .. i0 + 31 + (m-1)tau] of the current block; cw: x[j0 .. j0 + 63 + (m-1)tau] of the item
+    __shared__ MxWin<2> win[WAVES];
     for (int e = threadIdx.x; e < TG * LCAP; e += WG) lh[e] = 0;
     for (int e = threadIdx.x; e < P.nb / 4; e += WG) ((uint32_t *)lut_s)[e] = ((const uint32_t *)lut)[e];
     if (threadIdx.x < NBND) bnd_s[threadIdx.x] = (int)threadIdx.x < nt ? S.s[threadIdx.x] : __int_as_float(0x7F800000);
@@ -1020,9 +1068,11 @@ __global__ __launch_bounds__(WG) void k_lam_mx(const float *__restrict__ x, cons
     const int64_t nbands = (nc + WAVE - 1) / WAVE;
     const int64_t nitems = W.items ? W.nitems : nbands * W.nseg;
     const size_t cstride = (size_t)nc * W.cseg;
-    const int m2 = (m + 1) / 2;
-    const int ob = col + h * tau, omax = col + (m - 1) * tau;   // operand offsets of dimension h, m-1
-    float *rw = rwin[wave], *cw = cwin[wave];
+    MxWin<2> &Wn = win[wave];
+    float *rw = Wn.rw, *cw = Wn.cw;
+    mx_win_init(Wn, lane);
+    const float *pa[8];
+    gram_addrs(Wn, col, h, m, tau, pa);
 
     for (int64_t item = (int64_t)blockIdx.x * WAVES + wave; item < nitems; item += (int64_t)gridDim.x * WAVES) {
         int64_t band;
@@ -1070,14 +1120,21 @@ __global__ __launch_bounds__(WG) void k_lam_mx(const float *__restrict__ x, cons
 #pragma unroll
             for (int e = 0; e < 16; e++) rowm |= ((8 * (e >> 2) + 4 * h + (e & 3)) < nb ? 1u : 0u) << e;
             uint32_t Rr[8], fm = 0u;
-#pragma unroll
-            for (int p = 0; p < 2; p++) {
-                const floatx16 acc = gram_tile(rw, cw + 32 * p, ob, omax, m2, tau);
+            {
+                const floatx16 acc = gram_tile<2, 0>(Wn, pa, col, h, m, tau);
                 uint32_t Rq[4], f;
-                tile_ranks(acc, ni, nrm[j0 + 32 * p + col], lut_s, P, bnd_s, Rq, f);
+                tile_ranks(acc, ni, nrm[j0 + col], lut_s, P, bnd_s, Rq, f);
 #pragma unroll
-                for (int c = 0; c < 4; c++) Rr[4 * p + c] = Rq[c];
-                if (j0 + 32 * p + col < nc) fm |= (f & rowm) << (16 * p);
+                for (int c = 0; c < 4; c++) Rr[c] = Rq[c];
+                if (j0 + col < nc) fm |= f & rowm;
+            }
+            {
+                const floatx16 acc = gram_tile<2, 1>(Wn, pa, col, h, m, tau);
+                uint32_t Rq[4], f;
+                tile_ranks(acc, ni, nrm[j0 + 32 + col], lut_s, P, bnd_s, Rq, f);
+#pragma unroll
+                for (int c = 0; c < 4; c++) Rr[4 + c] = Rq[c];
+                if (j0 + 32 + col < nc) fm |= (f & rowm) << 16;
             }
             // exact re-decision of the ambiguous elements by their own lane
             while (fm) {
@@ -1136,8 +1193,8 @@ __global__ __launch_bounds__(WG) void k_det_mx(const float *__restrict__ x, cons
     __shared__ uint8_t lut_s[NBX];
     __shared__ uint32_t skew[WAVES][WAVE * SKB / 4];
     __shared__ float bnd_s[NBND];
-    __shared__ float rwin[WAVES][MXW];             // x[s0 .. s0 + 31 + (m-1)tau]
-    __shared__ float cwin[WAVES][MXW + WAVE];      // x[c0 .. c0 + 95 + (m-1)tau]
+    // rw: x[s0 .. s0 + 31 + (m-1)tau]; cw: x[c0 .. c0 + 95 + (m-1)tau]
+    __shared__ MxWin<3> win[WAVES];
     for (int e = threadIdx.x; e < TG * LCAP; e += WG) lh[e] = 0;
     for (int e = threadIdx.x; e < P.nb / 4; e += WG) ((uint32_t *)lut_s)[e] = ((const uint32_t *)lut)[e];
     if (threadIdx.x < NBND) bnd_s[threadIdx.x] = (int)threadIdx.x < nt ? S.s[threadIdx.x] : __int_as_float(0x7F800000);
@@ -1151,10 +1208,12 @@ __global__ __launch_bounds__(WG) void k_det_mx(const float *__restrict__ x, cons
     const int col = lane & 31;
     const int h = lane >> 5;
     uint8_t *sk8 = (uint8_t *)skew[wave];
-    float *rw = rwin[wave], *cw = cwin[wave];
+    MxWin<3> &Wn = win[wave];
+    float *rw = Wn.rw, *cw = Wn.cw;
+    mx_win_init(Wn, lane);
+    const float *pa[8];
+    gram_addrs(Wn, col, h, m, tau, pa);
     const size_t cstride = (size_t)(nc - 1) * nseg;
-    const int m2 = (m + 1) / 2;
-    const int ob = col + h * tau, omax = col + (m - 1) * tau;   // operand offsets of dimension h, m-1
     const int rspan = 32 + (m - 1) * tau, cspan = 96 + (m - 1) * tau;   // <= MXW, MXW + 64
 
     for (int64_t item = (int64_t)blockIdx.x * WAVES + wave; item < nitems; item += (int64_t)gridDim.x * WAVES) {
@@ -1212,7 +1271,9 @@ __global__ __launch_bounds__(WG) void k_det_mx(const float *__restrict__ x, cons
             uint64_t fma = 0;
 #pragma unroll
             for (int p = 0; p < 3; p++) {
-                const floatx16 acc = gram_tile(rw, cw + 32 * p, ob, omax, m2, tau);
+                const floatx16 acc = p == 0   ? gram_tile<3, 0>(Wn, pa, col, h, m, tau)
+                                     : p == 1 ? gram_tile<3, 1>(Wn, pa, col, h, m, tau)
+                                              : gram_tile<3, 2>(Wn, pa, col, h, m, tau);
                 uint32_t Rq[4], fm;
                 tile_ranks(acc, ni, nrm[c0 + 32 * p + col], lut_s, P, bnd_s, Rq, fm);
                 uint32_t need = 0;

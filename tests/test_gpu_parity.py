@@ -338,6 +338,11 @@ def test_mfma_form_bit_exact(mfma_form, oracle):
     cases.append(("special", xs, [1], [9], [2], [-1.0, 0.0, 0.5, 2.0, np.inf]))
     cases.append(("m33", rng.standard_normal(2100).astype(np.float32), [2], [33], [2, 4],
                   list(np.linspace(1.0, 12.0, 7).astype(np.float32))))
+    # tau = 0 (every coordinate the same sample): dimensions past m must add
+    # nothing although their window offsets coincide with real ones; odd m,
+    # unused steps in the first chunk of 16 dimensions, and a second chunk
+    cases.append(("tau0", rng.standard_normal(1500).astype(np.float32), [0, 3], [10, 21, 5], [2],
+                  list(np.linspace(0.1, 4.0, 9).astype(np.float32))))
     for name, x, taus, embs, lm, thrs in cases:
         say("mfma", name)
         check_case(amd, oracle, x, taus, embs, lm, thrs, 1, f"mfma {name}")
