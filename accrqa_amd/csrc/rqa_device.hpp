@@ -511,91 +511,114 @@ __global__ __launch_bounds__(WG) void k_det(const F *__restrict__ x, int64_t nc,
 }
 
 // ---------------------------------------------------------------- k_det_scan
-// DET pass over the rank tiles the LAM pass kept.  Work items as k_det.  The
-// pairs of a block, (s0+u, s0+r0+l+u) for lane l and step u < 32, lie in row
-// block s0/32 and columns c0 .. c0+94 (c0 = s0 + r0): at most three
-// consecutive tiles, which the wave copies to LDS (one 36-byte record per
-// tile lane: conflict-free gathers) before each lane collects its diagonal.
-constexpr int TREC = 9;   // LDS words per tile lane (8 + 1 pad)
+// DET pass over the rank tiles the LAM pass kept.  Work item = (group of
+// four strips of 64 diagonals, step segment): wave w takes strip w, the
+// diagonals r0 = rg + 64w + l (rg = 1 + 256 * group).  The pairs of a 32-step
+// block, (s0+u, s0+r0+l+u), lie in row block s0/32 and columns s0+r0 ..
+// s0+r0+94: tiles J0+w .. J0+w+2 (J0 = band of s0+rg), so the group's block
+// needs the six tiles J0 .. J0+5, each read once and shared through LDS (one
+// 36-byte record per tile lane: conflict-free gathers), against three per
+// strip when the waves fetch their own.  The next block's records are in
+// registers while a block is gathered and scanned.  (One LDS buffer: 22 KB
+// per workgroup for 16 thresholds, which leaves registers, not LDS, to set
+// the number of waves per SIMD.)
+constexpr int TREC = 9;              // LDS words per tile lane (8 + 1 pad)
+constexpr int DTILES = WAVES + 2;    // tiles of one block of a strip group
+constexpr int DRECS = DTILES * WAVE; // their records (tile lanes)
 
 template <int TG>
 __global__ __launch_bounds__(WG) void k_det_scan(const uint8_t *__restrict__ tiles, int64_t nbands, int64_t nc, int nt,
                                                  int seg_len, int nseg, const int2 *__restrict__ items,
                                                  int64_t nitems, unsigned long long *__restrict__ hist,
                                                  uint2 *__restrict__ carry, uint32_t *__restrict__ priv_base) {
+    static_assert(DRECS <= 2 * WG, "each thread moves at most two records per block");
     __shared__ uint32_t lh[TG * LCAP];
     __shared__ uint32_t dummy_bins[WAVE];
-    __shared__ uint32_t tl[WAVES][3 * WAVE * TREC];
+    __shared__ uint32_t tl[DRECS * TREC];
     for (int e = threadIdx.x; e < TG * LCAP; e += WG) lh[e] = 0;
     priv_zero<TG>(priv_slice<TG>(priv_base), nt, WG);
     __syncthreads();
     uint32_t *priv = priv_slice<TG>(priv_base);
     const Sink K{lh, dummy_bins + (threadIdx.x & (WAVE - 1)), priv, hist, nc};
 
-    const int lane = threadIdx.x & (WAVE - 1);
+    const int tid = threadIdx.x;
+    const int lane = tid & (WAVE - 1);
     const int wave = wave_id();
-    uint32_t *tw = tl[wave];
-    const uint8_t *tb = (const uint8_t *)tw;
     const size_t cstride = (size_t)(nc - 1) * nseg;
+    // records this thread moves: tid and tid + WG (tile rec/64, tile lane rec%64)
+    const int rec1 = tid + WG;
+    const bool has1 = rec1 < DRECS;
 
-    for (int64_t item = (int64_t)blockIdx.x * WAVES + wave; item < nitems; item += (int64_t)gridDim.x * WAVES) {
+    for (int64_t item = blockIdx.x; item < nitems; item += gridDim.x) {
         const int2 it = items[item];
-        const int64_t r0 = 1 + (int64_t)it.x * WAVE;
+        const int64_t rg = 1 + (int64_t)it.x * WG;
         const int q = it.y;
+        const int64_t r0 = rg + (int64_t)wave * WAVE;
         const int64_t r = r0 + lane;
         const bool rv = r < nc;
         const int64_t dlen = rv ? nc - r : 0;
         const int64_t sb = (int64_t)q * seg_len;
-        const int64_t se_nom = sb + seg_len;
-        const int64_t smax = nc - r0;
-        const int64_t se = se_nom < smax ? se_nom : smax;
+        const int64_t se_g = min(sb + (int64_t)seg_len, nc - rg);   // strip 0 is the group's longest
+        const int64_t se = min(sb + (int64_t)seg_len, nc - r0);     // this strip's part (none if <= sb)
+        const bool wact = sb < se;
         uint2 *cslot = carry + (size_t)(rv ? r - 1 : 0) * nseg + q;
 
         uint32_t run[TG];
 #pragma unroll
         for (int t = 0; t < TG; t++) run[t] = rv ? PRE : 0u;
 
-        // Tile records travel from HBM one block ahead: pf holds block s0 +
-        // BLK's records while block s0 is gathered and scanned.
-        uint4 pf[6];
+        uint4 pf[4];
         auto fetch = [&](int64_t s) {
-            const int64_t I = s / BLK, J0 = (s + r0) / WAVE;
-#pragma unroll
-            for (int t = 0; t < 3; t++) {
-                if (J0 + t < nbands) {
-                    const uint4 *src = (const uint4 *)(tiles + tile_index(I, J0 + t, nbands) * TILE_BYTES) + 2 * lane;
-                    pf[2 * t] = src[0];
-                    pf[2 * t + 1] = src[1];
-                }
+            const int64_t I = s / BLK, J0 = (s + rg) / WAVE;
+            if (J0 + tid / WAVE < nbands) {
+                const uint4 *src = (const uint4 *)(tiles + tile_index(I, J0 + tid / WAVE, nbands) * TILE_BYTES) +
+                                   2 * lane;
+                pf[0] = src[0];
+                pf[1] = src[1];
+            }
+            if (has1 && J0 + rec1 / WAVE < nbands) {
+                const uint4 *src = (const uint4 *)(tiles + tile_index(I, J0 + rec1 / WAVE, nbands) * TILE_BYTES) +
+                                   2 * lane;
+                pf[2] = src[0];
+                pf[3] = src[1];
             }
         };
-        if (sb < se) fetch(sb);
-        for (int64_t s0 = sb; s0 < se; s0 += BLK) {
-            const int64_t c0 = s0 + r0, J0 = c0 / WAVE;
-            wave_sync();   // the previous block's gathers are done
-#pragma unroll
-            for (int t = 0; t < 3; t++) {
-                if (J0 + t < nbands) {
-                    const uint4 a = pf[2 * t], b = pf[2 * t + 1];
-                    uint32_t *d = tw + (t * WAVE + lane) * TREC;
-                    d[0] = a.x; d[1] = a.y; d[2] = a.z; d[3] = a.w;
-                    d[4] = b.x; d[5] = b.y; d[6] = b.z; d[7] = b.w;
-                }
+        auto put = [&](int64_t s, uint32_t *buf) {
+            const int64_t J0 = (s + rg) / WAVE;
+            if (J0 + tid / WAVE < nbands) {
+                uint32_t *d = buf + tid * TREC;
+                d[0] = pf[0].x; d[1] = pf[0].y; d[2] = pf[0].z; d[3] = pf[0].w;
+                d[4] = pf[1].x; d[5] = pf[1].y; d[6] = pf[1].z; d[7] = pf[1].w;
             }
-            if (s0 + BLK < se) fetch(s0 + BLK);
-            wave_sync();
-            // step u of lane l: column c0 + l + u, i.e. record (c0 - 64 J0) + l + u
-            const int base = (int)(c0 - J0 * WAVE) + lane;
-            uint32_t R[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
+            if (has1 && J0 + rec1 / WAVE < nbands) {
+                uint32_t *d = buf + rec1 * TREC;
+                d[0] = pf[2].x; d[1] = pf[2].y; d[2] = pf[2].z; d[3] = pf[2].w;
+                d[4] = pf[3].x; d[5] = pf[3].y; d[6] = pf[3].z; d[7] = pf[3].w;
+            }
+        };
+        if (sb < se_g) fetch(sb);
+        for (int64_t s0 = sb; s0 < se_g; s0 += BLK) {
+            __syncthreads();   // the previous block's gathers are done
+            put(s0, tl);
+            if (s0 + BLK < se_g) fetch(s0 + BLK);
+            __syncthreads();
+            if (s0 < se) {
+                // step u of lane l: column s0 + r0 + l + u, i.e. record
+                // (s0 + rg - 64 J0) + 64 w + l + u of the group's block
+                const int64_t c0g = s0 + rg;
+                const int base = (int)(c0g - (c0g / WAVE) * WAVE) + wave * WAVE + lane;
+                const uint8_t *tb = (const uint8_t *)tl;
+                uint32_t R[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
 #pragma unroll
-            for (int u = 0; u < BLK; u++)
-                R[u & 7] |= (uint32_t)tb[(base + u) * (4 * TREC) + 4 * (u & 7) + (u >> 3)] << (8 * (u >> 3));
-            const int64_t nv64 = dlen - s0;
-            const int nvalid = nv64 <= 0 ? 0 : (nv64 >= BLK ? BLK : (int)nv64);
-            const uint32_t vmask = nvalid >= BLK ? 0xFFFFFFFFu : ((1u << nvalid) - 1u);
-            scan_words<TG>(R, 0xFFFFFFFFu, ~vmask, run, nt, K, cslot, cstride);
+                for (int u = 0; u < BLK; u++)
+                    R[u & 7] |= (uint32_t)tb[(base + u) * (4 * TREC) + 4 * (u & 7) + (u >> 3)] << (8 * (u >> 3));
+                const int64_t nv64 = dlen - s0;
+                const int nvalid = nv64 <= 0 ? 0 : (nv64 >= BLK ? BLK : (int)nv64);
+                const uint32_t vmask = nvalid >= BLK ? 0xFFFFFFFFu : ((1u << nvalid) - 1u);
+                scan_words<TG>(R, 0xFFFFFFFFu, ~vmask, run, nt, K, cslot, cstride);
+            }
         }
-        if (rv) close_segment<TG>(run, nt, cslot, cstride);
+        if (rv && wact) close_segment<TG>(run, nt, cslot, cstride);
     }
 
     __syncthreads();

@@ -50,15 +50,15 @@ struct Ctx {
     int device = -1;
     int ncu = 0;
     hipStream_t stream = nullptr;
-    DevBuf x, hist, carry, metric, scan, items_rr, items_det, items_lamv, items_lamh, counts, perm, lmins, lut, nrm, nmax, lut8, out_rr,
-        out_det, out_lam, priv, tiles, dump;
+    DevBuf x, hist, carry, metric, scan, items_rr, items_det, items_dets, items_lamv, items_lamh, counts, perm, lmins,
+        lut, nrm, nmax, lut8, out_rr, out_det, out_lam, priv, tiles, dump;
     // Raw-result sinks of accrqa_amd_histograms_*: when set, launch_group
     // copies each group's counts / h_D rows / h_V rows there, indexed by the
     // caller's threshold order.  Null for every other entry point.
     unsigned long long *dump_rr = nullptr, *dump_det = nullptr, *dump_lam = nullptr;
     int64_t items_nc = -1;     // (nc, seg_len) the item tables were built for
     int items_L = -1;
-    int64_t n_items_rr = 0, n_items_det = 0, n_items_lamv = 0, n_items_lamh = 0;
+    int64_t n_items_rr = 0, n_items_det = 0, n_items_dets = 0, n_items_lamv = 0, n_items_lamh = 0;
 };
 
 std::mutex g_mu;
@@ -128,7 +128,7 @@ int init_ctx() {
                              &g_ctx.items_lamh, &g_ctx.counts,  &g_ctx.perm,       &g_ctx.lmins,
                              &g_ctx.lut,     &g_ctx.nrm,        &g_ctx.nmax,       &g_ctx.lut8,
                              &g_ctx.out_rr,  &g_ctx.out_det,    &g_ctx.out_lam,    &g_ctx.priv,
-                             &g_ctx.tiles,   &g_ctx.dump};
+                             &g_ctx.tiles,   &g_ctx.dump,       &g_ctx.items_dets};
             g_ctx.items_nc = -1;
             for (DevBuf *b : all) {
                 if (b->p) (void)hipFree(b->p);
@@ -373,6 +373,12 @@ unsigned grid_for(Kern kern, int64_t nitems, int ncu) {
     return (unsigned)std::max<int64_t>(1, std::min<int64_t>(wgs, (int64_t)nb * ncu));
 }
 
+// Persistent grid for a kernel whose work items belong to whole workgroups.
+template <typename Kern>
+unsigned grid_for_wg(Kern kern, int64_t nitems, int ncu) {
+    return grid_for(kern, nitems * rqa::WAVES, ncu);
+}
+
 // Workgroup-private long-run bins for a launch of `grid` workgroups.
 template <int TG>
 int ensure_priv(unsigned grid) {
@@ -411,7 +417,7 @@ int build_items(int64_t nc, hipStream_t st) {
     if (c.items_nc == nc && c.items_L == L) return E_OK;
     // (item, steps) lists; each table is sorted by decreasing size so that the
     // persistent waves take the long items first
-    std::vector<std::pair<int64_t, int2>> rr, det, lamv, lamh;
+    std::vector<std::pair<int64_t, int2>> rr, det, dets, lamv, lamh;
     const int64_t nb = (nc + rqa::WAVE - 1) / rqa::WAVE;
     auto seg = [&](std::vector<std::pair<int64_t, int2>> &v, int64_t b, int64_t len) {
         for (int64_t q = 0; q * L < len; q++)
@@ -425,6 +431,9 @@ int build_items(int64_t nc, hipStream_t st) {
     }
     const int64_t nstrips = (nc - 1 + rqa::WAVE - 1) / rqa::WAVE;
     for (int64_t b = 0; b < nstrips; b++) seg(det, b, nc - (1 + b * rqa::WAVE));
+    // k_det_scan: groups of WAVES strips, segments of the group's longest strip
+    const int64_t ngroups = (nc - 1 + rqa::WG - 1) / rqa::WG;
+    for (int64_t g = 0; g < ngroups; g++) seg(dets, g, nc - (1 + g * rqa::WG));
     int e;
     auto upload = [&](std::vector<std::pair<int64_t, int2>> &v, DevBuf &buf, int64_t &n) -> int {
         std::stable_sort(v.begin(), v.end(), [](const auto &x, const auto &y) { return x.first > y.first; });
@@ -439,6 +448,7 @@ int build_items(int64_t nc, hipStream_t st) {
     HIP_TRY(hipStreamSynchronize(st));
     if ((e = upload(rr, c.items_rr, c.n_items_rr))) return e;
     if ((e = upload(det, c.items_det, c.n_items_det))) return e;
+    if ((e = upload(dets, c.items_dets, c.n_items_dets))) return e;
     if ((e = upload(lamv, c.items_lamv, c.n_items_lamv))) return e;
     if ((e = upload(lamh, c.items_lamh, c.n_items_lamh))) return e;
     c.items_nc = nc;
@@ -584,12 +594,13 @@ int launch_group(const F *dx, int64_t nc, int tau, int m, const F *bounds, int n
             const int64_t ni = c.n_items_det;
             bool done = false;
             if (tiles) {
-                const unsigned g = grid_for(rqa::k_det_scan<TG>, ni, c.ncu);
+                const int64_t nig = c.n_items_dets;
+                const unsigned g = grid_for_wg(rqa::k_det_scan<TG>, nig, c.ncu);
                 if ((e = ensure_priv<TG>(g))) return e;
                 prof_begin("k_det_scan", st);
                 hipLaunchKernelGGL((rqa::k_det_scan<TG>), dim3(g), dim3(rqa::WG), 0, st, (const uint8_t *)tiles,
                                    nbands, nc, nt, L, nseg,
-                                   (const int2 *)c.items_det.p, ni, hist, carry, (uint32_t *)c.priv.p);
+                                   (const int2 *)c.items_dets.p, nig, hist, carry, (uint32_t *)c.priv.p);
                 prof_end(st);
                 HIP_TRY(hipGetLastError());
                 done = true;
