@@ -47,10 +47,11 @@ constexpr int WAVES = WG / WAVE;
 #ifndef RQA_WPE_LAM
 #define RQA_WPE_LAM
 #endif
-// k_lam_mx: three waves per SIMD (168 VGPRs for 16 thresholds, no spills);
-// left to itself the compiler takes a few registers more and halves that.
+// k_lam_mx: three waves per SIMD with the line scan (168 VGPRs for 16
+// thresholds, no spills; left to itself the compiler takes a few registers
+// more and halves that), four without it (the LDS allows four workgroups).
 #ifndef RQA_WPE_MX
-#define RQA_WPE_MX __attribute__((amdgpu_waves_per_eu(3)))
+#define RQA_WPE_MX(scan) __attribute__((amdgpu_waves_per_eu((scan) ? 3 : 4)))
 #endif
 constexpr int DET_WIN = 992;       // LDS samples per wave for the k_det lane window
 
@@ -1061,23 +1062,25 @@ __device__ __forceinline__ void tile_row_norms(const float *__restrict__ nrm, in
 // 8(e>>2) + 4h + (e&3) of column l%32 of its tile, i.e. byte e>>2 of rank
 // register (e&3) + 4h of that column.  One cross-half swap per register then
 // leaves lane l (half 0) with column j0 + l and lane l (half 1) with column
-// j0 + l, all 32 rows, and the scan is k_lam's.
-template <int TG>
-__global__ __launch_bounds__(WG) RQA_WPE_MX void k_lam_mx(const float *__restrict__ x, const float *__restrict__ nrm, int64_t nc,
+// j0 + l, all 32 rows, and the scan is k_lam's.  SCAN = false (split form):
+// ranks and rank tiles only, the column runs left to k_lam_vscan.
+template <int TG, bool SCAN>
+__global__ __launch_bounds__(WG) RQA_WPE_MX(SCAN) void k_lam_mx(const float *__restrict__ x, const float *__restrict__ nrm, int64_t nc,
                                                int tau, int m, Bounds<float, TG> S, int nt,
                                                const uint8_t *__restrict__ lut, MxParams P, int seg_len, LamWork W,
                                                unsigned long long *__restrict__ hist, uint2 *__restrict__ carry,
                                                uint32_t *__restrict__ priv_base, uint8_t *__restrict__ tiles) {
-    __shared__ uint32_t lh[TG * LCAP];
+    __shared__ uint32_t lh[SCAN ? TG * LCAP : 1];
     __shared__ uint32_t dummy_bins[WAVE];
     __shared__ uint8_t lut_s[NBX_LAM];
     __shared__ float bnd_s[NBND];
     // rw: x[i0 .. i0 + 31 + (m-1)tau] of the current block; cw: x[j0 .. j0 + 63 + (m-1)tau] of the item
     __shared__ MxWin<2> win[WAVES];
-    for (int e = threadIdx.x; e < TG * LCAP; e += WG) lh[e] = 0;
+    if constexpr (SCAN)
+        for (int e = threadIdx.x; e < TG * LCAP; e += WG) lh[e] = 0;
     for (int e = threadIdx.x; e < P.nb / 4; e += WG) ((uint32_t *)lut_s)[e] = ((const uint32_t *)lut)[e];
     if (threadIdx.x < NBND) bnd_s[threadIdx.x] = (int)threadIdx.x < nt ? S.s[threadIdx.x] : __int_as_float(0x7F800000);
-    priv_zero<TG>(priv_slice<TG>(priv_base), nt, WG);
+    if constexpr (SCAN) priv_zero<TG>(priv_slice<TG>(priv_base), nt, WG);
     __syncthreads();
     uint32_t *priv = priv_slice<TG>(priv_base);
     const Sink K{lh, dummy_bins + (threadIdx.x & (WAVE - 1)), priv, hist, nc};
@@ -1181,6 +1184,77 @@ __global__ __launch_bounds__(WG) RQA_WPE_MX void k_lam_mx(const float *__restric
                 R[4 + c] = sw[1];
             }
             if (tiles) store_tile(tiles, i0 / BLK, band, nbands, R);
+            if constexpr (SCAN) {
+                const uint32_t fill = nb == BLK ? 0u : (0xFFFFFFFFu << nb);
+                scan_words<TG>(R, jv ? ~fill : 0u, jv ? 0u : 0xFFFFFFFFu, run, nt, K, cslot, cstride);
+                if (fill && jv) {
+#pragma unroll
+                    for (int t = 0; t < TG; t++) run[t] -= (uint32_t)(BLK - nb);
+                }
+            }
+        }
+        if constexpr (SCAN)
+            if (jv) close_segment<TG>(run, nt, cslot, cstride);
+    }
+
+    if constexpr (SCAN) {
+        __syncthreads();
+        flush_hist<TG>(lh, priv, nt, hist, nc, WG);
+    }
+}
+
+// ---------------------------------------------------------------- k_lam_vscan
+// The column part of LAM in the split form, read back from the rank tiles
+// (the MFMA column kernel then only ranks and stores).  Work item = (band J,
+// segment q of the rows [0, V_J)), as that kernel's; lane l walks column
+// 64J + l in blocks of 32 rows, each block's ranks being the lane's own
+// 32-byte record of tile (I, J), in the packed layout the scan takes: no
+// exchange between lanes.  The next block's record is in flight while a
+// block is scanned.
+template <int TG>
+__global__ __launch_bounds__(WG) void k_lam_vscan(const uint8_t *__restrict__ tiles, int64_t nbands, int64_t nc,
+                                                  int nt, int seg_len, LamWork W,
+                                                  unsigned long long *__restrict__ hist, uint2 *__restrict__ carry,
+                                                  uint32_t *__restrict__ priv_base) {
+    __shared__ uint32_t lh[TG * LCAP];
+    __shared__ uint32_t dummy_bins[WAVE];
+    for (int e = threadIdx.x; e < TG * LCAP; e += WG) lh[e] = 0;
+    priv_zero<TG>(priv_slice<TG>(priv_base), nt, WG);
+    __syncthreads();
+    uint32_t *priv = priv_slice<TG>(priv_base);
+    const Sink K{lh, dummy_bins + (threadIdx.x & (WAVE - 1)), priv, hist, nc};
+
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wave = wave_id();
+    const size_t cstride = (size_t)nc * W.cseg;
+
+    for (int64_t item = (int64_t)blockIdx.x * WAVES + wave; item < W.nitems; item += (int64_t)gridDim.x * WAVES) {
+        const int2 it = W.items[item];
+        const int64_t band = it.x;
+        const int q = it.y;
+        const int64_t vend = min((band + 1) * WAVE, nc);
+        const int64_t jown = band * WAVE + lane;
+        const bool jv = jown < nc;
+        const int64_t ib = (int64_t)q * seg_len;
+        const int64_t ie = ib + seg_len < vend ? ib + seg_len : vend;
+        uint2 *cslot = carry + (size_t)(jv ? jown : 0) * W.cseg + q;
+
+        uint32_t run[TG];
+#pragma unroll
+        for (int t = 0; t < TG; t++) run[t] = jv ? PRE : 0u;
+
+        uint4 v0 = make_uint4(0u, 0u, 0u, 0u), v1 = v0;
+        auto fetch = [&](int64_t i) {
+            const uint4 *src = (const uint4 *)(tiles + tile_index(i / BLK, band, nbands) * TILE_BYTES) + 2 * lane;
+            v0 = src[0];
+            v1 = src[1];
+        };
+        if (ib < ie) fetch(ib);
+        for (int64_t i0 = ib; i0 < ie; i0 += BLK) {
+            const int nb = ie - i0 < BLK ? (int)(ie - i0) : BLK;
+            const uint32_t R[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+            if (i0 + BLK < ie) fetch(i0 + BLK);
+            // rows past the part count as recurrent padding, taken back below
             const uint32_t fill = nb == BLK ? 0u : (0xFFFFFFFFu << nb);
             scan_words<TG>(R, jv ? ~fill : 0u, jv ? 0u : 0xFFFFFFFFu, run, nt, K, cslot, cstride);
             if (fill && jv) {

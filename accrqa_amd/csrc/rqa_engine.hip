@@ -6,8 +6,9 @@
 // (one pass over the pairs per group), and each group runs on one HIP stream
 //   RR alone:   k_rr -> k_rr_out
 //   LAM:        k_lam | k_lam_mx (column pass; keeps the rank tiles when they
-//               fit) [-> k_lam_hscan] -> k_stitch -> k_finalize, RR from its
-//               histogram -> k_rr_out
+//               fit, and then leaves the column runs to k_lam_vscan in the
+//               MFMA form) [-> k_lam_hscan] -> k_stitch -> k_finalize, RR from
+//               its histogram -> k_rr_out
 //   DET:        k_det_scan (from the tiles) | k_det | k_det_mx -> k_stitch
 //               -> k_finalize
 // (k_lam_mx / k_det_mx: MFMA Gram form, with k_norms and k_mx_lut first).  Device buffers live in a per-process context that grows
@@ -531,14 +532,33 @@ int launch_group(const F *dx, int64_t nc, int tau, int m, const F *bounds, int n
                 hipLaunchKernelGGL((rqa::k_mx_lut<TG>), dim3((MP.nb + 255) / 256), dim3(256), 0, st, Bf, nt, MP,
                                    (const unsigned int *)c.nmax.p, (uint8_t *)c.lut8.p);
                 HIP_TRY(hipGetLastError());
-                const unsigned g = grid_for(rqa::k_lam_mx<TG>, LW.nitems, c.ncu);
-                if ((e = ensure_priv<TG>(g))) return e;
-                prof_begin("k_lam_mx", st);
-                hipLaunchKernelGGL((rqa::k_lam_mx<TG>), dim3(g), dim3(rqa::WG), 0, st, (const float *)dx,
-                                   (const float *)c.nrm.p, nc, tau, m, Bf, nt, (const uint8_t *)c.lut8.p, MP, L, LW,
-                                   hist, carry, (uint32_t *)c.priv.p, tiles);
-                prof_end(st);
-                HIP_TRY(hipGetLastError());
+                if (tiles) {
+                    // split form: ranks and tiles, then the column runs from the tiles
+                    const unsigned g = grid_for(rqa::k_lam_mx<TG, false>, LW.nitems, c.ncu);
+                    prof_begin("k_lam_mx", st);
+                    hipLaunchKernelGGL((rqa::k_lam_mx<TG, false>), dim3(g), dim3(rqa::WG), 0, st, (const float *)dx,
+                                       (const float *)c.nrm.p, nc, tau, m, Bf, nt, (const uint8_t *)c.lut8.p, MP, L,
+                                       LW, hist, carry, (uint32_t *)c.priv.p, tiles);
+                    prof_end(st);
+                    HIP_TRY(hipGetLastError());
+                    const unsigned gv = grid_for(rqa::k_lam_vscan<TG>, LW.nitems, c.ncu);
+                    if ((e = ensure_priv<TG>(gv))) return e;
+                    prof_begin("k_lam_vscan", st);
+                    hipLaunchKernelGGL((rqa::k_lam_vscan<TG>), dim3(gv), dim3(rqa::WG), 0, st,
+                                       (const uint8_t *)tiles, nbands, nc, nt, L, LW, hist, carry,
+                                       (uint32_t *)c.priv.p);
+                    prof_end(st);
+                    HIP_TRY(hipGetLastError());
+                } else {
+                    const unsigned g = grid_for(rqa::k_lam_mx<TG, true>, LW.nitems, c.ncu);
+                    if ((e = ensure_priv<TG>(g))) return e;
+                    prof_begin("k_lam_mx", st);
+                    hipLaunchKernelGGL((rqa::k_lam_mx<TG, true>), dim3(g), dim3(rqa::WG), 0, st, (const float *)dx,
+                                       (const float *)c.nrm.p, nc, tau, m, Bf, nt, (const uint8_t *)c.lut8.p, MP, L,
+                                       LW, hist, carry, (uint32_t *)c.priv.p, tiles);
+                    prof_end(st);
+                    HIP_TRY(hipGetLastError());
+                }
             } else {
                 const unsigned g = grid_for(rqa::k_lam<F, NORM, TG>, LW.nitems, c.ncu);
                 if ((e = ensure_priv<TG>(g))) return e;

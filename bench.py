@@ -71,9 +71,10 @@ def tile_store_bytes(nc):
 # form and scaled by the same counters' readings for a 1 GiB device copy
 # (FETCH_SIZE reads 0.5 GiB for it: x2.0; WRITE_SIZE 1 GiB: x1.0).
 MEASURED_TRAFFIC = {
-    ("ar1_m16", "k_lam_mx"): 2.0 * 1.101e9 + 41.06e9,
-    ("ar1_m16", "k_det_scan"): 2.0 * 49.05e9 + 5.43e9,
-    ("ar1_m16", "k_lam_hscan"): 2.0 * 17.29e9 + 6.84e9,
+    ("ar1_m16", "k_lam_mx"): 2.0 * 1.0062e9 + 34.369e9,
+    ("ar1_m16", "k_lam_vscan"): 2.0 * 17.282e9 + 6.805e9,
+    ("ar1_m16", "k_lam_hscan"): 2.0 * 17.272e9 + 6.825e9,
+    ("ar1_m16", "k_det_scan"): 2.0 * 25.247e9 + 5.675e9,
 }
 
 
@@ -84,8 +85,9 @@ def kernel_rooflines(kern, nc, emb, dist, workload):
                     ranks (upper triangle and diagonal), 2m FLOP each, on MFMA;
       k_lam         the same pairs' exact-order distances, 3m (Euclidean) or 2m
                     (maximal) VALU FLOP each;
-      k_lam_hscan,  one read of the rank-tile store (one byte per pair of the
-      k_det_scan    upper triangle), from HBM."""
+      k_lam_vscan,  one read of the rank-tile store (one byte per pair of the
+      k_lam_hscan,  upper triangle), from HBM.
+      k_det_scan"""
     pairs = nc * (nc + 1) / 2.0
     out = {}
     for k, v in kern.items():
@@ -101,7 +103,7 @@ def kernel_rooflines(kern, nc, emb, dist, workload):
             out[k] = {"bound": "valu", "achieved": w / t / 1e12, "peak": VALU_PEAK_TOPS, "unit": "TFLOP/s",
                       "work_per_launch": w, "work_unit": "FLOP (exact-order sum per distinct pair)",
                       "peak_note": "FP32 VALU non-FMA issue rate (sub/mul/add in order, no FMA)"}
-        elif k in ("k_lam_hscan", "k_det_scan"):
+        elif k in ("k_lam_vscan", "k_lam_hscan", "k_det_scan"):
             w = float(tile_store_bytes(nc))
             out[k] = {"bound": "hbm", "achieved": w / t / 1e9, "peak": 8000.0, "unit": "GB/s",
                       "work_per_launch": w, "work_unit": "bytes (one read of the rank-tile store)"}
@@ -227,7 +229,8 @@ def main():
         L.accrqa_amd_profile(0)
         ms = max(ev0.elapsed_time(ev1), wall * 1e3) / args.steps
         kern = {}
-        for k in ("k_lam", "k_lam_mx", "k_lam_hscan", "k_det", "k_det_mx", "k_det_scan", "k_rr", "k_stitch", "k_finalize"):
+        for k in ("k_lam", "k_lam_mx", "k_lam_vscan", "k_lam_hscan", "k_det", "k_det_mx", "k_det_scan", "k_rr",
+                  "k_stitch", "k_finalize"):
             t, n = C.c_double(0), C.c_longlong(0)
             L.accrqa_amd_profile_read(k.encode(), C.byref(t), C.byref(n))
             if n.value:

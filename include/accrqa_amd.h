@@ -132,8 +132,8 @@ int accrqa_amd_set_split(int mode);
 /* Kernel timing (tracing aid, no counterpart in the reference): when enabled,
  * every kernel launch is bracketed by HIP events on its own stream and the
  * elapsed times are summed per kernel name ("k_lam", "k_lam_mx",
- * "k_lam_hscan", "k_det", "k_det_mx", "k_det_scan", "k_rr", "k_stitch",
- * "k_finalize").  Adds a stream synchronisation per call. */
+ * "k_lam_vscan", "k_lam_hscan", "k_det", "k_det_mx", "k_det_scan", "k_rr",
+ * "k_stitch", "k_finalize").  Adds a stream synchronisation per call. */
 void accrqa_amd_profile(int enable);
 void accrqa_amd_profile_reset(void);
 /* Returns 0 and fills the totals if the kernel was seen, else 1 and zeros. */
