@@ -47,8 +47,8 @@ constexpr int WAVES = WG / WAVE;
 #ifndef RQA_WPE_LAM
 #define RQA_WPE_LAM
 #endif
-// k_lam_mx: three waves per SIMD with the line scan (168 VGPRs for 16
-// thresholds, no spills; left to itself the compiler takes a few registers
+// k_lam_mx, k_det_mx: three waves per SIMD with the line scan (168 VGPRs for
+// 16 thresholds, no spills; left to itself the compiler takes a few registers
 // more and halves that), four without it (the LDS allows four workgroups).
 #ifndef RQA_WPE_MX
 #define RQA_WPE_MX(scan) __attribute__((amdgpu_waves_per_eu((scan) ? 3 : 4)))
@@ -812,9 +812,10 @@ constexpr int MXW = 128;     // LDS row window: 32 + (m-1)*tau samples at most
 constexpr int NBND = 64;     // bound slots reachable by a 6-bit table index
 
 struct MxParams {
-    uint32_t base;   // key bits of bucket 0 (half the smallest bound)
-    int sh;          // bucket width in key bits
+    uint32_t base;   // key bits of bucket 0 (half the smallest bound, rounded down to a bucket edge)
+    int sh;          // bucket width in key bits (log2)
     int nb;          // buckets used (<= NBX)
+    int qlo;         // base >> sh: bucket 0's number on the key >> sh scale
     float cE;        // (4m + 16) * 2^-24
 };
 
@@ -877,13 +878,16 @@ __global__ void k_mx_lut(Bounds<float, TG> S, int nt, MxParams P, const unsigned
     lut[b] = (uint8_t)e;
 }
 
-// Bucket of s~.  A slightly negative s~ (rounding near 0) has a negative bit
-// pattern as an integer and lands in bucket 0, like +0; keys below the table
-// saturate to bucket 0 and keys above it to the last bucket.
+// Bucket of s~ (bucket edges are multiples of 2^sh in key bits): key >> sh,
+// clamped to the table by one median-of-three.  A slightly negative s~
+// (rounding near 0) has a negative bit pattern as an integer and lands in
+// bucket 0, like +0; keys below the table saturate to bucket 0 and keys above
+// it to the last bucket.
 __device__ __forceinline__ uint32_t mx_key_bucket(float st, const MxParams &P) {
-    const uint32_t key = (uint32_t)max(__float_as_int(st), 0);
-    const uint32_t d = __builtin_elementwise_sub_sat(key, P.base);
-    return min(d >> P.sh, (uint32_t)(P.nb - 1));
+    const int q = __float_as_int(st) >> P.sh;
+    int qc;
+    asm("v_med3_i32 %0, %1, %2, %3" : "=v"(qc) : "v"(q), "v"(P.qlo), "v"(P.qlo + P.nb - 1));
+    return (uint32_t)(qc - P.qlo);
 }
 
 // Second stage of one flagged element (table byte with bit 7 set): the one
@@ -1279,7 +1283,7 @@ __global__ __launch_bounds__(WG) void k_lam_vscan(const uint8_t *__restrict__ ti
 constexpr int SKB = 36;   // bytes per diagonal row (9 words: conflict-free stores and loads)
 
 template <int TG>
-__global__ __launch_bounds__(WG) void k_det_mx(const float *__restrict__ x, const float *__restrict__ nrm, int64_t nc,
+__global__ __launch_bounds__(WG) RQA_WPE_MX(true) void k_det_mx(const float *__restrict__ x, const float *__restrict__ nrm, int64_t nc,
                                                int tau, int m, Bounds<float, TG> S, int nt,
                                                const uint8_t *__restrict__ lut, MxParams P, int seg_len, int nseg,
                                                const int2 *__restrict__ items, int64_t nitems,

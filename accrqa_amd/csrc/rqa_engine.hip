@@ -466,11 +466,14 @@ rqa::MxParams mx_params(const float *b, int nt, int m, int nbmax) {
     const uint32_t base = hkey(lo), top = hkey(hi);
     int sh = 0;
     while (((top - base) >> sh) > (uint32_t)(nbmax - 8)) sh++;
-    int nb = (int)((top - base) >> sh) + 2;
+    // bucket edges at multiples of 2^sh: the device finds a bucket as key >> sh
+    const uint32_t base_al = base & ~((1u << sh) - 1u);
+    int nb = (int)((top - base_al) >> sh) + 2;
     nb = (nb + 3) & ~3;
-    P.base = base;
+    P.base = base_al;
     P.sh = sh;
     P.nb = nb;
+    P.qlo = (int)(base_al >> sh);
     P.cE = (float)((4.0 * m + 16.0) * std::ldexp(1.0, -24));
     return P;
 }
