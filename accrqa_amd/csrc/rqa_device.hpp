@@ -95,6 +95,21 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// OR of x over the wave's 64 lanes, returned as a wave-uniform value:
+// prefix ORs within each row of 16 lanes (row_shr 1, 2, 4, 8), then across
+// rows (row_bcast 15 into rows 1 and 3, row_bcast 31 into rows 2 and 3), so
+// lane 63 ends with the OR of all lanes.  Lanes a shift leaves without a
+// source keep 0.
+__device__ __forceinline__ uint32_t wave_or(uint32_t x) {
+    x |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xF, 0xF, false);
+    x |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xF, 0xF, false);
+    x |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xF, 0xF, false);
+    x |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xF, 0xF, false);
+    x |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xA, 0xF, false);
+    x |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xC, 0xF, false);
+    return (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
+}
+
 __device__ __forceinline__ int wave_id() {
     return __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
 }
@@ -1028,11 +1043,17 @@ __device__ __forceinline__ void tile_ranks(const floatx16 &acc, const float (&ni
         Rq[e & 3] |= (uint32_t)lut_s[mx_key_bucket(st, P)] << (8 * (e >> 2));
     }
     fm = 0u;
-    if (__any(((Rq[0] | Rq[1] | Rq[2] | Rq[3]) & 0x80808080u) != 0u)) {
+    // this lane's flagged elements: bit 8g + c <=> element 4g + c (table byte bit 7)
+    const uint32_t fl = ((Rq[0] >> 7) & 0x01010101u) | ((Rq[1] >> 6) & 0x02020202u) |
+                        ((Rq[2] >> 5) & 0x04040404u) | ((Rq[3] >> 4) & 0x08080808u);
+    if (__builtin_amdgcn_ballot_w64(fl != 0u) != 0ull) {
+        // elements flagged in some lane, as one scalar word: the test per
+        // element below is then a scalar bit test, not vector compares
+        const uint32_t W = wave_or(fl);
 #pragma unroll
         for (int e = 0; e < 16; e++) {
             const int c = e & 3, sh = 8 * (e >> 2);
-            if (__any(((Rq[c] >> (sh + 7)) & 1u) != 0u)) {
+            if (W & (1u << (sh + c))) {
                 const uint32_t byte = (Rq[c] >> sh) & 0xFFu;
                 const float sum = ni[e] + nj;
                 const float st = fmaxf(fmaf(-2.0f, acc[e], sum), 0.0f);
