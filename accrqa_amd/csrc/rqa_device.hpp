@@ -374,7 +374,7 @@ __global__ __launch_bounds__(WG) RQA_WPE_LAM void k_lam(const F *__restrict__ x,
     const int wave = wave_id();
     const int64_t nbands = (nc + WAVE - 1) / WAVE;
     const int64_t nitems = W.items ? W.nitems : nbands * W.nseg;
-    const size_t cstride = (size_t)nc * W.cseg;   // carry stride between thresholds
+    const size_t cstride = (size_t)W.cseg * nc;   // carry stride between thresholds
     constexpr int MJ = sizeof(F) == 4 ? 16 : 8;  // column samples kept in registers
 
     for (int64_t item = (int64_t)blockIdx.x * WAVES + wave; item < nitems; item += (int64_t)gridDim.x * WAVES) {
@@ -393,7 +393,7 @@ __global__ __launch_bounds__(WG) RQA_WPE_LAM void k_lam(const F *__restrict__ x,
         const bool jv = j < nc;
         const int64_t ib = (int64_t)q * seg_len;
         const int64_t ie = ib + seg_len < vend ? ib + seg_len : vend;
-        uint2 *cslot = carry + (size_t)(jv ? j : 0) * W.cseg + q;
+        uint2 *cslot = carry + (size_t)q * nc + (jv ? j : 0);
 
         uint32_t run[TG];
 #pragma unroll
@@ -471,7 +471,7 @@ __global__ __launch_bounds__(WG) void k_det(const F *__restrict__ x, int64_t nc,
     // dimensions staged per window: span = BLK + WAVE - 1 + (kc-1)*tau <= DET_WIN
     const int kc_max = tau > 0 ? (DET_WIN - (BLK + WAVE - 1)) / tau + 1 : m;
     const int kc = kc_max < m ? kc_max : m;
-    const size_t cstride = (size_t)(nc - 1) * nseg;
+    const size_t cstride = (size_t)nseg * (nc - 1);
 
     for (int64_t item = (int64_t)blockIdx.x * WAVES + wave; item < nitems; item += (int64_t)gridDim.x * WAVES) {
         const int2 it = items[item];
@@ -484,7 +484,7 @@ __global__ __launch_bounds__(WG) void k_det(const F *__restrict__ x, int64_t nc,
         const int64_t se_nom = sb + seg_len;
         const int64_t smax = nc - r0;                // lane 0's diagonal length
         const int64_t se = se_nom < smax ? se_nom : smax;
-        uint2 *cslot = carry + (size_t)(rv ? r - 1 : 0) * nseg + q;
+        uint2 *cslot = carry + (size_t)q * (nc - 1) + (rv ? r - 1 : 0);
 
         uint32_t run[TG];
 #pragma unroll
@@ -560,7 +560,7 @@ __global__ __launch_bounds__(WG) void k_det_scan(const uint8_t *__restrict__ til
     const int tid = threadIdx.x;
     const int lane = tid & (WAVE - 1);
     const int wave = wave_id();
-    const size_t cstride = (size_t)(nc - 1) * nseg;
+    const size_t cstride = (size_t)nseg * (nc - 1);
     // records this thread moves: tid and tid + WG (tile rec/64, tile lane rec%64)
     const int rec1 = tid + WG;
     const bool has1 = rec1 < DRECS;
@@ -577,7 +577,7 @@ __global__ __launch_bounds__(WG) void k_det_scan(const uint8_t *__restrict__ til
         const int64_t se_g = min(sb + (int64_t)seg_len, nc - rg);   // strip 0 is the group's longest
         const int64_t se = min(sb + (int64_t)seg_len, nc - r0);     // this strip's part (none if <= sb)
         const bool wact = sb < se;
-        uint2 *cslot = carry + (size_t)(rv ? r - 1 : 0) * nseg + q;
+        uint2 *cslot = carry + (size_t)q * (nc - 1) + (rv ? r - 1 : 0);
 
         uint32_t run[TG];
 #pragma unroll
@@ -670,7 +670,7 @@ __global__ __launch_bounds__(WG) void k_lam_hscan(const uint8_t *__restrict__ ti
     const int a = lane >> 5, rr = lane & 31;
     // lane's byte in a record of its row block: register rr&7, byte rr>>3
     const int gofs = (a * 32) * (4 * TREC) + 4 * (rr & 7) + (rr >> 3);
-    const size_t cstride = (size_t)nc * 2 * nseg;
+    const size_t cstride = (size_t)2 * nseg * nc;
 
     for (int64_t item = (int64_t)blockIdx.x * WAVES + wave; item < nitems; item += (int64_t)gridDim.x * WAVES) {
         const int2 it = items[item];
@@ -682,7 +682,7 @@ __global__ __launch_bounds__(WG) void k_lam_hscan(const uint8_t *__restrict__ ti
         const int64_t cb = hb + (int64_t)q * seg_len;
         const int64_t ce = cb + seg_len < nc ? cb + seg_len : nc;
         const int64_t I = 2 * Jr + a;                    // row block of this lane's row
-        uint2 *cslot = carry + (size_t)(jv ? j : 0) * 2 * nseg + nseg + q;
+        uint2 *cslot = carry + (size_t)(nseg + q) * nc + (jv ? j : 0);
 
         uint32_t run[TG];
 #pragma unroll
@@ -1118,7 +1118,7 @@ __global__ __launch_bounds__(WG) RQA_WPE_MX(SCAN) void k_lam_mx(const float *__r
     const int cspan = 64 + (m - 1) * tau;   // <= MXW + 32
     const int64_t nbands = (nc + WAVE - 1) / WAVE;
     const int64_t nitems = W.items ? W.nitems : nbands * W.nseg;
-    const size_t cstride = (size_t)nc * W.cseg;
+    const size_t cstride = (size_t)W.cseg * nc;
     MxWin<2> &Wn = win[wave];
     float *rw = Wn.rw, *cw = Wn.cw;
     mx_win_init(Wn, lane);
@@ -1142,7 +1142,7 @@ __global__ __launch_bounds__(WG) RQA_WPE_MX(SCAN) void k_lam_mx(const float *__r
         const bool jv = jown < nc;
         const int64_t ib = (int64_t)q * seg_len;
         const int64_t ie = ib + seg_len < vend ? ib + seg_len : vend;
-        uint2 *cslot = carry + (size_t)(jv ? jown : 0) * W.cseg + q;
+        uint2 *cslot = carry + (size_t)q * nc + (jv ? jown : 0);
 
         uint32_t run[TG];
 #pragma unroll
@@ -1251,7 +1251,7 @@ __global__ __launch_bounds__(WG) void k_lam_vscan(const uint8_t *__restrict__ ti
 
     const int lane = threadIdx.x & (WAVE - 1);
     const int wave = wave_id();
-    const size_t cstride = (size_t)nc * W.cseg;
+    const size_t cstride = (size_t)W.cseg * nc;
 
     for (int64_t item = (int64_t)blockIdx.x * WAVES + wave; item < W.nitems; item += (int64_t)gridDim.x * WAVES) {
         const int2 it = W.items[item];
@@ -1262,7 +1262,7 @@ __global__ __launch_bounds__(WG) void k_lam_vscan(const uint8_t *__restrict__ ti
         const bool jv = jown < nc;
         const int64_t ib = (int64_t)q * seg_len;
         const int64_t ie = ib + seg_len < vend ? ib + seg_len : vend;
-        uint2 *cslot = carry + (size_t)(jv ? jown : 0) * W.cseg + q;
+        uint2 *cslot = carry + (size_t)q * nc + (jv ? jown : 0);
 
         uint32_t run[TG];
 #pragma unroll
@@ -1335,7 +1335,7 @@ __global__ __launch_bounds__(WG) RQA_WPE_MX(true) void k_det_mx(const float *__r
     mx_win_init(Wn, lane);
     const float *pa[8];
     gram_addrs(Wn, col, h, m, tau, pa);
-    const size_t cstride = (size_t)(nc - 1) * nseg;
+    const size_t cstride = (size_t)nseg * (nc - 1);
     const int rspan = 32 + (m - 1) * tau, cspan = 96 + (m - 1) * tau;   // <= MXW, MXW + 64
 
     for (int64_t item = (int64_t)blockIdx.x * WAVES + wave; item < nitems; item += (int64_t)gridDim.x * WAVES) {
@@ -1349,7 +1349,7 @@ __global__ __launch_bounds__(WG) RQA_WPE_MX(true) void k_det_mx(const float *__r
         const int64_t se_nom = sb + seg_len;
         const int64_t smax = nc - r0;
         const int64_t se = se_nom < smax ? se_nom : smax;
-        uint2 *cslot = carry + (size_t)(rv ? r - 1 : 0) * nseg + q;
+        uint2 *cslot = carry + (size_t)q * (nc - 1) + (rv ? r - 1 : 0);
 
         uint32_t run[TG];
 #pragma unroll
@@ -1441,11 +1441,13 @@ __global__ __launch_bounds__(WG) RQA_WPE_MX(true) void k_det_mx(const float *__r
 // Joins the (prefix, suffix) runs of each sequence's segments in order.
 // Block (x, t): threshold t, sequences x*STT + tid, + gridDim.x*STT, ...
 // Joined runs shorter than SCAP are counted in an LDS histogram per block.
-// A sequence has cseg carry slots.  DET sequences are diagonals r = sq + 1 of
-// length nc - r; LAM sequences are full columns, either one part (slots
-// 0 ..) or, split (split != 0), rows [0, V) in slots 0 .. and rows [V, nc)
-// in slots nseg .., V = min(64 (sq/64 + 1), nc).  Part segment q covers
-// [q*seg_len, min((q+1)*seg_len, part length)).
+// A sequence has cseg carry slots, slot q of sequence sq for threshold t at
+// carry[(t*cseg + q)*nseq + sq] (the sequences of one slot side by side, so
+// a wave's 64 lanes write, and the stitch reads, 512 contiguous bytes).  DET
+// sequences are diagonals r = sq + 1 of length nc - r; LAM sequences are full
+// columns, either one part (slots 0 ..) or, split (split != 0), rows [0, V)
+// in slots 0 .. and rows [V, nc) in slots nseg .., V = min(64 (sq/64 + 1),
+// nc).  Part segment q covers [q*seg_len, min((q+1)*seg_len, part length)).
 constexpr int STT = 256;
 constexpr int SCAP = 4096;
 
@@ -1464,12 +1466,12 @@ __global__ __launch_bounds__(STT) void k_stitch(const uint2 *__restrict__ carry,
             glob_add(&h[len], 1ull);
     };
     for (int64_t sq = (int64_t)blockIdx.x * STT + threadIdx.x; sq < nseq; sq += (int64_t)gridDim.x * STT) {
-        const uint2 *c = carry + ((size_t)t * nseq + sq) * cseg;
+        const uint2 *c = carry + (size_t)t * cseg * nseq + sq;
         uint64_t cur = 0;
         auto part = [&](int64_t plen, const uint2 *pc0) {
             const int64_t np = (plen + seg_len - 1) / seg_len;
             for (int64_t q = 0; q < np; q++) {
-                const uint2 pc = pc0[q];
+                const uint2 pc = pc0[(size_t)q * nseq];
                 const int64_t L = (q + 1) * seg_len <= plen ? seg_len : plen - q * seg_len;
                 if ((int64_t)pc.x == L) {   // whole segment recurrent
                     cur += (uint64_t)L;
@@ -1485,7 +1487,7 @@ __global__ __launch_bounds__(STT) void k_stitch(const uint2 *__restrict__ carry,
         } else if (split) {
             const int64_t V = min((sq / WAVE + 1) * WAVE, nc);
             part(V, c);
-            part(nc - V, c + nseg);
+            part(nc - V, c + (size_t)nseg * nseq);
         } else {
             part(nc, c);
         }
