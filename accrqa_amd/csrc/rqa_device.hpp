@@ -1503,13 +1503,45 @@ __global__ __launch_bounds__(STT) void k_stitch(const uint2 *__restrict__ carry,
 // five measures for every l_min (AccRQA_functions.cpp:48-133 and
 // rqa_process_length_histogram, AccRQA_CPU_function.cpp:181-204).
 // For DET it first applies correctDETHistogram (AccRQA_CPU_function.cpp:156-162).
-// metric/scan are workspaces of nt*(nc+1) u64 each.
+// metric/scan are workspaces of nt*(nc+1) u64 each (suffix sums, written up
+// to the highest non-zero bin below the last).
 // Output per (t, l): 5 values in F, laid out by out_index(t, l).
 constexpr int FIN = 1024;
 
+// Before k_finalize, grid (blocks, threshold): for DET, correctDETHistogram
+// (AccRQA_CPU_function.cpp:156-162: every bin doubled, then the
+// line-of-identity bin nc incremented); for both, the highest non-zero bin
+// below nc of each threshold, as tops[t] = bin + 1 (0 if none; tops zeroed
+// before the launch).
+constexpr int HPT = 256;
+__global__ __launch_bounds__(HPT) void k_hist_prep(unsigned long long *__restrict__ hist, int64_t nc, int is_det,
+                                                   unsigned long long *__restrict__ tops) {
+    const int t = blockIdx.y;
+    const int64_t hs = nc + 1;
+    unsigned long long *h = hist + (size_t)t * hs;
+    unsigned long long best = 0;
+    for (int64_t f = (int64_t)blockIdx.x * HPT + threadIdx.x; f < hs; f += (int64_t)gridDim.x * HPT) {
+        unsigned long long v = h[f];
+        if (is_det) {
+            v = (unsigned long long)(2.0 * (double)v) + (f == hs - 1 ? 1ull : 0ull);
+            h[f] = v;
+        }
+        if (v && f < hs - 1) best = (unsigned long long)f + 1;   // f grows, so this is the thread's highest
+    }
+    __shared__ unsigned long long red[HPT];
+    red[threadIdx.x] = best;
+    __syncthreads();
+    for (int off = HPT / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) red[threadIdx.x] = max(red[threadIdx.x], red[threadIdx.x + off]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && red[0]) atomicMax(&tops[t], red[0]);
+}
+
 template <typename F>
 __global__ __launch_bounds__(FIN) void k_finalize(unsigned long long *__restrict__ hist, int64_t nc, int nt,
-                                                  int is_det, const int *__restrict__ lmins, int nl,
+                                                  const unsigned long long *__restrict__ tops,
+                                                  const int *__restrict__ lmins, int nl,
                                                   unsigned long long *__restrict__ metric,
                                                   unsigned long long *__restrict__ scan, F *__restrict__ out,
                                                   int det_layout, const int *__restrict__ perm, int T,
@@ -1521,49 +1553,68 @@ __global__ __launch_bounds__(FIN) void k_finalize(unsigned long long *__restrict
     unsigned long long *M = metric + (size_t)t * hs;
     unsigned long long *Sc = scan + (size_t)t * hs;
     const int tid = threadIdx.x;
+    const bool is_det = det_layout != 0;
 
-    if (is_det) {
-        for (int64_t f = tid; f < hs; f += FIN) h[f] = (unsigned long long)(2.0 * (double)h[f]);
-        __syncthreads();
-        if (tid == 0) h[hs - 1] += 1;
-        __syncthreads();
-    }
+    // k_hist_prep has applied the DET correction and found top = the highest
+    // non-zero bin below hs - 1 (-1 if none): the histogram is zero above its
+    // longest line except the line-of-identity bin hs - 1.  Suffix sums are
+    // formed over [0, top] only, and read above it through SC / MM, where only
+    // bin hs - 1 contributes.
+    const int64_t top = (int64_t)tops[t] - 1;
+    const unsigned long long hl = h[hs - 1];
+    const unsigned long long ml_top = (unsigned long long)(hs - 1) * hl;   // M above top
 
-    // suffix sums: thread tid owns chunk [tid*C, min((tid+1)*C, hs))
-    const int64_t C = (hs + FIN - 1) / FIN;
-    const int64_t c0 = (int64_t)tid * C;
-    const int64_t c1 = c0 + C < hs ? c0 + C : hs;
-    unsigned long long sm = 0, ss = 0;
-    for (int64_t f = c0; f < c1; f++) {
-        sm += (unsigned long long)f * h[f];
-        ss += h[f];
-    }
+    // suffix sums M[f] = sum_{g >= f} g h[g], Sc[f] = sum_{g >= f} h[g] for
+    // f <= top: tiles of FIN*FPT bins from the top down, FPT consecutive bins
+    // per thread, an inclusive suffix scan of the threads' totals inside each
+    // tile, the tiles above carried in (cm, cs)
+    constexpr int FPT = 8;
+    constexpr int64_t TILE = (int64_t)FIN * FPT;
     __shared__ unsigned long long bm[FIN], bs[FIN];
-    bm[tid] = sm;
-    bs[tid] = ss;
-    __syncthreads();
-    // inclusive suffix scan over chunk totals (Hillis-Steele)
-    for (int off = 1; off < FIN; off <<= 1) {
-        unsigned long long am = 0, as = 0;
-        if (tid + off < FIN) {
-            am = bm[tid + off];
-            as = bs[tid + off];
+    unsigned long long cm = ml_top, cs = hl;
+    for (int64_t b0 = top >= 0 ? (top / TILE) * TILE : -1; b0 >= 0; b0 -= TILE) {
+        const int64_t f0 = b0 + (int64_t)tid * FPT;
+        unsigned long long v[FPT], lm = 0, ls = 0;
+#pragma unroll
+        for (int k = 0; k < FPT; k++) {
+            v[k] = f0 + k <= top ? h[f0 + k] : 0ull;
+            lm += (unsigned long long)(f0 + k) * v[k];
+            ls += v[k];
         }
+        bm[tid] = lm;
+        bs[tid] = ls;
         __syncthreads();
-        bm[tid] += am;
-        bs[tid] += as;
+        for (int off = 1; off < FIN; off <<= 1) {
+            unsigned long long am = 0, as = 0;
+            if (tid + off < FIN) {
+                am = bm[tid + off];
+                as = bs[tid + off];
+            }
+            __syncthreads();
+            bm[tid] += am;
+            bs[tid] += as;
+            __syncthreads();
+        }
+        unsigned long long rm = (tid + 1 < FIN ? bm[tid + 1] : 0ull) + cm;   // bins above my chunk
+        unsigned long long rs = (tid + 1 < FIN ? bs[tid + 1] : 0ull) + cs;
+#pragma unroll
+        for (int k = FPT - 1; k >= 0; k--) {
+            rm += (unsigned long long)(f0 + k) * v[k];
+            rs += v[k];
+            if (f0 + k <= top) {
+                M[f0 + k] = rm;
+                Sc[f0 + k] = rs;
+            }
+        }
+        const unsigned long long tm = bm[0], ts = bs[0];
         __syncthreads();
-    }
-    unsigned long long rm = tid + 1 < FIN ? bm[tid + 1] : 0ull;   // sum of chunks after mine
-    unsigned long long rs = tid + 1 < FIN ? bs[tid + 1] : 0ull;
-    for (int64_t f = c1 - 1; f >= c0; f--) {
-        rm += (unsigned long long)f * h[f];
-        rs += h[f];
-        M[f] = rm;
-        Sc[f] = rs;
+        cm += tm;
+        cs += ts;
     }
     __syncthreads();
     __threadfence_block();
+    auto SC = [&](int64_t f) -> unsigned long long { return f <= top ? Sc[f] : (f <= hs - 1 ? hl : 0ull); };
+    auto MM = [&](int64_t f) -> unsigned long long { return f <= top ? M[f] : (f <= hs - 1 ? ml_top : 0ull); };
 
     // Lmax / TTmax: the reference's bisection (tertiary_metric), one thread.
     __shared__ unsigned long long lmax_s;
@@ -1572,12 +1623,12 @@ __global__ __launch_bounds__(FIN) void k_finalize(unsigned long long *__restrict
         bool found = false;
         end = hs - 1;
         mid = hs / 2;
-        last = is_det ? Sc[end] : 0ull;
+        last = is_det ? SC((int64_t)end) : 0ull;
         start = 0;
         length = hs;
         while (!found) {
-            mv = Sc[mid];
-            mv1 = mid > 0 ? Sc[mid - 1] : 0ull;
+            mv = SC((int64_t)mid);
+            mv1 = mid > 0 ? SC((int64_t)mid - 1) : 0ull;
             if (mv == last && mv1 > mv) {
                 mid = mid - 1;
                 found = true;
@@ -1598,7 +1649,7 @@ __global__ __launch_bounds__(FIN) void k_finalize(unsigned long long *__restrict
             }
         }
         lmax_s = mid;
-        if (m0_out) m0_out[t] = M[0];   // recurrence count (sum of l * h[l])
+        if (m0_out) m0_out[t] = MM(0);   // recurrence count (sum of l * h[l])
     }
     __syncthreads();
 
@@ -1606,10 +1657,14 @@ __global__ __launch_bounds__(FIN) void k_finalize(unsigned long long *__restrict
     const int ot = perm ? perm[t] : t;   // position of this threshold in the caller's order
     for (int l = 0; l < nl; l++) {
         const int64_t lm = lmins[l];
-        const unsigned long long sl = Sc[lm];
+        const unsigned long long sl = SC(lm);
         double e = 0.0;
-        for (int64_t f = lm + tid; f < hs; f += FIN) {
+        for (int64_t f = lm + tid; f <= top; f += FIN) {
             const double p = (double)h[f] / (double)sl;
+            if (p > 0) e += p * log(p);
+        }
+        if (tid == 0 && hs - 1 > top && hs - 1 >= lm) {   // bin hs - 1
+            const double p = (double)hl / (double)sl;
             if (p > 0) e += p * log(p);
         }
         red[tid] = e;
@@ -1619,7 +1674,7 @@ __global__ __launch_bounds__(FIN) void k_finalize(unsigned long long *__restrict
             __syncthreads();
         }
         if (tid == 0) {
-            const unsigned long long m0 = M[0], ml = M[lm];
+            const unsigned long long m0 = MM(0), ml = MM(lm);
             const double v0 = m0 > 0 ? (double)ml / (double)m0 : __builtin_nan("");
             const double v1 = sl > 0 ? (double)ml / (double)sl : __builtin_nan("");
             const double v2 = (double)lmax_s;

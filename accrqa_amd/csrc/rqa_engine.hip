@@ -52,7 +52,7 @@ struct Ctx {
     int ncu = 0;
     hipStream_t stream = nullptr;
     DevBuf x, hist, carry, metric, scan, items_rr, items_det, items_dets, items_lamv, items_lamh, counts, perm, lmins,
-        lut, nrm, nmax, lut8, out_rr, out_det, out_lam, priv, tiles, dump;
+        lut, nrm, nmax, lut8, out_rr, out_det, out_lam, priv, tiles, dump, tops;
     // Raw-result sinks of accrqa_amd_histograms_*: when set, launch_group
     // copies each group's counts / h_D rows / h_V rows there, indexed by the
     // caller's threshold order.  Null for every other entry point.
@@ -129,7 +129,7 @@ int init_ctx() {
                              &g_ctx.items_lamh, &g_ctx.counts,  &g_ctx.perm,       &g_ctx.lmins,
                              &g_ctx.lut,     &g_ctx.nrm,        &g_ctx.nmax,       &g_ctx.lut8,
                              &g_ctx.out_rr,  &g_ctx.out_det,    &g_ctx.out_lam,    &g_ctx.priv,
-                             &g_ctx.tiles,   &g_ctx.dump,       &g_ctx.items_dets};
+                             &g_ctx.tiles,   &g_ctx.dump,       &g_ctx.items_dets, &g_ctx.tops};
             g_ctx.items_nc = -1;
             for (DevBuf *b : all) {
                 if (b->p) (void)hipFree(b->p);
@@ -265,6 +265,27 @@ __global__ void k_dump_rows(const unsigned long long *__restrict__ src, size_t h
     unsigned long long *o = out + (size_t)perm[t] * hs;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < hs; i += (size_t)gridDim.x * blockDim.x)
         o[i] = s[i];
+}
+
+// k_hist_prep (DET correction, highest occupied bin) then k_finalize (suffix
+// sums, Lmax, ENTR, the five measures) for the nt histograms of a group.
+template <typename F>
+int finalize(unsigned long long *hist, int64_t nc, int nt, int is_det, const int *d_lmins, int nl, F *out,
+             const int *d_perm, int T, unsigned long long *m0_out, hipStream_t st) {
+    Ctx &c = g_ctx;
+    int e;
+    if ((e = ensure(c.tops, std::max<size_t>(64, (size_t)nt) * sizeof(unsigned long long)))) return e;
+    unsigned long long *tops = (unsigned long long *)c.tops.p;
+    fill_zero(tops, (size_t)nt, st);
+    const int64_t hs = nc + 1;
+    const unsigned bx = (unsigned)std::min<int64_t>((hs + rqa::HPT - 1) / rqa::HPT, 64);
+    hipLaunchKernelGGL(rqa::k_hist_prep, dim3(bx, nt), dim3(rqa::HPT), 0, st, hist, nc, is_det, tops);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL((rqa::k_finalize<F>), dim3(nt), dim3(rqa::FIN), 0, st, hist, nc, nt,
+                       (const unsigned long long *)tops, d_lmins, nl, (unsigned long long *)c.metric.p,
+                       (unsigned long long *)c.scan.p, out, is_det, d_perm, T, m0_out);
+    HIP_TRY(hipGetLastError());
+    return E_OK;
 }
 
 int dump_rows(const unsigned long long *src, int nt, size_t hs, const int *d_perm, unsigned long long *out,
@@ -599,11 +620,8 @@ int launch_group(const F *dx, int64_t nc, int tau, int m, const F *bounds, int n
         // The LAM histogram covers every recurrent pair of the full matrix once,
         // so its sum of l*h[l] is the RR count as well.
         prof_begin("k_finalize", st);
-        hipLaunchKernelGGL((rqa::k_finalize<F>), dim3(nt), dim3(rqa::FIN), 0, st, hist, nc, nt, 0, d_lmins, nl,
-                           (unsigned long long *)c.metric.p, (unsigned long long *)c.scan.p, d_lam, 0, d_perm, T,
-                           counts);
+        if ((e = finalize<F>(hist, nc, nt, 0, d_lmins, nl, d_lam, d_perm, T, counts, st))) return e;
         prof_end(st);
-        HIP_TRY(hipGetLastError());
     }
     if (want_rr) {
         if ((e = dump_rows(counts, nt, 1, d_perm, c.dump_rr, st))) return e;
@@ -666,11 +684,8 @@ int launch_group(const F *dx, int64_t nc, int tau, int m, const F *bounds, int n
             HIP_TRY(hipGetLastError());
         }
         prof_begin("k_finalize", st);
-        hipLaunchKernelGGL((rqa::k_finalize<F>), dim3(nt), dim3(rqa::FIN), 0, st, hist, nc, nt, 1, d_lmins, nl,
-                           (unsigned long long *)c.metric.p, (unsigned long long *)c.scan.p, d_det, 1, d_perm, T,
-                           (unsigned long long *)nullptr);
+        if ((e = finalize<F>(hist, nc, nt, 1, d_lmins, nl, d_det, d_perm, T, nullptr, st))) return e;
         prof_end(st);
-        HIP_TRY(hipGetLastError());
         // finalize leaves h_D doubled with the line-of-identity bin
         if ((e = dump_rows(hist, nt, hs, d_perm, c.dump_det, st))) return e;
     }
@@ -701,10 +716,8 @@ int launch_never(int64_t nc, int nt, bool want_rr, bool want_det, bool want_lam,
         const bool det = kind == 1;
         if (det ? !want_det : !want_lam) continue;
         fill_zero(hist, (size_t)nt * hs, st);
-        hipLaunchKernelGGL((rqa::k_finalize<F>), dim3(nt), dim3(rqa::FIN), 0, st, hist, nc, nt, det ? 1 : 0,
-                           d_lmins, nl, (unsigned long long *)c.metric.p, (unsigned long long *)c.scan.p,
-                           det ? d_det : d_lam, det ? 1 : 0, d_perm, T, (unsigned long long *)nullptr);
-        HIP_TRY(hipGetLastError());
+        if ((e = finalize<F>(hist, nc, nt, det ? 1 : 0, d_lmins, nl, det ? d_det : d_lam, d_perm, T, nullptr, st)))
+            return e;
         if ((e = dump_rows(hist, nt, hs, d_perm, det ? c.dump_det : c.dump_lam, st))) return e;
     }
     return E_OK;
