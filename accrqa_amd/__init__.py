@@ -90,6 +90,10 @@ def load():
     L.accrqa_amd_set_form.restype = I
     L.accrqa_amd_set_split.argtypes = [I]
     L.accrqa_amd_set_split.restype = I
+    L.accrqa_amd_mx_stats.argtypes = [I]
+    L.accrqa_amd_mx_stats.restype = I
+    L.accrqa_amd_mx_stats_read.argtypes = [C.c_void_p]
+    L.accrqa_amd_mx_stats_read.restype = I
     L.accrqa_amd_last_error.restype = C.c_char_p
     L.accrqa_amd_version.restype = C.c_char_p
     _lib = L
@@ -119,6 +123,22 @@ def set_split(mode):
     e = load().accrqa_amd_set_split(int(mode))
     if e:
         raise AccrqaError(e, f"split mode {mode}")
+
+
+def mx_stats(enable=True):
+    """Switch the MFMA-form decision counters on (zeroing them) or off."""
+    e = load().accrqa_amd_mx_stats(int(bool(enable)))
+    if e:
+        raise AccrqaError(e, "mx_stats")
+
+
+def mx_stats_read():
+    """(elements ranked, flagged by the table, re-summed exactly) since mx_stats(True)."""
+    c = (C.c_ulonglong * 3)()
+    e = load().accrqa_amd_mx_stats_read(c)
+    if e:
+        raise AccrqaError(e, "mx_stats_read")
+    return int(c[0]), int(c[1]), int(c[2])
 
 
 def _dist(d):

@@ -1034,7 +1034,8 @@ __device__ __forceinline__ floatx16 gram_tile(const MxWin<NT> &W, const float *c
 // re-decides in exact order.  ni[e]: norm of element e's row, nj: norm of the
 // lane's column.
 __device__ __forceinline__ void tile_ranks(const floatx16 &acc, const float (&ni)[16], float nj, const uint8_t *lut_s,
-                                           const MxParams &P, const float *bnd_s, uint32_t (&Rq)[4], uint32_t &fm) {
+                                           const MxParams &P, const float *bnd_s, uint32_t (&Rq)[4], uint32_t &fm,
+                                           uint32_t &flw) {
 #pragma unroll
     for (int c = 0; c < 4; c++) Rq[c] = 0u;
 #pragma unroll
@@ -1046,6 +1047,7 @@ __device__ __forceinline__ void tile_ranks(const floatx16 &acc, const float (&ni
     // this lane's flagged elements: bit 8g + c <=> element 4g + c (table byte bit 7)
     const uint32_t fl = ((Rq[0] >> 7) & 0x01010101u) | ((Rq[1] >> 6) & 0x02020202u) |
                         ((Rq[2] >> 5) & 0x04040404u) | ((Rq[3] >> 4) & 0x08080808u);
+    flw = fl;
     if (__builtin_amdgcn_ballot_w64(fl != 0u) != 0ull) {
         // elements flagged in some lane, as one scalar word: the test per
         // element below is then a scalar bit test, not vector compares
@@ -1088,14 +1090,18 @@ __device__ __forceinline__ void tile_row_norms(const float *__restrict__ nrm, in
 // register (e&3) + 4h of that column.  One cross-half swap per register then
 // leaves lane l (half 0) with column j0 + l and lane l (half 1) with column
 // j0 + l, all 32 rows, and the scan is k_lam's.  SCAN = false (split form):
-// ranks and rank tiles only, the column runs left to k_lam_vscan.
-template <int TG, bool SCAN>
+// ranks and rank tiles only, the column runs left to k_lam_vscan.  STATS
+// (diagnostic instantiation): stats[0..2] += elements ranked, elements the
+// table flagged, elements re-decided in exact order.
+template <int TG, bool SCAN, bool STATS = false>
 __global__ __launch_bounds__(WG) RQA_WPE_MX(SCAN) void k_lam_mx(const float *__restrict__ x, const float *__restrict__ nrm, int64_t nc,
                                                int tau, int m, Bounds<float, TG> S, int nt,
                                                const uint8_t *__restrict__ lut, MxParams P, int seg_len, LamWork W,
                                                unsigned long long *__restrict__ hist, uint2 *__restrict__ carry,
-                                               uint32_t *__restrict__ priv_base, uint8_t *__restrict__ tiles) {
+                                               uint32_t *__restrict__ priv_base, uint8_t *__restrict__ tiles,
+                                               unsigned long long *__restrict__ stats) {
     __shared__ uint32_t lh[SCAN ? TG * LCAP : 1];
+    uint32_t st_rank = 0, st_flag = 0, st_exact = 0;
     __shared__ uint32_t dummy_bins[WAVE];
     __shared__ uint8_t lut_s[NBX_LAM];
     __shared__ float bnd_s[NBND];
@@ -1171,10 +1177,11 @@ __global__ __launch_bounds__(WG) RQA_WPE_MX(SCAN) void k_lam_mx(const float *__r
 #pragma unroll
             for (int e = 0; e < 16; e++) rowm |= ((8 * (e >> 2) + 4 * h + (e & 3)) < nb ? 1u : 0u) << e;
             uint32_t Rr[8], fm = 0u;
+            uint32_t flw0, flw1;
             {
                 const floatx16 acc = gram_tile<2, 0>(Wn, pa, col, h, m, tau);
                 uint32_t Rq[4], f;
-                tile_ranks(acc, ni, nrm[j0 + col], lut_s, P, bnd_s, Rq, f);
+                tile_ranks(acc, ni, nrm[j0 + col], lut_s, P, bnd_s, Rq, f, flw0);
 #pragma unroll
                 for (int c = 0; c < 4; c++) Rr[c] = Rq[c];
                 if (j0 + col < nc) fm |= f & rowm;
@@ -1182,10 +1189,23 @@ __global__ __launch_bounds__(WG) RQA_WPE_MX(SCAN) void k_lam_mx(const float *__r
             {
                 const floatx16 acc = gram_tile<2, 1>(Wn, pa, col, h, m, tau);
                 uint32_t Rq[4], f;
-                tile_ranks(acc, ni, nrm[j0 + 32 + col], lut_s, P, bnd_s, Rq, f);
+                tile_ranks(acc, ni, nrm[j0 + 32 + col], lut_s, P, bnd_s, Rq, f, flw1);
 #pragma unroll
                 for (int c = 0; c < 4; c++) Rr[4 + c] = Rq[c];
                 if (j0 + 32 + col < nc) fm |= (f & rowm) << 16;
+            }
+            if constexpr (STATS) {
+                // flag words to element bits (element e <-> bit 8(e>>2) + (e&3))
+                uint32_t e0 = 0, e1 = 0;
+#pragma unroll
+                for (int e = 0; e < 16; e++) {
+                    e0 |= ((flw0 >> (8 * (e >> 2) + (e & 3))) & 1u) << e;
+                    e1 |= ((flw1 >> (8 * (e >> 2) + (e & 3))) & 1u) << e;
+                }
+                const uint32_t v0 = j0 + col < nc ? rowm : 0u, v1 = j0 + 32 + col < nc ? rowm : 0u;
+                st_rank += __builtin_popcount(v0) + __builtin_popcount(v1);
+                st_flag += __builtin_popcount(e0 & v0) + __builtin_popcount(e1 & v1);
+                st_exact += __builtin_popcount(fm);
             }
             // exact re-decision of the ambiguous elements by their own lane
             while (fm) {
@@ -1225,6 +1245,11 @@ __global__ __launch_bounds__(WG) RQA_WPE_MX(SCAN) void k_lam_mx(const float *__r
     if constexpr (SCAN) {
         __syncthreads();
         flush_hist<TG>(lh, priv, nt, hist, nc, WG);
+    }
+    if constexpr (STATS) {
+        if (st_rank) atomicAdd(&stats[0], (unsigned long long)st_rank);
+        if (st_flag) atomicAdd(&stats[1], (unsigned long long)st_flag);
+        if (st_exact) atomicAdd(&stats[2], (unsigned long long)st_exact);
     }
 }
 
@@ -1396,8 +1421,8 @@ __global__ __launch_bounds__(WG) RQA_WPE_MX(true) void k_det_mx(const float *__r
                 const floatx16 acc = p == 0   ? gram_tile<3, 0>(Wn, pa, col, h, m, tau)
                                      : p == 1 ? gram_tile<3, 1>(Wn, pa, col, h, m, tau)
                                               : gram_tile<3, 2>(Wn, pa, col, h, m, tau);
-                uint32_t Rq[4], fm;
-                tile_ranks(acc, ni, nrm[c0 + 32 * p + col], lut_s, P, bnd_s, Rq, fm);
+                uint32_t Rq[4], fm, flw;
+                tile_ranks(acc, ni, nrm[c0 + 32 * p + col], lut_s, P, bnd_s, Rq, fm, flw);
                 uint32_t need = 0;
 #pragma unroll
                 for (int e = 0; e < 16; e++) {

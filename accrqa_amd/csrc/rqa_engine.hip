@@ -52,7 +52,7 @@ struct Ctx {
     int ncu = 0;
     hipStream_t stream = nullptr;
     DevBuf x, hist, carry, metric, scan, items_rr, items_det, items_dets, items_lamv, items_lamh, counts, perm, lmins,
-        lut, nrm, nmax, lut8, out_rr, out_det, out_lam, priv, tiles, dump, tops;
+        lut, nrm, nmax, lut8, out_rr, out_det, out_lam, priv, tiles, dump, tops, mxstats;
     // Raw-result sinks of accrqa_amd_histograms_*: when set, launch_group
     // copies each group's counts / h_D rows / h_V rows there, indexed by the
     // caller's threshold order.  Null for every other entry point.
@@ -72,6 +72,11 @@ int g_form = 0;
 
 // LAM/DET split through kept rank tiles: 0 = when the tiles fit, 1 = never.
 int g_split = 0;
+
+// MFMA-form decision statistics (diagnostic): when on, the split form's rank
+// kernel counts elements ranked, flagged by the table and re-decided in exact
+// order into the context's mxstats buffer (3 u64).
+bool g_mxstats = false;
 
 template <typename F>
 bool use_mfma(int dist, int m, int tau) {
@@ -129,7 +134,8 @@ int init_ctx() {
                              &g_ctx.items_lamh, &g_ctx.counts,  &g_ctx.perm,       &g_ctx.lmins,
                              &g_ctx.lut,     &g_ctx.nrm,        &g_ctx.nmax,       &g_ctx.lut8,
                              &g_ctx.out_rr,  &g_ctx.out_det,    &g_ctx.out_lam,    &g_ctx.priv,
-                             &g_ctx.tiles,   &g_ctx.dump,       &g_ctx.items_dets, &g_ctx.tops};
+                             &g_ctx.tiles,   &g_ctx.dump,       &g_ctx.items_dets, &g_ctx.tops,
+                             &g_ctx.mxstats};
             g_ctx.items_nc = -1;
             for (DevBuf *b : all) {
                 if (b->p) (void)hipFree(b->p);
@@ -558,11 +564,21 @@ int launch_group(const F *dx, int64_t nc, int tau, int m, const F *bounds, int n
                 HIP_TRY(hipGetLastError());
                 if (tiles) {
                     // split form: ranks and tiles, then the column runs from the tiles
-                    const unsigned g = grid_for(rqa::k_lam_mx<TG, false>, LW.nitems, c.ncu);
                     prof_begin("k_lam_mx", st);
-                    hipLaunchKernelGGL((rqa::k_lam_mx<TG, false>), dim3(g), dim3(rqa::WG), 0, st, (const float *)dx,
-                                       (const float *)c.nrm.p, nc, tau, m, Bf, nt, (const uint8_t *)c.lut8.p, MP, L,
-                                       LW, hist, carry, (uint32_t *)c.priv.p, tiles);
+                    if (g_mxstats) {
+                        if ((e = ensure(c.mxstats, 3 * sizeof(unsigned long long)))) return e;
+                        const unsigned g = grid_for(rqa::k_lam_mx<TG, false, true>, LW.nitems, c.ncu);
+                        hipLaunchKernelGGL((rqa::k_lam_mx<TG, false, true>), dim3(g), dim3(rqa::WG), 0, st,
+                                           (const float *)dx, (const float *)c.nrm.p, nc, tau, m, Bf, nt,
+                                           (const uint8_t *)c.lut8.p, MP, L, LW, hist, carry, (uint32_t *)c.priv.p,
+                                           tiles, (unsigned long long *)c.mxstats.p);
+                    } else {
+                        const unsigned g = grid_for(rqa::k_lam_mx<TG, false>, LW.nitems, c.ncu);
+                        hipLaunchKernelGGL((rqa::k_lam_mx<TG, false>), dim3(g), dim3(rqa::WG), 0, st,
+                                           (const float *)dx, (const float *)c.nrm.p, nc, tau, m, Bf, nt,
+                                           (const uint8_t *)c.lut8.p, MP, L, LW, hist, carry, (uint32_t *)c.priv.p,
+                                           tiles, (unsigned long long *)nullptr);
+                    }
                     prof_end(st);
                     HIP_TRY(hipGetLastError());
                     const unsigned gv = grid_for(rqa::k_lam_vscan<TG>, LW.nitems, c.ncu);
@@ -579,7 +595,7 @@ int launch_group(const F *dx, int64_t nc, int tau, int m, const F *bounds, int n
                     prof_begin("k_lam_mx", st);
                     hipLaunchKernelGGL((rqa::k_lam_mx<TG, true>), dim3(g), dim3(rqa::WG), 0, st, (const float *)dx,
                                        (const float *)c.nrm.p, nc, tau, m, Bf, nt, (const uint8_t *)c.lut8.p, MP, L,
-                                       LW, hist, carry, (uint32_t *)c.priv.p, tiles);
+                                       LW, hist, carry, (uint32_t *)c.priv.p, tiles, (unsigned long long *)nullptr);
                     prof_end(st);
                     HIP_TRY(hipGetLastError());
                 }
@@ -1066,6 +1082,27 @@ int accrqa_amd_set_split(int mode) {
     if (mode < 0 || mode > 1) return E_INVALID;
     std::lock_guard<std::mutex> lk(g_mu);
     g_split = mode;
+    return E_OK;
+}
+
+int accrqa_amd_mx_stats(int enable) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    int e = init_ctx();
+    if (e) return e;
+    if ((e = ensure(g_ctx.mxstats, 3 * sizeof(unsigned long long)))) return e;
+    fill_zero((unsigned long long *)g_ctx.mxstats.p, 3, g_ctx.stream);
+    HIP_TRY(hipStreamSynchronize(g_ctx.stream));
+    g_mxstats = enable != 0;
+    return E_OK;
+}
+
+int accrqa_amd_mx_stats_read(unsigned long long *counts) {
+    if (!counts) return E_NULL;
+    std::lock_guard<std::mutex> lk(g_mu);
+    counts[0] = counts[1] = counts[2] = 0;
+    if (!g_ctx.mxstats.p) return E_OK;
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(counts, g_ctx.mxstats.p, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
     return E_OK;
 }
 

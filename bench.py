@@ -169,6 +169,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--form", default="both", choices=["both", "exact", "mfma"],
                     help="Euclidean distance form(s) to time; 'both' reports the MFMA form as value")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the untimed extras: decision counters, m = 32 / 64 and fp64-input steps")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -289,9 +291,70 @@ def main():
         "mismatches_mfma_vs_exact": mismatches,
         "rr_min_max": [float(rr.min()), float(rr.max())],
     }
+    if not args.no_extras and "mfma" in res:
+        out.update(extras(accrqa_amd, L, torch, args, c, x, step))
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(x, tau, emb, thr, lmin, dist, args.cpu_budget)
     print(json.dumps(out), flush=True)
+
+
+def extras(accrqa_amd, L, torch, args, c, x, step):
+    """Reported beside the headline, outside its timed region:
+    * mfma_decisions: one MFMA-form step with the decision counters on:
+      elements ranked by the rank pass, the fraction whose table byte was
+      flagged and the fraction re-summed in exact order (the recheck fraction);
+    * config3_m: the same series at m = 32 and 64 (thresholds
+      linspace(0.1, 1.6) * sqrt(m)), both forms, outputs compared bit for bit;
+    * fp64_input: the m = 16 workload with the series and thresholds in fp64
+      (exact fp64 form; the MFMA form is fp32-only)."""
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    out = {}
+    accrqa_amd.set_form(accrqa_amd.FORM_MFMA)
+    accrqa_amd.mx_stats(True)
+    step()
+    torch.cuda.synchronize()
+    ranked, flagged, exact = accrqa_amd.mx_stats_read()
+    accrqa_amd.mx_stats(False)
+    accrqa_amd.set_form(accrqa_amd.FORM_AUTO)
+    out["mfma_decisions"] = {"elements_ranked": ranked, "flagged_frac": flagged / max(ranked, 1),
+                             "recheck_frac": exact / max(ranked, 1)}
+
+    def timed(xs, tau, emb, thr, lmin, form, dtype, steps=2):
+        T, nl = len(thr), len(lmin)
+        dxs = torch.from_numpy(xs).to(dev)
+        tdt = torch.float64 if dtype == np.float64 else torch.float32
+        o = [torch.zeros(n, dtype=tdt, device=dev) for n in (T, T * nl * 5, nl * T * 5)]
+        accrqa_amd.set_form(form)
+
+        def one():
+            accrqa_amd.rqa_device(dxs.data_ptr(), xs.size, tau, emb, thr, lmin, 1, o[0].data_ptr(), o[1].data_ptr(),
+                                  o[2].data_ptr(), stream=stream.cuda_stream, dtype=dtype)
+        one()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            one()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        accrqa_amd.set_form(accrqa_amd.FORM_AUTO)
+        nc = xs.size - (emb - 1) * tau
+        return ms, float(nc) * nc * T / (ms / 1e3), [v.cpu().numpy().copy() for v in o]
+
+    rows = {}
+    for m in (32, 64):
+        thr = [float(v) for v in np.linspace(0.1, 1.6, 16) * np.sqrt(m)]
+        me, ve, oe = timed(x, 1, m, thr, [2], accrqa_amd.FORM_EXACT, np.float32)
+        mm, vm, om = timed(x, 1, m, thr, [2], accrqa_amd.FORM_MFMA, np.float32)
+        mis = int(sum(int((u.view(np.uint32) != v.view(np.uint32)).sum()) for u, v in zip(oe, om)))
+        rows[f"m{m}"] = {"exact_ms_per_step": me, "exact_value": ve, "mfma_ms_per_step": mm, "mfma_value": vm,
+                         "mfma_vs_exact_speedup": me / mm, "mismatches_mfma_vs_exact": mis}
+    out["config3_m"] = rows
+    x64 = np.ascontiguousarray(c["x"], dtype=np.float64)
+    ms64, v64, _ = timed(x64, c["tau"][0], c["emb"][0], [float(v) for v in c["thr"]], c["lmin"],
+                         accrqa_amd.FORM_EXACT, np.float64)
+    out["fp64_input"] = {"form": "exact-order FP64 sums on VALU", "ms_per_step": ms64, "value": v64}
+    return out
 
 
 if __name__ == "__main__":

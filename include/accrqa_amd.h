@@ -129,6 +129,18 @@ int accrqa_amd_set_form(int form);
  * either way.  Process-wide; returns 0, or 2 for an unknown mode. */
 int accrqa_amd_set_split(int mode);
 
+/* MFMA-form decision statistics (diagnostic, no counterpart in the
+ * reference): accrqa_amd_mx_stats(1) zeroes the counters and switches them
+ * on, 0 switches them off.  While on, every MFMA-form LAM pass that keeps the
+ * rank tiles adds to counts[0] the pair elements it ranked (upper triangle and
+ * the diagonal bands), counts[1] those whose table byte was flagged (a bound
+ * within the Gram error bound of the bucket) and counts[2] those re-summed in
+ * exact order; counts[2] / counts[0] is the recheck fraction.  A separate
+ * kernel instantiation does the counting, so the default path is unchanged.
+ * Both return 0 or an Accrqa_Error code. */
+int accrqa_amd_mx_stats(int enable);
+int accrqa_amd_mx_stats_read(unsigned long long *counts);
+
 /* Kernel timing (tracing aid, no counterpart in the reference): when enabled,
  * every kernel launch is bracketed by HIP events on its own stream and the
  * elapsed times are summed per kernel name ("k_lam", "k_lam_mx",

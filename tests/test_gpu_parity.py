@@ -383,3 +383,30 @@ def test_split_and_direct_forms(amd, oracle):
                             assert np.array_equal(a[kind][k], want), (name, dt, dist, kind, thr[k])
     finally:
         amd.set_split(amd.SPLIT_AUTO)
+
+
+def test_mfma_decision_counters(mfma_form, oracle):
+    """The MFMA form's diagnostic counters: elements ranked equals the split
+    form's column-pass pairs (each band's columns times V_J rows), flagged and
+    re-summed counts are ordered, and counting changes no result."""
+    amd = mfma_form
+    rng = np.random.default_rng(5)
+    x = np.cumsum(rng.standard_normal(3000)).astype(np.float32)
+    x = ((x - x.mean()) / x.std()).astype(np.float32)
+    thr = np.linspace(0.3, 3.0, 16).astype(np.float32)
+    tau, m = 1, 16
+    nc = x.size - (m - 1) * tau
+    base = amd.histograms(x, tau, m, thr, 1, which=6)
+    amd.mx_stats(True)
+    try:
+        got = amd.histograms(x, tau, m, thr, 1, which=6)
+        ranked, flagged, exact = amd.mx_stats_read()
+    finally:
+        amd.mx_stats(False)
+    assert np.array_equal(got["det"], base["det"]) and np.array_equal(got["lam"], base["lam"])
+    want = sum(min(64, nc - 64 * J) * min(64 * (J + 1), nc) for J in range(-(-nc // 64)))
+    assert ranked == want, (ranked, want)
+    assert 0 <= exact <= flagged <= ranked, (ranked, flagged, exact)
+    assert exact < ranked // 100, (ranked, exact)
+    for k, t in enumerate(thr):
+        assert np.array_equal(got["lam"][k], oracle.hist("lam", x, np.float32(t), tau, m, 1)), t
