@@ -837,7 +837,8 @@ struct MxParams {
 // n_i = sum_k x[i + k tau]^2 (fp64, rounded to fp32); entries nc .. nc+127 = 0.
 // nmax_bits: atomic max of the float bits (non-negative floats order as uints;
 // NaN bits are above +inf, so a NaN norm forces the all-exact fallback).
-__global__ void k_norms(const float *__restrict__ x, int64_t nc, int tau, int m, float *__restrict__ nrm,
+template <typename F>
+__global__ void k_norms(const F *__restrict__ x, int64_t nc, int tau, int m, float *__restrict__ nrm,
                         unsigned int *__restrict__ nmax_bits) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nc + 2 * WAVE) return;
@@ -919,10 +920,11 @@ __device__ __forceinline__ uint32_t mx_second(uint32_t byte, float st, float sum
 
 // Exact-order fp32 squared distance of one pair (the reference's sum).
 // Loads go out four dimensions at a time; the sum keeps the order k = 0..m-1.
-__device__ __forceinline__ float exact_sqdist(const float *__restrict__ x, int64_t i, int64_t j, int tau, int m) {
-    float s = 0.f;
+template <typename F>
+__device__ __forceinline__ F exact_sqdist(const F *__restrict__ x, int64_t i, int64_t j, int tau, int m) {
+    F s = F(0);
     for (int k0 = 0; k0 < m; k0 += 4) {
-        float a[4], b[4];
+        F a[4], b[4];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             const int64_t o = (int64_t)(k0 + k < m ? k0 + k : m - 1) * tau;
@@ -932,8 +934,8 @@ __device__ __forceinline__ float exact_sqdist(const float *__restrict__ x, int64
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             if (k0 + k < m) {
-                const float d = a[k] - b[k];
-                const float q = d * d;
+                const F d = a[k] - b[k];
+                const F q = d * d;
                 s = s + q;
             }
         }
@@ -1093,9 +1095,20 @@ __device__ __forceinline__ void tile_row_norms(const float *__restrict__ nrm, in
 // ranks and rank tiles only, the column runs left to k_lam_vscan.  STATS
 // (diagnostic instantiation): stats[0..2] += elements ranked, elements the
 // table flagged, elements re-decided in exact order.
-template <int TG, bool SCAN, bool STATS = false>
-__global__ __launch_bounds__(WG) RQA_WPE_MX(SCAN) void k_lam_mx(const float *__restrict__ x, const float *__restrict__ nrm, int64_t nc,
-                                               int tau, int m, Bounds<float, TG> S, int nt,
+//
+// F = double (fp64 input): the windows hold the series rounded to fp32, the
+// norms are fp64 sums rounded to fp32, and the exact re-decision sums the
+// fp64 series in the reference's order against the fp64 bounds SX.  Against
+// the reference's fp64 s: each rounded product adds at most 2u|x_i x_j|
+// (u = 2^-24), i.e. u (n_i + n_j) to the dot product and 2u (n_i + n_j) to
+// s~ after doubling; the reference's own fp64 error is below u (n_i + n_j);
+// the table's and the second stage's bounds are the fp64 bounds rounded to
+// fp32, off by at most 2u (n_i + n_j) near any s~ that reaches them.  The
+// host adds 4u to the fp32 coefficient, (4m + 20) u (n_i + n_j), which
+// covers all of it.
+template <typename F, int TG, bool SCAN, bool STATS = false>
+__global__ __launch_bounds__(WG) RQA_WPE_MX(SCAN) void k_lam_mx(const F *__restrict__ x, const float *__restrict__ nrm, int64_t nc,
+                                               int tau, int m, Bounds<float, TG> S, Bounds<F, TG> SX, int nt,
                                                const uint8_t *__restrict__ lut, MxParams P, int seg_len, LamWork W,
                                                unsigned long long *__restrict__ hist, uint2 *__restrict__ carry,
                                                uint32_t *__restrict__ priv_base, uint8_t *__restrict__ tiles,
@@ -1105,12 +1118,24 @@ __global__ __launch_bounds__(WG) RQA_WPE_MX(SCAN) void k_lam_mx(const float *__r
     __shared__ uint32_t dummy_bins[WAVE];
     __shared__ uint8_t lut_s[NBX_LAM];
     __shared__ float bnd_s[NBND];
-    // rw: x[i0 .. i0 + 31 + (m-1)tau] of the current block; cw: x[j0 .. j0 + 63 + (m-1)tau] of the item
+    // the exact re-decision's bounds in the input type (fp64 input: the
+    // table and the second stage use them rounded to fp32, S, with the
+    // rounding inside the error bound cE)
+    __shared__ F bndx_s[sizeof(F) == 8 ? NBND : 1];
+    // rw: x[i0 .. i0 + 31 + (m-1)tau] of the current block; cw: x[j0 .. j0 + 63 + (m-1)tau] of the item,
+    // both rounded to fp32
     __shared__ MxWin<2> win[WAVES];
     if constexpr (SCAN)
         for (int e = threadIdx.x; e < TG * LCAP; e += WG) lh[e] = 0;
     for (int e = threadIdx.x; e < P.nb / 4; e += WG) ((uint32_t *)lut_s)[e] = ((const uint32_t *)lut)[e];
     if (threadIdx.x < NBND) bnd_s[threadIdx.x] = (int)threadIdx.x < nt ? S.s[threadIdx.x] : __int_as_float(0x7F800000);
+    if constexpr (sizeof(F) == 8)
+        if (threadIdx.x < NBND) bndx_s[threadIdx.x] = (int)threadIdx.x < nt ? SX.s[threadIdx.x] : (F)__builtin_inf();
+    const F *bndx;
+    if constexpr (sizeof(F) == 8)
+        bndx = bndx_s;
+    else
+        bndx = bnd_s;
     if constexpr (SCAN) priv_zero<TG>(priv_slice<TG>(priv_base), nt, WG);
     __syncthreads();
     uint32_t *priv = priv_slice<TG>(priv_base);
@@ -1212,8 +1237,8 @@ __global__ __launch_bounds__(WG) RQA_WPE_MX(SCAN) void k_lam_mx(const float *__r
                 const uint32_t b = __builtin_ctz(fm);
                 fm &= fm - 1u;
                 const uint32_t e = b & 15u, p = b >> 4;
-                const float sx = exact_sqdist(x, i0 + 4 * h + 8 * (e >> 2) + (e & 3), j0 + 32 * p + col, tau, m);
-                const uint32_t rk = rank_direct<float, TG>(sx, bnd_s, nt);
+                const F sx = exact_sqdist<F>(x, i0 + 4 * h + 8 * (e >> 2) + (e & 3), j0 + 32 * p + col, tau, m);
+                const uint32_t rk = rank_direct<F, TG>(sx, bndx, nt);
                 const uint32_t c = 4 * p + (e & 3), sh = 8 * (e >> 2), keep = ~(0xFFu << sh), val = rk << sh;
 #pragma unroll
                 for (int cc = 0; cc < 8; cc++)
@@ -1442,7 +1467,7 @@ __global__ __launch_bounds__(WG) RQA_WPE_MX(true) void k_det_mx(const float *__r
                 fma &= fma - 1u;
                 const uint32_t e = b & 15u, p = b >> 4;
                 const uint32_t u = 8 * (e >> 2) + 4 * h + (e & 3);
-                const float sx = exact_sqdist(x, s0 + u, c0 + 32 * p + col, tau, m);
+                const float sx = exact_sqdist<float>(x, s0 + u, c0 + 32 * p + col, tau, m);
                 const uint32_t l = 32 * p + col - u;
                 sk8[l * SKB + 4 * (u & 7) + (u >> 3)] = (uint8_t)rank_direct<float, TG>(sx, bnd_s, nt);
             }

@@ -78,9 +78,11 @@ int g_split = 0;
 // order into the context's mxstats buffer (3 u64).
 bool g_mxstats = false;
 
+// MFMA form for F = float and, in the LAM pass, F = double (fp32 Gram of the
+// rounded series, exact re-decision in fp64); the direct DET kernel is fp32-only.
 template <typename F>
 bool use_mfma(int dist, int m, int tau) {
-    if (sizeof(F) != 4 || dist != 1) return false;
+    if (dist != 1) return false;
     if ((int64_t)(m - 1) * tau > rqa::MXW - 32) return false;   // operand window must fit in LDS
     if (g_form == 1) return false;
     if (g_form == 2) return true;
@@ -485,7 +487,10 @@ int build_items(int64_t nc, hipStream_t st) {
 }
 
 // Parameters of the MFMA form's byte table for sorted bounds b[0..nt).
-rqa::MxParams mx_params(const float *b, int nt, int m, int nbmax) {
+// fp64 input: the Gram's inputs and the table's bounds are rounded to fp32;
+// extra_u units of 2^-24 (n_i + n_j) cover that (see the fp64 note at
+// rqa::k_lam_mx).
+rqa::MxParams mx_params(const float *b, int nt, int m, int nbmax, int extra_u = 0) {
     rqa::MxParams P;
     const float lo = b[0] * 0.5f;
     float hi = b[nt - 1] * 2.0f;
@@ -501,7 +506,7 @@ rqa::MxParams mx_params(const float *b, int nt, int m, int nbmax) {
     P.sh = sh;
     P.nb = nb;
     P.qlo = (int)(base_al >> sh);
-    P.cE = (float)((4.0 * m + 16.0) * std::ldexp(1.0, -24));
+    P.cE = (float)((4.0 * m + 16.0 + extra_u) * std::ldexp(1.0, -24));
     return P;
 }
 
@@ -553,11 +558,11 @@ int launch_group(const F *dx, int64_t nc, int tau, int m, const F *bounds, int n
 
     if (want_lam) {
         fill_zero(hist, (size_t)nt * hs, st);
-        if constexpr (sizeof(F) == 4 && NORM == 1) {
+        if constexpr (NORM == 1) {
             if (use_mfma<F>(1, m, tau)) {
                 rqa::Bounds<float, TG> Bf;
                 for (int t = 0; t < TG; t++) Bf.s[t] = (float)B.s[t];
-                const rqa::MxParams MP = mx_params((const float *)bounds, nt, m, rqa::NBX_LAM);
+                const rqa::MxParams MP = mx_params(Bf.s, nt, m, rqa::NBX_LAM, sizeof(F) == 8 ? 4 : 0);
                 if ((e = ensure(c.lut8, rqa::NBX_LAM))) return e;
                 hipLaunchKernelGGL((rqa::k_mx_lut<TG>), dim3((MP.nb + 255) / 256), dim3(256), 0, st, Bf, nt, MP,
                                    (const unsigned int *)c.nmax.p, (uint8_t *)c.lut8.p);
@@ -567,15 +572,15 @@ int launch_group(const F *dx, int64_t nc, int tau, int m, const F *bounds, int n
                     prof_begin("k_lam_mx", st);
                     if (g_mxstats) {
                         if ((e = ensure(c.mxstats, 3 * sizeof(unsigned long long)))) return e;
-                        const unsigned g = grid_for(rqa::k_lam_mx<TG, false, true>, LW.nitems, c.ncu);
-                        hipLaunchKernelGGL((rqa::k_lam_mx<TG, false, true>), dim3(g), dim3(rqa::WG), 0, st,
-                                           (const float *)dx, (const float *)c.nrm.p, nc, tau, m, Bf, nt,
+                        const unsigned g = grid_for(rqa::k_lam_mx<F, TG, false, true>, LW.nitems, c.ncu);
+                        hipLaunchKernelGGL((rqa::k_lam_mx<F, TG, false, true>), dim3(g), dim3(rqa::WG), 0, st,
+                                           dx, (const float *)c.nrm.p, nc, tau, m, Bf, B, nt,
                                            (const uint8_t *)c.lut8.p, MP, L, LW, hist, carry, (uint32_t *)c.priv.p,
                                            tiles, (unsigned long long *)c.mxstats.p);
                     } else {
-                        const unsigned g = grid_for(rqa::k_lam_mx<TG, false>, LW.nitems, c.ncu);
-                        hipLaunchKernelGGL((rqa::k_lam_mx<TG, false>), dim3(g), dim3(rqa::WG), 0, st,
-                                           (const float *)dx, (const float *)c.nrm.p, nc, tau, m, Bf, nt,
+                        const unsigned g = grid_for(rqa::k_lam_mx<F, TG, false>, LW.nitems, c.ncu);
+                        hipLaunchKernelGGL((rqa::k_lam_mx<F, TG, false>), dim3(g), dim3(rqa::WG), 0, st,
+                                           dx, (const float *)c.nrm.p, nc, tau, m, Bf, B, nt,
                                            (const uint8_t *)c.lut8.p, MP, L, LW, hist, carry, (uint32_t *)c.priv.p,
                                            tiles, (unsigned long long *)nullptr);
                     }
@@ -590,11 +595,11 @@ int launch_group(const F *dx, int64_t nc, int tau, int m, const F *bounds, int n
                     prof_end(st);
                     HIP_TRY(hipGetLastError());
                 } else {
-                    const unsigned g = grid_for(rqa::k_lam_mx<TG, true>, LW.nitems, c.ncu);
+                    const unsigned g = grid_for(rqa::k_lam_mx<F, TG, true>, LW.nitems, c.ncu);
                     if ((e = ensure_priv<TG>(g))) return e;
                     prof_begin("k_lam_mx", st);
-                    hipLaunchKernelGGL((rqa::k_lam_mx<TG, true>), dim3(g), dim3(rqa::WG), 0, st, (const float *)dx,
-                                       (const float *)c.nrm.p, nc, tau, m, Bf, nt, (const uint8_t *)c.lut8.p, MP, L,
+                    hipLaunchKernelGGL((rqa::k_lam_mx<F, TG, true>), dim3(g), dim3(rqa::WG), 0, st, dx,
+                                       (const float *)c.nrm.p, nc, tau, m, Bf, B, nt, (const uint8_t *)c.lut8.p, MP, L,
                                        LW, hist, carry, (uint32_t *)c.priv.p, tiles, (unsigned long long *)nullptr);
                     prof_end(st);
                     HIP_TRY(hipGetLastError());
@@ -806,7 +811,7 @@ int run_block(const F *dx, size_t n, int tau, int emb, const F *thr, int T, cons
         if ((e = ensure(c.nmax, 256))) return e;
         fill_zero((unsigned long long *)c.nmax.p, 1, st);
         const int64_t nthr = nc + 2 * rqa::WAVE;
-        hipLaunchKernelGGL(rqa::k_norms, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, st, (const float *)dx, nc,
+        hipLaunchKernelGGL((rqa::k_norms<F>), dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, st, dx, nc,
                            tau, emb, (float *)c.nrm.p, (unsigned int *)c.nmax.p);
         HIP_TRY(hipGetLastError());
     }

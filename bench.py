@@ -305,8 +305,9 @@ def extras(accrqa_amd, L, torch, args, c, x, step):
       flagged and the fraction re-summed in exact order (the recheck fraction);
     * config3_m: the same series at m = 32 and 64 (thresholds
       linspace(0.1, 1.6) * sqrt(m)), both forms, outputs compared bit for bit;
-    * fp64_input: the m = 16 workload with the series and thresholds in fp64
-      (exact fp64 form; the MFMA form is fp32-only)."""
+    * fp64_input: the m = 16 workload with the series and thresholds in fp64,
+      both forms (the MFMA form: FP32 Gram of the rounded series, exact
+      re-decision in FP64), outputs compared bit for bit."""
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream(dev)
     out = {}
@@ -351,9 +352,14 @@ def extras(accrqa_amd, L, torch, args, c, x, step):
                          "mfma_vs_exact_speedup": me / mm, "mismatches_mfma_vs_exact": mis}
     out["config3_m"] = rows
     x64 = np.ascontiguousarray(c["x"], dtype=np.float64)
-    ms64, v64, _ = timed(x64, c["tau"][0], c["emb"][0], [float(v) for v in c["thr"]], c["lmin"],
-                         accrqa_amd.FORM_EXACT, np.float64)
-    out["fp64_input"] = {"form": "exact-order FP64 sums on VALU", "ms_per_step": ms64, "value": v64}
+    args64 = (x64, c["tau"][0], c["emb"][0], [float(v) for v in c["thr"]], c["lmin"])
+    me, ve, oe = timed(*args64, accrqa_amd.FORM_EXACT, np.float64)
+    mm, vm, om = timed(*args64, accrqa_amd.FORM_MFMA, np.float64)
+    mis = int(sum(int((u.view(np.uint64) != v.view(np.uint64)).sum()) for u, v in zip(oe, om)))
+    out["fp64_input"] = {"exact_ms_per_step": me, "exact_value": ve, "mfma_ms_per_step": mm, "mfma_value": vm,
+                         "mfma_vs_exact_speedup": me / mm, "mismatches_mfma_vs_exact": mis,
+                         "forms": {"exact": "exact-order FP64 sums on VALU",
+                                   "mfma": "FP32 MFMA Gram of the rounded series + FP64 exact-order recheck"}}
     return out
 
 

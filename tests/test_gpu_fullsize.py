@@ -79,6 +79,23 @@ def test_config3_gate_shape(forms):
     check_counts(amd, x, tau, emb, thr, 1, mx, "config3")
 
 
+def test_config3_fp64_input(forms):
+    """The gate workload in fp64: the MFMA form (fp32 Gram of the rounded
+    series, exact re-decision in fp64) against the exact fp64 form."""
+    amd = forms
+    c = W.config3(m=16, dtype=np.float64)
+    x, tau, emb, thr = c["x"], c["tau"][0], c["emb"][0], c["thr"]
+    amd.set_form(amd.FORM_EXACT)
+    say("config3 fp64 exact")
+    ex = hists(amd, x, tau, emb, thr, 1)
+    amd.set_form(amd.FORM_MFMA)
+    say("config3 fp64 mfma")
+    mx = hists(amd, x, tau, emb, thr, 1)
+    assert_same(ex, mx, "config3 fp64 exact vs mfma")
+    say("config3 fp64 counts")
+    check_counts(amd, x, tau, emb, thr, 1, mx, "config3 fp64")
+
+
 def test_config2_both_norms(forms):
     """Lorenz x, N = 131,072, tau = 8, m = 3, 20 thresholds, both norms."""
     amd = forms

@@ -343,13 +343,22 @@ def test_mfma_form_bit_exact(mfma_form, oracle):
     # unused steps in the first chunk of 16 dimensions, and a second chunk
     cases.append(("tau0", rng.standard_normal(1500).astype(np.float32), [0, 3], [10, 21, 5], [2],
                   list(np.linspace(0.1, 4.0, 9).astype(np.float32))))
+    # fp64 input: fp32 Gram of the rounded series, exact re-decision in fp64
+    # (the same cases, the thresholds and series widened, plus one series
+    # whose fp64 values are not fp32 values)
+    cases += [(n + " fp64", x.astype(np.float64), taus, embs, lm, [float(t) for t in thrs])
+              for n, x, taus, embs, lm, thrs in list(cases)]
+    w = np.cumsum(rng.standard_normal(2500))
+    cases.append(("walk fp64 fine", (w - w.mean()) / w.std(), [1], [12], [2],
+                  list(np.linspace(0.2, 3.0, 17))))
     for name, x, taus, embs, lm, thrs in cases:
         say("mfma", name)
+        dt = x.dtype.type
         check_case(amd, oracle, x, taus, embs, lm, thrs, 1, f"mfma {name}")
-        got = amd.histograms(x, taus[0], embs[0], np.asarray(thrs, np.float32), 1, which=7)
+        got = amd.histograms(x, taus[0], embs[0], np.asarray(thrs, dt), 1, which=7)
         for k, t in enumerate(thrs):
-            assert np.array_equal(got["lam"][k], oracle.hist("lam", x, np.float32(t), taus[0], embs[0], 1)), (name, t)
-            assert np.array_equal(got["det"][k], oracle.hist("det", x, np.float32(t), taus[0], embs[0], 1)), (name, t)
+            assert np.array_equal(got["lam"][k], oracle.hist("lam", x, dt(t), taus[0], embs[0], 1)), (name, t)
+            assert np.array_equal(got["det"][k], oracle.hist("det", x, dt(t), taus[0], embs[0], 1)), (name, t)
 
 
 @pytest.mark.gpu
