@@ -101,3 +101,37 @@ def config5(n=1048576, seed=5):
     """Roessler x, N = 1,048,576, tau 1, m 10, Euclidean, 32 thresholds log-spaced 0.01..0.3 std."""
     return dict(x=rossler_x(n, seed=seed), tau=[1], emb=[10], thr=list(log_spaced(0.01, 0.3, 32)), lmin=[2],
                 norms=[1])
+
+
+def config4_series(k, seed=4, nseries=4096):
+    """Series k of the batched configuration: lengths log-uniform in
+    [512, 32768]; a seeded mix of one third AR(1) with phi ~ U(-0.9, 0.9), one
+    third logistic maps with r ~ U(3.5, 4.0), one third Gaussian white noise;
+    fp32.  Each series is drawn from its own seed, so any subset can be made
+    without the others."""
+    rng = np.random.default_rng([seed, k])
+    n = int(np.exp(rng.uniform(np.log(512), np.log(32768))))
+    kind = k % 3
+    if kind == 0:
+        phi, xi, x, v = rng.uniform(-0.9, 0.9), rng.standard_normal(n), np.empty(n), 0.0
+        for t in range(n):
+            v = phi * v + xi[t]
+            x[t] = v
+    elif kind == 1:
+        r, v, x = rng.uniform(3.5, 4.0), rng.uniform(0.1, 0.9), np.empty(n)
+        for t in range(n):
+            x[t] = v
+            v = r * v * (1.0 - v)
+    else:
+        x = rng.standard_normal(n)
+    return x.astype(np.float32)
+
+
+def config4(sample=None, seed=4, nseries=4096):
+    """4,096 fp32 series (config4_series), grid tau in {1, 2, 4} x m in
+    {2, 3, 5} x 8 thresholds {0.05, ..., 0.40} x the series' std, both norms,
+    l_min 2.  sample: indices of the series to make (all by default)."""
+    idx = range(nseries) if sample is None else sample
+    series = [config4_series(k, seed, nseries) for k in idx]
+    return dict(series=series, tau=[1, 2, 4], emb=[2, 3, 5], thr_rel=[0.05 * (i + 1) for i in range(8)],
+                lmin=[2], norms=[1, 2])

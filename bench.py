@@ -298,6 +298,32 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+def config4_loop(accrqa_amd, stride=64):
+    """The batched configuration (4,096 series, tau {1, 2, 4} x m {2, 3, 5} x 8
+    thresholds x both norms) through the public host API, one series at a time
+    (RR, DET and LAM calls per series and norm), on the systematic sample of
+    every stride-th series; there is no batched kernel in this build."""
+    from accrqa_amd import workloads as W
+    c = W.config4(sample=range(0, 4096, stride))
+    pairs = 0
+    t0 = time.perf_counter()
+    for xs in c["series"]:
+        thr = [float(r * xs.std()) for r in c["thr_rel"]]
+        for d in c["norms"]:
+            accrqa_amd.RR(xs, c["tau"], c["emb"], thr, d)
+            accrqa_amd.DET(xs, c["tau"], c["emb"], c["lmin"], thr, d)
+            accrqa_amd.LAM(xs, c["tau"], c["emb"], c["lmin"], thr, d)
+            for tau in c["tau"]:
+                for m in c["emb"]:
+                    nc = xs.size - (m - 1) * tau
+                    pairs += nc * nc * len(thr)
+    dt = time.perf_counter() - t0
+    k = len(c["series"])
+    return {"sample_series": k, "seconds": dt, "series_per_s": k / dt, "pair_evals_per_s": pairs / dt,
+            "full_config_seconds_est": dt * 4096 / k,
+            "path": "per-series loop through accrqa_amd.RR/DET/LAM (host arrays), every %dth series" % stride}
+
+
 def extras(accrqa_amd, L, torch, args, c, x, step):
     """Reported beside the headline, outside its timed region:
     * mfma_decisions: one MFMA-form step with the decision counters on:
@@ -307,7 +333,9 @@ def extras(accrqa_amd, L, torch, args, c, x, step):
       linspace(0.1, 1.6) * sqrt(m)), both forms, outputs compared bit for bit;
     * fp64_input: the m = 16 workload with the series and thresholds in fp64,
       both forms (the MFMA form: FP32 Gram of the rounded series, exact
-      re-decision in FP64), outputs compared bit for bit."""
+      re-decision in FP64), outputs compared bit for bit;
+    * config4_per_series: the batched configuration's throughput through the
+      public API, one series at a time (config4_loop)."""
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream(dev)
     out = {}
@@ -356,6 +384,7 @@ def extras(accrqa_amd, L, torch, args, c, x, step):
     me, ve, oe = timed(*args64, accrqa_amd.FORM_EXACT, np.float64)
     mm, vm, om = timed(*args64, accrqa_amd.FORM_MFMA, np.float64)
     mis = int(sum(int((u.view(np.uint64) != v.view(np.uint64)).sum()) for u, v in zip(oe, om)))
+    out["config4_per_series"] = config4_loop(accrqa_amd)
     out["fp64_input"] = {"exact_ms_per_step": me, "exact_value": ve, "mfma_ms_per_step": mm, "mfma_value": vm,
                          "mfma_vs_exact_speedup": me / mm, "mismatches_mfma_vs_exact": mis,
                          "forms": {"exact": "exact-order FP64 sums on VALU",
