@@ -71,10 +71,10 @@ def tile_store_bytes(nc):
 # form and scaled by the same counters' readings for a 1 GiB device copy
 # (FETCH_SIZE reads 0.5 GiB for it: x2.0; WRITE_SIZE 1 GiB: x1.0).
 MEASURED_TRAFFIC = {
-    ("ar1_m16", "k_lam_mx"): 2.0 * 1.0062e9 + 34.369e9,
-    ("ar1_m16", "k_lam_vscan"): 2.0 * 17.282e9 + 6.805e9,
-    ("ar1_m16", "k_lam_hscan"): 2.0 * 17.272e9 + 6.825e9,
-    ("ar1_m16", "k_det_scan"): 2.0 * 25.247e9 + 5.675e9,
+    ("ar1_m16", "k_lam_mx"): 2.0 * 0.9515e9 + 34.371e9,
+    ("ar1_m16", "k_lam_vscan"): 2.0 * 17.281e9 + 5.266e9,
+    ("ar1_m16", "k_lam_hscan"): 2.0 * 17.271e9 + 5.249e9,
+    ("ar1_m16", "k_det_scan"): 2.0 * 25.399e9 + 3.933e9,
 }
 
 
