@@ -104,9 +104,11 @@ FORM_AUTO, FORM_EXACT, FORM_MFMA = 0, 1, 2
 
 
 def set_form(form):
-    """Euclidean distance form: FORM_AUTO (MFMA Gram + exact recheck for fp32
-    input with m >= 8), FORM_EXACT (exact VALU sums only) or FORM_MFMA (MFMA
-    form wherever it applies).  Results are identical; only speed differs."""
+    """Euclidean distance form: FORM_AUTO (MFMA Gram + exact recheck for
+    m >= 8), FORM_EXACT (exact VALU sums only) or FORM_MFMA (MFMA form wherever
+    it applies: fp32 input, and fp64 input in the LAM pass with an fp32 Gram of
+    the rounded series and an fp64 recheck).  Results are identical; only
+    speed differs."""
     e = load().accrqa_amd_set_form(int(form))
     if e:
         raise AccrqaError(e, f"form {form}")

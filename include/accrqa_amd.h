@@ -111,13 +111,14 @@ int accrqa_amd_rqa_device_f64(const double *d_input, size_t data_size, int tau, 
                               int distance_type, double *d_rr, double *d_det, double *d_lam, void *stream);
 
 /* Form of the Euclidean distance (no counterpart in the reference, which has
- * one form per platform).  0 = automatic: the MFMA form for fp32 input with
- * emb >= 8, exact VALU sums otherwise; 1 = exact VALU sums only; 2 = MFMA form
- * wherever it applies (fp32 input, Euclidean norm).  The MFMA form computes
+ * one form per platform).  0 = automatic: the MFMA form for emb >= 8, exact
+ * VALU sums otherwise; 1 = exact VALU sums only; 2 = MFMA form wherever it
+ * applies (Euclidean norm; fp32 input, and fp64 input in the LAM pass, whose
+ * Gram then runs on the series rounded to fp32).  The MFMA form computes
  * |x_i|^2 + |x_j|^2 - 2 x_i.x_j with the Gram tile on matrix cores and
  * re-decides every pair within a rigorous error bound of a threshold in the
- * reference's exact order, so results are identical for every form.
- * Process-wide; returns 0, or 2 for an unknown form. */
+ * reference's exact order and precision, so results are identical for every
+ * form.  Process-wide; returns 0, or 2 for an unknown form. */
 int accrqa_amd_set_form(int form);
 
 /* LAM/DET work split (no counterpart in the reference).  0 = automatic: when
