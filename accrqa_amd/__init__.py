@@ -94,6 +94,8 @@ def load():
     L.accrqa_amd_mx_stats.restype = I
     L.accrqa_amd_mx_stats_read.argtypes = [C.c_void_p]
     L.accrqa_amd_mx_stats_read.restype = I
+    L.accrqa_amd_mfma16_probe.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.accrqa_amd_mfma16_probe.restype = I
     L.accrqa_amd_last_error.restype = C.c_char_p
     L.accrqa_amd_version.restype = C.c_char_p
     _lib = L
@@ -106,8 +108,9 @@ FORM_AUTO, FORM_EXACT, FORM_MFMA = 0, 1, 2
 def set_form(form):
     """Euclidean distance form: FORM_AUTO (MFMA Gram + exact recheck for
     m >= 8), FORM_EXACT (exact VALU sums only) or FORM_MFMA (MFMA form wherever
-    it applies: fp32 input, and fp64 input in the LAM pass with an fp32 Gram of
-    the rounded series and an fp64 recheck).  Results are identical; only
+    it applies: fp32 input, and fp64 input in the LAM pass with the Gram of
+    the series rounded to fp32 and an fp64 recheck).  The Gram runs on FP16
+    matrix cores with each sample split into two halves (hi/lo).  Results are identical; only
     speed differs."""
     e = load().accrqa_amd_set_form(int(form))
     if e:
@@ -141,6 +144,21 @@ def mx_stats_read():
     if e:
         raise AccrqaError(e, "mx_stats_read")
     return int(c[0]), int(c[1]), int(c[2])
+
+
+def mfma16_probe(a, b, c):
+    """c + a @ b by one matrix-core instruction of the MFMA form's Gram
+    (diagnostic): a float16 [32, 16], b float16 [16, 32], c float32 [32, 32]."""
+    a = np.ascontiguousarray(a, dtype=np.float16)
+    b = np.ascontiguousarray(b, dtype=np.float16)
+    c = np.ascontiguousarray(c, dtype=np.float32)
+    if a.shape != (32, 16) or b.shape != (16, 32) or c.shape != (32, 32):
+        raise ValueError("mfma16_probe: shapes (32,16), (16,32), (32,32)")
+    d = np.empty((32, 32), dtype=np.float32)
+    e = load().accrqa_amd_mfma16_probe(a.ctypes.data, b.ctypes.data, c.ctypes.data, d.ctypes.data)
+    if e:
+        raise AccrqaError(e, "mfma16_probe")
+    return d
 
 
 def _dist(d):

@@ -808,12 +808,22 @@ __global__ __launch_bounds__(WG) void k_rr(const F *__restrict__ x, int64_t nc, 
 
 // ================================================================ MFMA form
 // Euclidean norm, fp32 input.  The squared distance is approximated as
-//   s~ = (n_i + n_j) - 2 g_ij,   g = x_i . x_j on MFMA (v_mfma_f32_32x32x2f32),
-// n_i = |x_i|^2 (accumulated in fp64, rounded to fp32).  For the reference's
-// exact-order fp32 sum s:
-//   |s - s~| <= (4m + 16) * 2^-24 * (n_i + n_j) + 1e-30 =: E_ij
-// (exact-order sum: gamma_{m+1} * 2(n_i+n_j); rounded norms 2u; the fp32 dot
-// product gamma_m (n_i+n_j)/2 doubled; the final add and fma 3u; 10u spare).
+//   s~ = (n_i + n_j) - 2 g_ij,   g = x_i . x_j on MFMA,
+// n_i = |x_i|^2 (accumulated in fp64, rounded to fp32).  The Gram runs in
+// FP16 hi/lo form (v_mfma_f32_32x32x16_f16): each sample, scaled by 2^e so
+// that |a| < 2^14, is split as a = h + l + r with h = fp16(a), l = fp16(a - h)
+// (both round to nearest, subnormals kept), |r| <= 2^-22 |a| + 2^-25; the
+// products h.l, l.h and then h.h are accumulated, l.l is dropped.  With
+// u = 2^-24, C = ceil(m/16) and A, B the scaled norms, in units of s~:
+//  * split: 2 sum_k (|l_a l_b| + |r_a b| + |a r_b| + |r_a r_b|)
+//      <= 14.04u (A + B) + m 2^-25.9, using sqrt(mA) <= 2A + m/8;
+//  * accumulation, allowing each MFMA up to 16 additions of relative error
+//    2u (a faithful adder, any order; h.h of every chunk after all small
+//    products): 2 * 32.2C u (A + B) / 2.
+// For the reference's exact-order fp32 sum s:
+//   |s - s~| <= (3m + 31 + 33C) u (n_i + n_j) + m 2^-25 2^-2e + 1e-30 =: E_ij
+// (exact-order sum: gamma_{m+1} * 2(n_i+n_j); rounded norms 2u; the Gram as
+// above; the final add and fma 3u; the rest spare).
 // A byte table over the bits of s~ gives the rank r = #{t : B_t < s} exactly
 // for every bucket that has no bound within E_max = max E_ij of its value
 // range; a bucket with one bound within reach names that bound, and its pairs
@@ -831,8 +841,35 @@ struct MxParams {
     int sh;          // bucket width in key bits (log2)
     int nb;          // buckets used (<= NBX)
     int qlo;         // base >> sh: bucket 0's number on the key >> sh scale
-    float cE;        // (4m + 16) * 2^-24
+    float cE;        // (3m + 31 + 33 ceil(m/16)) * 2^-24 (+ 4 * 2^-24 for fp64 input)
+    float dA;        // m * 2^-25: the additive part of E_ij in scaled units (x 2^-2e)
 };
+
+// Scale of the FP16 hi/lo Gram, from the largest norm: samples enter the
+// windows as a = x 2^e with |a| < 2^14 (every sample is a coordinate of some
+// row, so |x| <= sqrt(n_max)); the Gram comes back through c2 = -2^(1-2e) in
+// the fma that forms s~ (exact: a power of two); dE = m 2^-25 2^-2e + 1e-30.
+// Norms that are zero, not finite or above 1e30 keep e = 0 (the last two
+// flag every bucket anyway).
+struct MxScale {
+    float s, c2, dE;
+};
+
+__device__ __forceinline__ MxScale mx_scale(const unsigned int *__restrict__ nmax_bits, const MxParams &P) {
+    const uint32_t nb = *nmax_bits;
+    const float nmax = __uint_as_float(nb);
+    int e = 0;
+    if (nmax > 0.0f && nmax <= 1e30f) {
+        const int q = (int)(nb >> 23) - 126;   // n_max < 2^q
+        e = 14 - ((q + 1) >> 1);               // sqrt(n_max) 2^e < 2^14
+        e = e < -60 ? -60 : (e > 60 ? 60 : e);
+    }
+    MxScale Z;
+    Z.s = __int_as_float((127 + e) << 23);
+    Z.c2 = -__int_as_float((128 - 2 * e) << 23);
+    Z.dE = fmaf(P.dA, __int_as_float((127 - 2 * e) << 23), 1e-30f);
+    return Z;
+}
 
 // n_i = sum_k x[i + k tau]^2 (fp64, rounded to fp32); entries nc .. nc+127 = 0.
 // nmax_bits: atomic max of the float bits (non-negative floats order as uints;
@@ -867,8 +904,9 @@ __global__ void k_mx_lut(Bounds<float, TG> S, int nt, MxParams P, const unsigned
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= P.nb) return;
     const float nmax = __uint_as_float(*nmax_bits);
+    const MxScale Z = mx_scale(nmax_bits, P);
     float emax = P.cE * 2.0f * nmax;
-    emax = emax * 1.0001f + 1e-30f;
+    emax = (emax + Z.dE) * 1.0001f;
     if (!(nmax <= 1e30f)) emax = __int_as_float(0x7F800000);
     const uint32_t k0 = P.base + ((uint32_t)b << P.sh);
     const uint64_t k1l = (uint64_t)P.base + ((uint64_t)(b + 1) << P.sh);
@@ -910,10 +948,10 @@ __device__ __forceinline__ uint32_t mx_key_bucket(float st, const MxParams &P) {
 // bound within reach decides unless it lies within E_ij of s~, or several
 // bounds were within reach; amb is set for those pairs.
 __device__ __forceinline__ uint32_t mx_second(uint32_t byte, float st, float sum, const float *bnd, float cE,
-                                              bool &amb) {
+                                              float dE, bool &amb) {
     const uint32_t idx = byte & 0x3Fu;
     const float d = st - bnd[idx];
-    const float E = fmaf(cE, sum, 1e-30f) * 1.0001f;
+    const float E = fmaf(cE, sum, dE) * 1.0001f;
     amb = (byte & 0x40u) != 0u || !(fabsf(d) > E);
     return idx + (d > 0.0f ? 1u : 0u);
 }
@@ -958,12 +996,24 @@ __device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, uint32_t &total) 
 
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
-// Gram tile G[a][b] = x_(i0+a) . x_(c0+b), a, b < 32, on MFMA.  rw / cw hold
-// x[i0 ..] and x[c0 ..] (LDS windows); lane l supplies A[row l%32][k l/32]
-// and B[k l/32][col l%32], i.e. dimension 2 k2 + h.  For odd m the last step
-// of half 1 reads dimension m-1 again and zeroes its row operand.
-// Per-wave operand windows of the MFMA kernels: rw holds the series values
-// of the block's rows, cw those of its NT column tiles (tile p at +32).  Both
+typedef _Float16 halfx8 __attribute__((ext_vector_type(8)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// One window word: the scaled sample a split as h = fp16(a) (bits 0..15) and
+// l = fp16(a - h) (bits 16..31); a - h is exact in fp32.
+__device__ __forceinline__ float hl_word(float a) {
+    const _Float16 hi = (_Float16)a;
+    const _Float16 lo = (_Float16)(a - (float)hi);
+    return __uint_as_float((uint32_t)__builtin_bit_cast(uint16_t, hi) |
+                           ((uint32_t)__builtin_bit_cast(uint16_t, lo) << 16));
+}
+
+// Gram tile G[a][b] = x_(i0+a) . x_(c0+b), a, b < 32, on MFMA.  Lane l
+// supplies A[row l%32][k] and B[k][col l%32] for the eight dimensions
+// k = 8(l/32) + t, t < 8, of each chunk of 16 (slot t; A and B take the same
+// slots, so the sum over k does not depend on the instruction's slot order).
+// Per-wave operand windows of the MFMA kernels: rw holds the window words of
+// the block's rows, cw those of its NT column tiles (tile p at +32).  Both
 // are read at one per-lane offset o (cw tile p at o + 32p), so one LDS
 // address per operand step serves the row operand and every column tile,
 // the tiles being immediate offsets from it.  Slot ZS of rw and slots
@@ -972,7 +1022,7 @@ typedef float floatx16 __attribute__((ext_vector_type(16)));
 template <int NT>
 struct MxWin {
     static constexpr int ZS = MXW + 32 * (NT - 1);   // past the content of both windows
-    static constexpr int CWO = ZS + 1;                // cw[0] relative to rw[0], in floats
+    static constexpr int CWO = ZS + 1;                // cw[0] relative to rw[0], in words
     float rw[ZS + 1];
     float cw[ZS + 32 * (NT - 1) + 1];
 };
@@ -983,47 +1033,90 @@ __device__ __forceinline__ void mx_win_init(MxWin<NT> &W, int lane) {
     if (lane < NT) W.cw[MxWin<NT>::ZS + 32 * lane] = 0.f;
 }
 
-// Operand k of the Gram tiles: lane (col, h) supplies dimension k = 2u + h of
-// step u of a chunk, at window offset col + k*tau (the zero slot for k >= m).
-// The first chunk's eight addresses depend only on the lane, m and tau.
+// Window address of dimension k for lane (col, h): col + k*tau, or the zero
+// slot for k >= m.  The first chunk's eight (k = 8h + t) depend only on the
+// lane, m and tau.
 template <int NT>
-__device__ __forceinline__ void gram_addrs(const MxWin<NT> &W, int col, int h, int m, int tau, const float *(&pa)[8]) {
-    const float *wb = W.rw;
-#pragma unroll
-    for (int u = 0; u < 8; u++) {
-        const int k = 2 * u + h;
-        pa[u] = wb + (k < m ? col + k * tau : MxWin<NT>::ZS);
-    }
+__device__ __forceinline__ const float *gram_addr(const MxWin<NT> &W, int col, int k, int m, int tau) {
+    return W.rw + (k < m ? col + k * tau : MxWin<NT>::ZS);
 }
 
-// One 32x32 Gram tile (column tile p): G[row][32p + col] over the m dimensions.
+template <int NT>
+__device__ __forceinline__ void gram_addrs(const MxWin<NT> &W, int col, int h, int m, int tau, const float *(&pa)[8]) {
+#pragma unroll
+    for (int t = 0; t < 8; t++) pa[t] = gram_addr(W, col, 8 * h + t, m, tau);
+}
+
+// The h halves (sel 0x05040100) or l halves (0x07060302) of eight window
+// words, as the eight fp16 operands of one MFMA.
+template <uint32_t SEL>
+__device__ __forceinline__ halfx8 hl_halves(const uint32_t (&w)[8]) {
+    u32x4 v;
+#pragma unroll
+    for (int i = 0; i < 4; i++) v[i] = __builtin_amdgcn_perm(w[2 * i + 1], w[2 * i], SEL);
+    return __builtin_bit_cast(halfx8, v);
+}
+
+__device__ __forceinline__ floatx16 mfma16(halfx8 a, halfx8 b, floatx16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+}
+
+// One MFMA of the Gram's instruction on given operands (diagnostic, one
+// wave): a = A[32][16], b = B[16][32] as fp16 bits, row-major; c = the
+// accumulator C[32][32] in, d = C + A B out.  Lanes supply the operands as
+// gram_tile does (dimension 8h + t in slot t) and hold C in its layout.
+__global__ void k_mfma16_probe(const uint16_t *__restrict__ a, const uint16_t *__restrict__ b,
+                               const float *__restrict__ c, float *__restrict__ d) {
+    const int lane = threadIdx.x & (WAVE - 1), col = lane & 31, h = lane >> 5;
+    halfx8 av, bv;
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+        av[t] = __builtin_bit_cast(_Float16, a[col * 16 + 8 * h + t]);
+        bv[t] = __builtin_bit_cast(_Float16, b[(8 * h + t) * 32 + col]);
+    }
+    floatx16 acc;
+#pragma unroll
+    for (int e = 0; e < 16; e++) acc[e] = c[(8 * (e >> 2) + 4 * h + (e & 3)) * 32 + col];
+    acc = mfma16(av, bv, acc);
+#pragma unroll
+    for (int e = 0; e < 16; e++) d[(8 * (e >> 2) + 4 * h + (e & 3)) * 32 + col] = acc[e];
+}
+
+// One 32x32 Gram tile (column tile p) in scaled units: G[row][32p + col]
+// over the m dimensions.  Order of accumulation: l.h and h.l of every chunk,
+// then h.h of every chunk (the bound on the adds is taken in that order).
 template <int NT, int p>
 __device__ __forceinline__ floatx16 gram_tile(const MxWin<NT> &W, const float *const (&pa)[8], int col, int h, int m,
                                               int tau) {
     constexpr int CW = MxWin<NT>::CWO + 32 * p;
+    constexpr uint32_t SH = 0x05040100u, SL = 0x07060302u;
     floatx16 acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    {
-        float a[8], b[8];
+    uint32_t a[8], b[8];
+    // words of chunk k0 (the first chunk's addresses are pa)
+    auto load = [&](int k0) {
 #pragma unroll
-        for (int u = 0; u < 8; u++) {
-            a[u] = pa[u][0];
-            b[u] = pa[u][CW];
+        for (int t = 0; t < 8; t++) {
+            const float *q = k0 == 0 ? pa[t] : gram_addr(W, col, k0 + 8 * h + t, m, tau);
+            a[t] = __float_as_uint(q[0]);
+            b[t] = __float_as_uint(q[CW]);
         }
-#pragma unroll
-        for (int u = 0; u < 8; u++) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[u], b[u], acc, 0, 0, 0);
+    };
+    if (m <= 16) {
+        load(0);
+        const halfx8 ah = hl_halves<SH>(a), bh = hl_halves<SH>(b);
+        acc = mfma16(hl_halves<SL>(a), bh, acc);
+        acc = mfma16(ah, hl_halves<SL>(b), acc);
+        return mfma16(ah, bh, acc);
     }
-    // dimensions 16 and up (m > 16), addresses formed per chunk
-    for (int k0 = 16; k0 < m; k0 += 16) {
-        float a[8], b[8];
-#pragma unroll
-        for (int u = 0; u < 8; u++) {
-            const int k = k0 + 2 * u + h;
-            const float *q = W.rw + (k < m ? col + k * tau : MxWin<NT>::ZS);
-            a[u] = q[0];
-            b[u] = q[CW];
-        }
-#pragma unroll
-        for (int u = 0; u < 8; u++) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[u], b[u], acc, 0, 0, 0);
+    // more chunks: the small products of all, then h.h chunk by chunk
+    for (int k0 = 0; k0 < m; k0 += 16) {
+        load(k0);
+        acc = mfma16(hl_halves<SL>(a), hl_halves<SH>(b), acc);
+        acc = mfma16(hl_halves<SH>(a), hl_halves<SL>(b), acc);
+    }
+    for (int k0 = 0; k0 < m; k0 += 16) {
+        load(k0);
+        acc = mfma16(hl_halves<SH>(a), hl_halves<SH>(b), acc);
     }
     return acc;
 }
@@ -1036,13 +1129,13 @@ __device__ __forceinline__ floatx16 gram_tile(const MxWin<NT> &W, const float *c
 // re-decides in exact order.  ni[e]: norm of element e's row, nj: norm of the
 // lane's column.
 __device__ __forceinline__ void tile_ranks(const floatx16 &acc, const float (&ni)[16], float nj, const uint8_t *lut_s,
-                                           const MxParams &P, const float *bnd_s, uint32_t (&Rq)[4], uint32_t &fm,
-                                           uint32_t &flw) {
+                                           const MxParams &P, const MxScale &Z, const float *bnd_s, uint32_t (&Rq)[4],
+                                           uint32_t &fm, uint32_t &flw) {
 #pragma unroll
     for (int c = 0; c < 4; c++) Rq[c] = 0u;
 #pragma unroll
     for (int e = 0; e < 16; e++) {
-        const float st = fmaf(-2.0f, acc[e], ni[e] + nj);
+        const float st = fmaf(Z.c2, acc[e], ni[e] + nj);
         Rq[e & 3] |= (uint32_t)lut_s[mx_key_bucket(st, P)] << (8 * (e >> 2));
     }
     fm = 0u;
@@ -1060,9 +1153,9 @@ __device__ __forceinline__ void tile_ranks(const floatx16 &acc, const float (&ni
             if (W & (1u << (sh + c))) {
                 const uint32_t byte = (Rq[c] >> sh) & 0xFFu;
                 const float sum = ni[e] + nj;
-                const float st = fmaxf(fmaf(-2.0f, acc[e], sum), 0.0f);
+                const float st = fmaxf(fmaf(Z.c2, acc[e], sum), 0.0f);
                 bool amb;
-                const uint32_t rk = mx_second(byte, st, sum, bnd_s, P.cE, amb);
+                const uint32_t rk = mx_second(byte, st, sum, bnd_s, P.cE, Z.dE, amb);
                 if (byte & 0x80u) {
                     Rq[c] = (Rq[c] & ~(0xFFu << sh)) | (rk << sh);
                     if (amb) fm |= 1u << e;
@@ -1096,7 +1189,8 @@ __device__ __forceinline__ void tile_row_norms(const float *__restrict__ nrm, in
 // (diagnostic instantiation): stats[0..2] += elements ranked, elements the
 // table flagged, elements re-decided in exact order.
 //
-// F = double (fp64 input): the windows hold the series rounded to fp32, the
+// F = double (fp64 input): the windows hold the series rounded to fp32 (then
+// scaled and split as for fp32 input, the Gram's bound taken on them), the
 // norms are fp64 sums rounded to fp32, and the exact re-decision sums the
 // fp64 series in the reference's order against the fp64 bounds SX.  Against
 // the reference's fp64 s: each rounded product adds at most 2u|x_i x_j|
@@ -1104,10 +1198,10 @@ __device__ __forceinline__ void tile_row_norms(const float *__restrict__ nrm, in
 // s~ after doubling; the reference's own fp64 error is below u (n_i + n_j);
 // the table's and the second stage's bounds are the fp64 bounds rounded to
 // fp32, off by at most 2u (n_i + n_j) near any s~ that reaches them.  The
-// host adds 4u to the fp32 coefficient, (4m + 20) u (n_i + n_j), which
-// covers all of it.
+// host adds 4u to the fp32 coefficient cE, which covers all of it.
 template <typename F, int TG, bool SCAN, bool STATS = false>
-__global__ __launch_bounds__(WG) RQA_WPE_MX(SCAN) void k_lam_mx(const F *__restrict__ x, const float *__restrict__ nrm, int64_t nc,
+__global__ __launch_bounds__(WG) RQA_WPE_MX(SCAN) void k_lam_mx(const F *__restrict__ x, const float *__restrict__ nrm,
+                                               const unsigned int *__restrict__ nmax_bits, int64_t nc,
                                                int tau, int m, Bounds<float, TG> S, Bounds<F, TG> SX, int nt,
                                                const uint8_t *__restrict__ lut, MxParams P, int seg_len, LamWork W,
                                                unsigned long long *__restrict__ hist, uint2 *__restrict__ carry,
@@ -1123,7 +1217,7 @@ __global__ __launch_bounds__(WG) RQA_WPE_MX(SCAN) void k_lam_mx(const F *__restr
     // rounding inside the error bound cE)
     __shared__ F bndx_s[sizeof(F) == 8 ? NBND : 1];
     // rw: x[i0 .. i0 + 31 + (m-1)tau] of the current block; cw: x[j0 .. j0 + 63 + (m-1)tau] of the item,
-    // both rounded to fp32
+    // both rounded to fp32, scaled and split into window words
     __shared__ MxWin<2> win[WAVES];
     if constexpr (SCAN)
         for (int e = threadIdx.x; e < TG * LCAP; e += WG) lh[e] = 0;
@@ -1155,6 +1249,7 @@ __global__ __launch_bounds__(WG) RQA_WPE_MX(SCAN) void k_lam_mx(const F *__restr
     mx_win_init(Wn, lane);
     const float *pa[8];
     gram_addrs(Wn, col, h, m, tau, pa);
+    const MxScale Z = mx_scale(nmax_bits, P);
 
     for (int64_t item = (int64_t)blockIdx.x * WAVES + wave; item < nitems; item += (int64_t)gridDim.x * WAVES) {
         int64_t band;
@@ -1180,15 +1275,15 @@ __global__ __launch_bounds__(WG) RQA_WPE_MX(SCAN) void k_lam_mx(const F *__restr
         for (int t = 0; t < TG; t++) run[t] = jv ? PRE : 0u;
 
         // column window once per item; row window of the first block
-        for (int e = lane; e < cspan; e += WAVE) cw[e] = x[j0 + e];
+        for (int e = lane; e < cspan; e += WAVE) cw[e] = hl_word((float)x[j0 + e] * Z.s);
         float pre0 = lane < rspan ? x[ib + lane] : 0.f;
         float pre1 = lane + WAVE < rspan ? x[ib + WAVE + lane] : 0.f;
 
         for (int64_t i0 = ib; i0 < ie; i0 += BLK) {
             const int nb = ie - i0 < BLK ? (int)(ie - i0) : BLK;
             wave_sync();
-            if (lane < rspan) rw[lane] = pre0;
-            if (lane + WAVE < rspan) rw[WAVE + lane] = pre1;
+            if (lane < rspan) rw[lane] = hl_word(pre0 * Z.s);
+            if (lane + WAVE < rspan) rw[WAVE + lane] = hl_word(pre1 * Z.s);
             wave_sync();
             // next block's row window, in flight while this block computes
             // (only if there is a next block: its window ends by x[N + 30])
@@ -1206,7 +1301,7 @@ __global__ __launch_bounds__(WG) RQA_WPE_MX(SCAN) void k_lam_mx(const F *__restr
             {
                 const floatx16 acc = gram_tile<2, 0>(Wn, pa, col, h, m, tau);
                 uint32_t Rq[4], f;
-                tile_ranks(acc, ni, nrm[j0 + col], lut_s, P, bnd_s, Rq, f, flw0);
+                tile_ranks(acc, ni, nrm[j0 + col], lut_s, P, Z, bnd_s, Rq, f, flw0);
 #pragma unroll
                 for (int c = 0; c < 4; c++) Rr[c] = Rq[c];
                 if (j0 + col < nc) fm |= f & rowm;
@@ -1214,7 +1309,7 @@ __global__ __launch_bounds__(WG) RQA_WPE_MX(SCAN) void k_lam_mx(const F *__restr
             {
                 const floatx16 acc = gram_tile<2, 1>(Wn, pa, col, h, m, tau);
                 uint32_t Rq[4], f;
-                tile_ranks(acc, ni, nrm[j0 + 32 + col], lut_s, P, bnd_s, Rq, f, flw1);
+                tile_ranks(acc, ni, nrm[j0 + 32 + col], lut_s, P, Z, bnd_s, Rq, f, flw1);
 #pragma unroll
                 for (int c = 0; c < 4; c++) Rr[4 + c] = Rq[c];
                 if (j0 + 32 + col < nc) fm |= (f & rowm) << 16;
@@ -1354,7 +1449,8 @@ __global__ __launch_bounds__(WG) void k_lam_vscan(const uint8_t *__restrict__ ti
 constexpr int SKB = 36;   // bytes per diagonal row (9 words: conflict-free stores and loads)
 
 template <int TG>
-__global__ __launch_bounds__(WG) RQA_WPE_MX(true) void k_det_mx(const float *__restrict__ x, const float *__restrict__ nrm, int64_t nc,
+__global__ __launch_bounds__(WG) RQA_WPE_MX(true) void k_det_mx(const float *__restrict__ x, const float *__restrict__ nrm,
+                                               const unsigned int *__restrict__ nmax_bits, int64_t nc,
                                                int tau, int m, Bounds<float, TG> S, int nt,
                                                const uint8_t *__restrict__ lut, MxParams P, int seg_len, int nseg,
                                                const int2 *__restrict__ items, int64_t nitems,
@@ -1385,6 +1481,7 @@ __global__ __launch_bounds__(WG) RQA_WPE_MX(true) void k_det_mx(const float *__r
     mx_win_init(Wn, lane);
     const float *pa[8];
     gram_addrs(Wn, col, h, m, tau, pa);
+    const MxScale Z = mx_scale(nmax_bits, P);
     const size_t cstride = (size_t)nseg * (nc - 1);
     const int rspan = 32 + (m - 1) * tau, cspan = 96 + (m - 1) * tau;   // <= MXW, MXW + 64
 
@@ -1417,10 +1514,10 @@ __global__ __launch_bounds__(WG) RQA_WPE_MX(true) void k_det_mx(const float *__r
             wave_sync();
 #pragma unroll
             for (int c = 0; c < 2; c++)
-                if (lane + c * WAVE < rspan) rw[lane + c * WAVE] = pr[c];
+                if (lane + c * WAVE < rspan) rw[lane + c * WAVE] = hl_word(pr[c] * Z.s);
 #pragma unroll
             for (int c = 0; c < 3; c++)
-                if (lane + c * WAVE < cspan) cw[lane + c * WAVE] = pc[c];
+                if (lane + c * WAVE < cspan) cw[lane + c * WAVE] = hl_word(pc[c] * Z.s);
             wave_sync();
             // next block's windows, in flight while this block computes (only
             // if there is a next block: its windows end by x[N + 94])
@@ -1447,7 +1544,7 @@ __global__ __launch_bounds__(WG) RQA_WPE_MX(true) void k_det_mx(const float *__r
                                      : p == 1 ? gram_tile<3, 1>(Wn, pa, col, h, m, tau)
                                               : gram_tile<3, 2>(Wn, pa, col, h, m, tau);
                 uint32_t Rq[4], fm, flw;
-                tile_ranks(acc, ni, nrm[c0 + 32 * p + col], lut_s, P, bnd_s, Rq, fm, flw);
+                tile_ranks(acc, ni, nrm[c0 + 32 * p + col], lut_s, P, Z, bnd_s, Rq, fm, flw);
                 uint32_t need = 0;
 #pragma unroll
                 for (int e = 0; e < 16; e++) {

@@ -51,19 +51,35 @@ struct Ctx {
     int device = -1;
     int ncu = 0;
     hipStream_t stream = nullptr;
-    DevBuf x, hist, carry, metric, scan, items_rr, items_det, items_dets, items_lamv, items_lamh, counts, perm, lmins,
-        lut, nrm, nmax, lut8, out_rr, out_det, out_lam, priv, tiles, dump, tops, mxstats;
+    DevBuf x, hist, carry, metric, scan, counts, perm, lmins, lut, nrm, nmax, lut8, out_rr, out_det, out_lam, priv,
+        tiles, dump, tops, mxstats, probe;
+    // work-item tables of the current (nc, seg_len): views of an ItemSet
+    // entry (owned there, never freed through these)
+    DevBuf items_rr, items_det, items_dets, items_lamv, items_lamh;
     // Raw-result sinks of accrqa_amd_histograms_*: when set, launch_group
     // copies each group's counts / h_D rows / h_V rows there, indexed by the
     // caller's threshold order.  Null for every other entry point.
     unsigned long long *dump_rr = nullptr, *dump_det = nullptr, *dump_lam = nullptr;
-    int64_t items_nc = -1;     // (nc, seg_len) the item tables were built for
-    int items_L = -1;
     int64_t n_items_rr = 0, n_items_det = 0, n_items_dets = 0, n_items_lamv = 0, n_items_lamh = 0;
 };
 
 std::mutex g_mu;
 Ctx g_ctx;
+
+// Work-item tables kept for the last few (nc, seg_len): a call with several
+// (tau, m) blocks, or a loop over many series of a few lengths, alternates
+// between them, and rebuilding costs a host pass, uploads and a stream
+// synchronisation.  Least recently used entry replaced.
+struct ItemSet {
+    int64_t nc = -1;
+    int L = -1;
+    uint64_t used = 0;
+    DevBuf rr, det, dets, lamv, lamh;
+    int64_t n_rr = 0, n_det = 0, n_dets = 0, n_lamv = 0, n_lamh = 0;
+};
+constexpr int NITEMSETS = 8;
+ItemSet g_items[NITEMSETS];
+uint64_t g_items_clock = 0;
 
 // Euclidean distance form: 0 = automatic (MFMA Gram + exact recheck for fp32
 // input with m >= 8, exact VALU sums otherwise), 1 = exact VALU sums only,
@@ -78,8 +94,8 @@ int g_split = 0;
 // order into the context's mxstats buffer (3 u64).
 bool g_mxstats = false;
 
-// MFMA form for F = float and, in the LAM pass, F = double (fp32 Gram of the
-// rounded series, exact re-decision in fp64); the direct DET kernel is fp32-only.
+// MFMA form for F = float and, in the LAM pass, F = double (hi/lo Gram of the
+// series rounded to fp32, exact re-decision in fp64); the direct DET kernel is fp32-only.
 template <typename F>
 bool use_mfma(int dist, int m, int tau) {
     if (dist != 1) return false;
@@ -107,6 +123,8 @@ int ensure(DevBuf &b, size_t bytes) {
     if (bytes == 0) bytes = 256;
     if (b.cap >= bytes) return E_OK;
     if (b.p) {
+        // kernels queued by earlier blocks of the same call may still read it
+        if (g_ctx.stream) HIP_TRY(hipStreamSynchronize(g_ctx.stream));
         hipError_t e = hipFree(b.p);
         b.p = nullptr;
         b.cap = 0;
@@ -132,17 +150,21 @@ int init_ctx() {
             int cur = dev;
             (void)hipSetDevice(g_ctx.device);
             DevBuf *all[] = {&g_ctx.x,       &g_ctx.hist,       &g_ctx.carry,      &g_ctx.metric,
-                             &g_ctx.scan,    &g_ctx.items_rr,   &g_ctx.items_det,  &g_ctx.items_lamv,
-                             &g_ctx.items_lamh, &g_ctx.counts,  &g_ctx.perm,       &g_ctx.lmins,
+                             &g_ctx.scan,    &g_ctx.counts,     &g_ctx.perm,       &g_ctx.lmins,
                              &g_ctx.lut,     &g_ctx.nrm,        &g_ctx.nmax,       &g_ctx.lut8,
                              &g_ctx.out_rr,  &g_ctx.out_det,    &g_ctx.out_lam,    &g_ctx.priv,
-                             &g_ctx.tiles,   &g_ctx.dump,       &g_ctx.items_dets, &g_ctx.tops,
-                             &g_ctx.mxstats};
-            g_ctx.items_nc = -1;
+                             &g_ctx.tiles,   &g_ctx.dump,       &g_ctx.tops,
+                             &g_ctx.mxstats, &g_ctx.probe};
             for (DevBuf *b : all) {
                 if (b->p) (void)hipFree(b->p);
                 *b = DevBuf();
             }
+            for (ItemSet &s : g_items) {
+                for (DevBuf *b : {&s.rr, &s.det, &s.dets, &s.lamv, &s.lamh})
+                    if (b->p) (void)hipFree(b->p);
+                s = ItemSet();
+            }
+            g_ctx.items_rr = g_ctx.items_det = g_ctx.items_dets = g_ctx.items_lamv = g_ctx.items_lamh = DevBuf();
             if (g_ctx.stream) (void)hipStreamDestroy(g_ctx.stream);
             g_ctx.stream = nullptr;
             (void)hipSetDevice(cur);
@@ -441,10 +463,32 @@ unsigned stitch_blocks(int64_t nseq) {
 // Work-item tables (uploaded once per (N_c, segment length)):
 //   k_rr : (band b, row segment q) over rows 0 .. min(64(b+1), N_c) - 1
 //   k_det: (strip b, step segment q) over steps 0 .. N_c - (1 + 64b) - 1
-int build_items(int64_t nc, hipStream_t st) {
+void select_items(const ItemSet &s) {
     Ctx &c = g_ctx;
+    c.items_rr = s.rr;
+    c.items_det = s.det;
+    c.items_dets = s.dets;
+    c.items_lamv = s.lamv;
+    c.items_lamh = s.lamh;
+    c.n_items_rr = s.n_rr;
+    c.n_items_det = s.n_det;
+    c.n_items_dets = s.n_dets;
+    c.n_items_lamv = s.n_lamv;
+    c.n_items_lamh = s.n_lamh;
+}
+
+int build_items(int64_t nc, hipStream_t st) {
     const int L = seg_len_for(nc);
-    if (c.items_nc == nc && c.items_L == L) return E_OK;
+    ItemSet *slot = &g_items[0];
+    for (ItemSet &s : g_items) {
+        if (s.nc == nc && s.L == L) {
+            s.used = ++g_items_clock;
+            select_items(s);
+            return E_OK;
+        }
+        if (s.used < slot->used) slot = &s;
+    }
+    ItemSet &S = *slot;
     // (item, steps) lists; each table is sorted by decreasing size so that the
     // persistent waves take the long items first
     std::vector<std::pair<int64_t, int2>> rr, det, dets, lamv, lamh;
@@ -475,14 +519,18 @@ int build_items(int64_t nc, hipStream_t st) {
         n = (int64_t)it.size();
         return E_OK;
     };
+    // the entry replaced may still be read by queued kernels
     HIP_TRY(hipStreamSynchronize(st));
-    if ((e = upload(rr, c.items_rr, c.n_items_rr))) return e;
-    if ((e = upload(det, c.items_det, c.n_items_det))) return e;
-    if ((e = upload(dets, c.items_dets, c.n_items_dets))) return e;
-    if ((e = upload(lamv, c.items_lamv, c.n_items_lamv))) return e;
-    if ((e = upload(lamh, c.items_lamh, c.n_items_lamh))) return e;
-    c.items_nc = nc;
-    c.items_L = L;
+    S.nc = -1;
+    if ((e = upload(rr, S.rr, S.n_rr))) return e;
+    if ((e = upload(det, S.det, S.n_det))) return e;
+    if ((e = upload(dets, S.dets, S.n_dets))) return e;
+    if ((e = upload(lamv, S.lamv, S.n_lamv))) return e;
+    if ((e = upload(lamh, S.lamh, S.n_lamh))) return e;
+    S.nc = nc;
+    S.L = L;
+    S.used = ++g_items_clock;
+    select_items(S);
     return E_OK;
 }
 
@@ -506,7 +554,9 @@ rqa::MxParams mx_params(const float *b, int nt, int m, int nbmax, int extra_u = 
     P.sh = sh;
     P.nb = nb;
     P.qlo = (int)(base_al >> sh);
-    P.cE = (float)((4.0 * m + 16.0 + extra_u) * std::ldexp(1.0, -24));
+    const int C = (m + 15) / 16;   // chunks of 16 dimensions in the FP16 hi/lo Gram
+    P.cE = (float)((3.0 * m + 31.0 + 33.0 * C + extra_u) * std::ldexp(1.0, -24));
+    P.dA = (float)(m * std::ldexp(1.0, -25));
     return P;
 }
 
@@ -574,13 +624,13 @@ int launch_group(const F *dx, int64_t nc, int tau, int m, const F *bounds, int n
                         if ((e = ensure(c.mxstats, 3 * sizeof(unsigned long long)))) return e;
                         const unsigned g = grid_for(rqa::k_lam_mx<F, TG, false, true>, LW.nitems, c.ncu);
                         hipLaunchKernelGGL((rqa::k_lam_mx<F, TG, false, true>), dim3(g), dim3(rqa::WG), 0, st,
-                                           dx, (const float *)c.nrm.p, nc, tau, m, Bf, B, nt,
+                                           dx, (const float *)c.nrm.p, (const unsigned int *)c.nmax.p, nc, tau, m, Bf, B, nt,
                                            (const uint8_t *)c.lut8.p, MP, L, LW, hist, carry, (uint32_t *)c.priv.p,
                                            tiles, (unsigned long long *)c.mxstats.p);
                     } else {
                         const unsigned g = grid_for(rqa::k_lam_mx<F, TG, false>, LW.nitems, c.ncu);
                         hipLaunchKernelGGL((rqa::k_lam_mx<F, TG, false>), dim3(g), dim3(rqa::WG), 0, st,
-                                           dx, (const float *)c.nrm.p, nc, tau, m, Bf, B, nt,
+                                           dx, (const float *)c.nrm.p, (const unsigned int *)c.nmax.p, nc, tau, m, Bf, B, nt,
                                            (const uint8_t *)c.lut8.p, MP, L, LW, hist, carry, (uint32_t *)c.priv.p,
                                            tiles, (unsigned long long *)nullptr);
                     }
@@ -599,7 +649,7 @@ int launch_group(const F *dx, int64_t nc, int tau, int m, const F *bounds, int n
                     if ((e = ensure_priv<TG>(g))) return e;
                     prof_begin("k_lam_mx", st);
                     hipLaunchKernelGGL((rqa::k_lam_mx<F, TG, true>), dim3(g), dim3(rqa::WG), 0, st, dx,
-                                       (const float *)c.nrm.p, nc, tau, m, Bf, B, nt, (const uint8_t *)c.lut8.p, MP, L,
+                                       (const float *)c.nrm.p, (const unsigned int *)c.nmax.p, nc, tau, m, Bf, B, nt, (const uint8_t *)c.lut8.p, MP, L,
                                        LW, hist, carry, (uint32_t *)c.priv.p, tiles, (unsigned long long *)nullptr);
                     prof_end(st);
                     HIP_TRY(hipGetLastError());
@@ -681,7 +731,8 @@ int launch_group(const F *dx, int64_t nc, int tau, int m, const F *bounds, int n
                     if ((e = ensure_priv<TG>(g))) return e;
                     prof_begin("k_det_mx", st);
                     hipLaunchKernelGGL((rqa::k_det_mx<TG>), dim3(g), dim3(rqa::WG), 0, st, (const float *)dx,
-                                       (const float *)c.nrm.p, nc, tau, m, Bf, nt, (const uint8_t *)lut8d, MP, L, nseg,
+                                       (const float *)c.nrm.p, (const unsigned int *)c.nmax.p, nc, tau, m, Bf, nt,
+                                       (const uint8_t *)lut8d, MP, L, nseg,
                                        (const int2 *)c.items_det.p, ni, hist, carry, (uint32_t *)c.priv.p);
                     prof_end(st);
                     HIP_TRY(hipGetLastError());
@@ -898,29 +949,30 @@ void host_api(int kind, F *output, const F *x, size_t n, const int *tau, int nta
         *error = e;
         return;
     }
+    // every (tau, emb) block writes its own part of one device output, which
+    // comes back in one copy
     const size_t per = kind == 0 ? (size_t)nthr : (size_t)nthr * nl * 5;
+    const size_t total = per * (size_t)ntau * (size_t)nemb;
     DevBuf &ob = kind == 0 ? c.out_rr : (kind == 1 ? c.out_det : c.out_lam);
-    if ((e = ensure(ob, per * sizeof(F)))) {
+    if ((e = ensure(ob, total * sizeof(F)))) {
         *error = e;
         return;
     }
     for (int a = 0; a < ntau; a++)
         for (int b = 0; b < nemb; b++) {
-            F *dout = (F *)ob.p;
+            F *dout = (F *)ob.p + ((size_t)a * nemb + b) * per;
             e = run_block<F>((const F *)c.x.p, n, tau[a], emb[b], thr, nthr, has_l ? lm : nullptr, has_l ? nl : 0,
                              dist, kind == 0 ? dout : nullptr, kind == 1 ? dout : nullptr, kind == 2 ? dout : nullptr,
                              st);
-            if (!e) {
-                hipError_t he = hipMemcpyAsync(output + ((size_t)a * nemb + b) * per, dout, per * sizeof(F),
-                                               hipMemcpyDeviceToHost, st);
-                if (he == hipSuccess) he = hipStreamSynchronize(st);
-                if (he != hipSuccess) e = map_err(he);
-            }
             if (e) {
+                (void)hipStreamSynchronize(st);
                 *error = e;
                 return;
             }
         }
+    hipError_t he = hipMemcpyAsync(output, ob.p, total * sizeof(F), hipMemcpyDeviceToHost, st);
+    if (he == hipSuccess) he = hipStreamSynchronize(st);
+    if (he != hipSuccess) *error = map_err(he);
 }
 
 template <typename F>
@@ -1108,6 +1160,26 @@ int accrqa_amd_mx_stats_read(unsigned long long *counts) {
     if (!g_ctx.mxstats.p) return E_OK;
     HIP_TRY(hipDeviceSynchronize());
     HIP_TRY(hipMemcpy(counts, g_ctx.mxstats.p, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    return E_OK;
+}
+
+int accrqa_amd_mfma16_probe(const uint16_t *a, const uint16_t *b, const float *c, float *d) {
+    if (!a || !b || !c || !d) return E_NULL;
+    std::lock_guard<std::mutex> lk(g_mu);
+    int e = init_ctx();
+    if (e) return e;
+    // a[32x16] and b[16x32] (fp16 bits) at 0 and 1 KiB, c at 2 KiB, d at 6 KiB
+    if ((e = ensure(g_ctx.probe, 10240))) return e;
+    char *p = (char *)g_ctx.probe.p;
+    hipStream_t st = g_ctx.stream;
+    HIP_TRY(hipMemcpyAsync(p, a, 1024, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(p + 1024, b, 1024, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(p + 2048, c, 4096, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(rqa::k_mfma16_probe, dim3(1), dim3(rqa::WAVE), 0, st, (const uint16_t *)p,
+                       (const uint16_t *)(p + 1024), (const float *)(p + 2048), (float *)(p + 6144));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(d, p + 6144, 4096, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
     return E_OK;
 }
 

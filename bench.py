@@ -39,6 +39,7 @@ sys.path.insert(0, ROOT)
 # dual-issue FP32 as 2.
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
 PUBLISHED_FP32_TFLOPS = 157.3
+PUBLISHED_FP16_TFLOPS = 2500.0   # dense FP16 MFMA
 
 
 def workload(name):
@@ -81,8 +82,10 @@ MEASURED_TRAFFIC = {
 def kernel_rooflines(kern, nc, emb, dist, workload):
     """Roofline figures of the main kernels from their average launch times.
     Algorithmic work per launch:
-      k_lam_mx      Gram dot products of the N_c(N_c+1)/2 distinct pairs it
-                    ranks (upper triangle and diagonal), 2m FLOP each, on MFMA;
+      k_lam_mx      one write of the rank-tile store (its HBM floor); beside
+                    it the FP16 MFMA work of the Gram: h.h, h.l and l.h
+                    products of the N_c(N_c+1)/2 distinct pairs it ranks
+                    (upper triangle and diagonal), 3 x 2m FLOP each;
       k_lam         the same pairs' exact-order distances, 3m (Euclidean) or 2m
                     (maximal) VALU FLOP each;
       k_lam_vscan,  one read of the rank-tile store (one byte per pair of the
@@ -95,9 +98,13 @@ def kernel_rooflines(kern, nc, emb, dist, workload):
         if t <= 0:
             continue
         if k == "k_lam_mx":
-            w = pairs * 2 * emb
-            out[k] = {"bound": "mfma", "achieved": w / t / 1e12, "peak": PUBLISHED_FP32_TFLOPS, "unit": "TFLOP/s",
-                      "work_per_launch": w, "work_unit": "FLOP (2m per distinct pair)"}
+            w = float(tile_store_bytes(nc))
+            g = pairs * 3 * 2 * emb
+            out[k] = {"bound": "hbm", "achieved": w / t / 1e9, "peak": 8000.0, "unit": "GB/s",
+                      "work_per_launch": w, "work_unit": "bytes (one write of the rank-tile store)",
+                      "mfma": {"achieved": g / t / 1e12, "peak": PUBLISHED_FP16_TFLOPS, "unit": "TFLOP/s",
+                               "frac": g / t / 1e12 / PUBLISHED_FP16_TFLOPS,
+                               "work_unit": "FP16 MFMA FLOP (h.h, h.l, l.h: 3 x 2m per distinct pair)"}}
         elif k == "k_lam":
             w = pairs * (3 * emb if dist == 1 else 2 * emb)
             out[k] = {"bound": "valu", "achieved": w / t / 1e12, "peak": VALU_PEAK_TOPS, "unit": "TFLOP/s",
@@ -281,7 +288,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": c["desc"], "N": int(x.size), "N_c": int(nc), "thresholds": T,
-                   "form": {"mfma": "MFMA Gram (fp32) + exact-order recheck near thresholds",
+                   "form": {"mfma": "MFMA Gram (FP16 hi/lo split, FP32 accumulation) + exact-order recheck near thresholds",
                             "exact": "exact-order FP32 sums on VALU"}[head]},
         "roofline": roofline,
         "kernels_ms_per_step": {k: v["ms_total"] / args.steps for k, v in kern.items()},
@@ -332,7 +339,7 @@ def extras(accrqa_amd, L, torch, args, c, x, step):
     * config3_m: the same series at m = 32 and 64 (thresholds
       linspace(0.1, 1.6) * sqrt(m)), both forms, outputs compared bit for bit;
     * fp64_input: the m = 16 workload with the series and thresholds in fp64,
-      both forms (the MFMA form: FP32 Gram of the rounded series, exact
+      both forms (the MFMA form: FP16 hi/lo Gram of the series rounded to FP32, exact
       re-decision in FP64), outputs compared bit for bit;
     * config4_per_series: the batched configuration's throughput through the
       public API, one series at a time (config4_loop)."""
@@ -388,7 +395,7 @@ def extras(accrqa_amd, L, torch, args, c, x, step):
     out["fp64_input"] = {"exact_ms_per_step": me, "exact_value": ve, "mfma_ms_per_step": mm, "mfma_value": vm,
                          "mfma_vs_exact_speedup": me / mm, "mismatches_mfma_vs_exact": mis,
                          "forms": {"exact": "exact-order FP64 sums on VALU",
-                                   "mfma": "FP32 MFMA Gram of the rounded series + FP64 exact-order recheck"}}
+                                   "mfma": "FP16 hi/lo MFMA Gram of the series rounded to FP32 + FP64 exact-order recheck"}}
     return out
 
 

@@ -31,6 +31,7 @@
 #define ACCRQA_AMD_H
 
 #include <stddef.h>
+#include <stdint.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -115,7 +116,8 @@ int accrqa_amd_rqa_device_f64(const double *d_input, size_t data_size, int tau, 
  * VALU sums otherwise; 1 = exact VALU sums only; 2 = MFMA form wherever it
  * applies (Euclidean norm; fp32 input, and fp64 input in the LAM pass, whose
  * Gram then runs on the series rounded to fp32).  The MFMA form computes
- * |x_i|^2 + |x_j|^2 - 2 x_i.x_j with the Gram tile on matrix cores and
+ * |x_i|^2 + |x_j|^2 - 2 x_i.x_j with the Gram tile on FP16 matrix cores
+ * (each scaled sample split into two halves, hi/lo) and
  * re-decides every pair within a rigorous error bound of a threshold in the
  * reference's exact order and precision, so results are identical for every
  * form.  Process-wide; returns 0, or 2 for an unknown form. */
@@ -141,6 +143,14 @@ int accrqa_amd_set_split(int mode);
  * Both return 0 or an Accrqa_Error code. */
 int accrqa_amd_mx_stats(int enable);
 int accrqa_amd_mx_stats_read(unsigned long long *counts);
+
+/* One matrix-core instruction of the MFMA form's Gram (diagnostic, no
+ * counterpart in the reference), for checking the properties its error bound
+ * assumes: d = c + a b with a = A[32][16] and b = B[16][32] as fp16 bit
+ * patterns and c, d = [32][32] fp32, all row-major host arrays, computed by one
+ * v_mfma_f32_32x32x16_f16 with the operands in the Gram kernels' lane layout.
+ * Returns 0 or an Accrqa_Error code. */
+int accrqa_amd_mfma16_probe(const uint16_t *a, const uint16_t *b, const float *c, float *d);
 
 /* Kernel timing (tracing aid, no counterpart in the reference): when enabled,
  * every kernel launch is bracketed by HIP events on its own stream and the

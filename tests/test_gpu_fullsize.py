@@ -80,8 +80,8 @@ def test_config3_gate_shape(forms):
 
 
 def test_config3_fp64_input(forms):
-    """The gate workload in fp64: the MFMA form (fp32 Gram of the rounded
-    series, exact re-decision in fp64) against the exact fp64 form."""
+    """The gate workload in fp64: the MFMA form (hi/lo Gram of the series
+    rounded to fp32, exact re-decision in fp64) against the exact fp64 form."""
     amd = forms
     c = W.config3(m=16, dtype=np.float64)
     x, tau, emb, thr = c["x"], c["tau"][0], c["emb"][0], c["thr"]

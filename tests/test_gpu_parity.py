@@ -343,7 +343,17 @@ def test_mfma_form_bit_exact(mfma_form, oracle):
     # unused steps in the first chunk of 16 dimensions, and a second chunk
     cases.append(("tau0", rng.standard_normal(1500).astype(np.float32), [0, 3], [10, 21, 5], [2],
                   list(np.linspace(0.1, 4.0, 9).astype(np.float32))))
-    # fp64 input: fp32 Gram of the rounded series, exact re-decision in fp64
+    # the Gram's scale and split: a spike 10^6 times the rest (the other
+    # samples' low halves are fp16 subnormals), and series of tiny and huge
+    # magnitude (scale 2^e at both clamps' sides)
+    xs = (1e-3 * rng.standard_normal(1700)).astype(np.float32)
+    xs[777] = 1e3
+    cases.append(("spike", xs, [1], [16, 9], [2], list(np.linspace(2e-4, 8e-3, 11).astype(np.float32))))
+    cases.append(("tiny", (1e-18 * rng.standard_normal(1200)).astype(np.float32), [1], [16], [2],
+                  list(np.linspace(1e-19, 6e-18, 9).astype(np.float32))))
+    cases.append(("huge", (1e12 * rng.standard_normal(1200)).astype(np.float32), [2], [16], [2],
+                  list(np.linspace(1e11, 6e12, 9).astype(np.float32))))
+    # fp64 input: hi/lo Gram of the series rounded to fp32, exact re-decision in fp64
     # (the same cases, the thresholds and series widened, plus one series
     # whose fp64 values are not fp32 values)
     cases += [(n + " fp64", x.astype(np.float64), taus, embs, lm, [float(t) for t in thrs])
@@ -359,6 +369,55 @@ def test_mfma_form_bit_exact(mfma_form, oracle):
         for k, t in enumerate(thrs):
             assert np.array_equal(got["lam"][k], oracle.hist("lam", x, dt(t), taus[0], embs[0], 1)), (name, t)
             assert np.array_equal(got["det"][k], oracle.hist("det", x, dt(t), taus[0], embs[0], 1)), (name, t)
+
+
+def test_mfma16_instruction_properties(amd):
+    """What the MFMA form's error bound takes of v_mfma_f32_32x32x16_f16, in
+    the Gram kernels' lane layout: products exact, fp16 subnormal operands
+    kept, and the result within 16 additions of relative error 2^-23 each of
+    the exact c + a b, i.e. |d - (c + ab)| <= 32u (|c| + |a||b|), u = 2^-24.
+    A layout fault (a and b in different dimension slots) fails every case."""
+    rng = np.random.default_rng(3)
+    u = 2.0 ** -24
+    g16 = 32 * u / (1 - 32 * u)
+
+    def check(a, b, c, what, exact_result=False):
+        a = np.asarray(a, np.float16)
+        b = np.asarray(b, np.float16)
+        c = np.asarray(c, np.float32)
+        d = amd.mfma16_probe(a, b, c).astype(np.float64)
+        A, B, Cc = a.astype(np.float64), b.astype(np.float64), c.astype(np.float64)
+        want = Cc + A @ B   # fp16 products are exact in fp64; 17-term sums off by < 2^-48 relative
+        if exact_result:
+            assert np.array_equal(d, want), (what, np.abs(d - want).max())
+        bound = g16 * (np.abs(Cc) + np.abs(A) @ np.abs(B)) * (1 + 2.0 ** -40)
+        assert np.all(np.abs(d - want) <= bound), (what, (np.abs(d - want) / np.maximum(bound, 1e-300)).max())
+
+    z = np.zeros((32, 32), np.float32)
+    # layout and plain sums
+    check(rng.standard_normal((32, 16)), rng.standard_normal((16, 32)), z, "normal")
+    check(rng.integers(-8, 9, (32, 16)), rng.integers(-8, 9, (16, 32)), rng.integers(-50, 50, (32, 32)),
+          "integers", exact_result=True)
+    # exponents spread over the fp16 range, random signs, and an accumulator
+    for s in range(4):
+        ea, eb = rng.integers(-14, 15, (32, 16)), rng.integers(-14, 15, (16, 32))
+        a = rng.choice([-1.0, 1.0], (32, 16)) * (1 + rng.random((32, 16))) * 2.0 ** ea
+        b = rng.choice([-1.0, 1.0], (16, 32)) * (1 + rng.random((16, 32))) * 2.0 ** eb
+        c = (rng.standard_normal((32, 32)) * 2.0 ** rng.integers(-20, 20, (32, 32))).astype(np.float32)
+        check(a, b, c, f"spread {s}")
+    # cancellation: pairs of equal and opposite products plus small terms
+    a = rng.standard_normal((32, 16)) * 100
+    a[:, 1::2] = a[:, 0::2]
+    b = rng.standard_normal((16, 32)) * 100
+    b[1::2, :] = -b[0::2, :] * (1 + 2.0 ** -10 * rng.integers(-3, 4, (8, 32)))
+    check(a, b, z, "cancel")
+    # fp16 subnormal operands (2^-24 .. 2^-15) against normal ones: kept, not flushed
+    sub = rng.integers(1, 1024, (32, 16)) * 2.0 ** -24
+    check(sub, np.full((16, 32), 2.0 ** 14), z, "subnormal x 2^14", exact_result=True)
+    check(rng.standard_normal((32, 16)) * 2.0 ** 13, rng.integers(-1023, 1024, (16, 32)) * 2.0 ** -24, z,
+          "normal x subnormal")
+    assert np.all(amd.mfma16_probe(np.full((32, 16), 2.0 ** -24, np.float16), np.ones((16, 32), np.float16), z)
+                  == np.float32(16 * 2.0 ** -24))
 
 
 @pytest.mark.gpu
