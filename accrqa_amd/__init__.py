@@ -16,7 +16,7 @@ import os
 
 import numpy as np
 
-__all__ = ["RR", "DET", "LAM", "histograms", "rqa_device", "set_form", "set_split", "AccrqaError", "library_path",
+__all__ = ["RR", "DET", "LAM", "RP", "rp_device", "histograms", "rqa_device", "set_form", "set_split", "AccrqaError", "library_path",
            "load", "PLT_AMD_GPU", "FORM_AUTO", "FORM_EXACT", "FORM_MFMA", "SPLIT_AUTO", "SPLIT_NEVER",
            "DST_EUCLIDEAN", "DST_MAXIMAL"]
 
@@ -86,6 +86,15 @@ def load():
         f = getattr(L, f"accrqa_amd_rqa_device_{sfx}")
         f.argtypes = [P, SZ, I, I, P, I, P, I, I, P, P, P, P]
         f.restype = I
+        fl = C.c_float if sfx == "f32" else C.c_double
+        f = getattr(L, f"accrqa_amd_RP_{sfx}")
+        f.argtypes = [P, P, SZ, I, I, fl, I, P]
+        f.restype = None
+        f = getattr(L, f"accrqa_amd_rp_device_{sfx}")
+        f.argtypes = [P, SZ, I, I, fl, I, P, P]
+        f.restype = I
+    L.accrqa_amd_RP_f64_i32.argtypes = [P, P, SZ, I, I, C.c_double, I, P]
+    L.accrqa_amd_RP_f64_i32.restype = None
     L.accrqa_amd_set_form.argtypes = [I]
     L.accrqa_amd_set_form.restype = I
     L.accrqa_amd_set_split.argtypes = [I]
@@ -202,6 +211,36 @@ def RR(input_data, tau_values, emb_values, threshold_values, distance_type="eucl
                                        thr.size, _dist(distance_type), C.byref(err))
     _raise(err)
     return out
+
+
+def RP(input_data, tau, emb, threshold, distance_type="euclidean", out_dtype=np.int8):
+    """Recurrence plot, int8 [N_c, N_c] (the reference's char output; int32
+    for fp64 input with out_dtype=np.int32), N_c = N - (emb - 1) tau."""
+    L = load()
+    x = np.ascontiguousarray(input_data)
+    sfx = _sfx(x.dtype)
+    nc = x.size - (int(emb) - 1) * int(tau)
+    i32 = np.dtype(out_dtype) == np.int32
+    if i32 and sfx != "f64":
+        raise TypeError("int32 output is for fp64 input (as the reference's accrqa_RP overloads)")
+    out = np.zeros((max(nc, 0), max(nc, 0)), dtype=np.int32 if i32 else np.int8)
+    err = C.c_int(-1)
+    fn = L.accrqa_amd_RP_f64_i32 if i32 else getattr(L, f"accrqa_amd_RP_{sfx}")
+    fn(_p(out), _p(x), x.size, int(tau), int(emb), float(threshold), _dist(distance_type), C.byref(err))
+    _raise(err)
+    return out
+
+
+def rp_device(d_input, n, tau, emb, threshold, distance_type, d_output, stream=0, dtype=np.float32):
+    """Device-resident recurrence plot: d_input (n samples) and d_output
+    (N_c * N_c bytes) are device pointers; asynchronous on `stream`."""
+    L = load()
+    sfx = _sfx(np.dtype(dtype))
+    e = getattr(L, f"accrqa_amd_rp_device_{sfx}")(C.c_void_p(d_input), n, int(tau), int(emb), float(threshold),
+                                                  _dist(distance_type), C.c_void_p(d_output),
+                                                  C.c_void_p(stream or None))
+    if e != 0:
+        raise AccrqaError(e, L.accrqa_amd_last_error().decode())
 
 
 def _dl(kind, input_data, tau_values, emb_values, l_values, threshold_values, distance_type, calculate_ENTR):

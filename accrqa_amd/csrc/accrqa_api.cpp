@@ -85,6 +85,35 @@ void accrqa_RR(double *output, double *input_data, size_t data_size, int *tau_va
     if (error) *error = (Accrqa_Error)e;
 }
 
+// accrqa_RP (AccRQA_functions.cpp:996-1060): no platform argument there (the
+// reference computes it on the CPU); here it runs on the AMD GPU.  As in the
+// reference, a call whose *error is not SUCCESS on entry does nothing.
+void accrqa_RP(char *output, float *input_data, size_t data_size, int tau, int emb, float threshold,
+               Accrqa_Distance distance_type, Accrqa_Error *error) {
+    if (*error != SUCCESS) return;
+    int e = 0;
+    accrqa_amd_RP_f32(output, input_data, data_size, tau, emb, threshold, (int)distance_type, &e);
+    *error = (Accrqa_Error)e;
+}
+void accrqa_RP(char *output, double *input_data, size_t data_size, int tau, int emb, double threshold,
+               Accrqa_Distance distance_type, Accrqa_Error *error) {
+    if (*error != SUCCESS) return;
+    int e = 0;
+    accrqa_amd_RP_f64(output, input_data, data_size, tau, emb, threshold, (int)distance_type, &e);
+    *error = (Accrqa_Error)e;
+}
+void accrqa_RP(int *output, double *input_data, size_t data_size, int tau, int emb, double threshold,
+               Accrqa_Distance distance_type, Accrqa_Error *error) {
+    if (*error != SUCCESS) return;
+    int e = 0;
+    accrqa_amd_RP_f64_i32(output, input_data, data_size, tau, emb, threshold, (int)distance_type, &e);
+    *error = (Accrqa_Error)e;
+}
+// AccRQA_functions.cpp:989-994: N_c (the plot is N_c x N_c)
+int64_t accrqa_RP_output_size_in_elements(int64_t input_size, int64_t tau, int64_t emb) {
+    return input_size - (emb - 1) * tau;
+}
+
 // Output sizes (AccRQA_functions.cpp:558-560, 812-814, 978-982).
 int accrqa_LAM_output_size_in_elements(int nTaus, int nEmbs, int nVmins, int nThresholds) {
     return nTaus * nEmbs * nVmins * nThresholds * 5;

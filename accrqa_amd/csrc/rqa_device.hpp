@@ -728,6 +728,45 @@ __global__ __launch_bounds__(WG) void k_lam_hscan(const uint8_t *__restrict__ ti
     flush_hist<TG>(lh, priv, nt, hist, nc, WG);
 }
 
+// ---------------------------------------------------------------- k_rp
+// The recurrence plot (accrqa_RP, src/AccRQA_functions.cpp:996-1060):
+// out[i N_c + j] = [s_ij <= B] (1 or 0) for the one threshold's bound B,
+// row-major.  Work item = one block of 32 rows x 64 columns, one per wave;
+// lane = column.  Row samples are wave-uniform (scalar) loads, as in k_rr;
+// each row of the block is one store of 64 consecutive elements.  Reads
+// reach x[N + 62] at most (the series buffer is padded with zeros).
+template <typename F, int NORM, typename O>
+__global__ __launch_bounds__(WG) void k_rp(const F *__restrict__ x, int64_t nc, int tau, int m, F bound,
+                                           O *__restrict__ out) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wave = wave_id();
+    const int64_t nbands = (nc + WAVE - 1) / WAVE;
+    const int64_t nblk = ((nc + BLK - 1) / BLK) * nbands;
+    for (int64_t b = (int64_t)blockIdx.x * WAVES + wave; b < nblk; b += (int64_t)gridDim.x * WAVES) {
+        const int64_t I = b / nbands, J = b - I * nbands;
+        const int64_t i0 = I * BLK;
+        const int64_t j = J * WAVE + lane;
+        F s[BLK];
+#pragma unroll
+        for (int u = 0; u < BLK; u++) s[u] = F(0);
+        F xj = x[j];
+        for (int k = 0; k < m; k++) {
+            const F xjn = x[j + (int64_t)(k + 1 < m ? k + 1 : k) * tau];
+            const F *xr = x + i0 + (int64_t)k * tau;
+#pragma unroll
+            for (int u = 0; u < BLK; u++) s[u] = dim_step<NORM>(s[u], xr[u], xj);
+            xj = xjn;
+        }
+        if (j < nc) {
+            const int nr = nc - i0 < BLK ? (int)(nc - i0) : BLK;
+            O *o = out + i0 * nc + j;
+#pragma unroll
+            for (int u = 0; u < BLK; u++)
+                if (u < nr) o[(int64_t)u * nc] = (O)(s[u] <= bound ? 1 : 0);
+        }
+    }
+}
+
 // ---------------------------------------------------------------- k_rr
 // Work item = (band of 64 columns, row segment q) over rows i <= max column
 // of the band.  Pair weight: 2 if i < j, 1 if i == j, 0 otherwise.

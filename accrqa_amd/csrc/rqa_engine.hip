@@ -52,7 +52,7 @@ struct Ctx {
     int ncu = 0;
     hipStream_t stream = nullptr;
     DevBuf x, hist, carry, metric, scan, counts, perm, lmins, lut, nrm, nmax, lut8, out_rr, out_det, out_lam, priv,
-        tiles, dump, tops, mxstats, probe;
+        tiles, dump, tops, mxstats, probe, out_rp;
     // work-item tables of the current (nc, seg_len): views of an ItemSet
     // entry (owned there, never freed through these)
     DevBuf items_rr, items_det, items_dets, items_lamv, items_lamh;
@@ -154,7 +154,7 @@ int init_ctx() {
                              &g_ctx.lut,     &g_ctx.nrm,        &g_ctx.nmax,       &g_ctx.lut8,
                              &g_ctx.out_rr,  &g_ctx.out_det,    &g_ctx.out_lam,    &g_ctx.priv,
                              &g_ctx.tiles,   &g_ctx.dump,       &g_ctx.tops,
-                             &g_ctx.mxstats, &g_ctx.probe};
+                             &g_ctx.mxstats, &g_ctx.probe, &g_ctx.out_rp};
             for (DevBuf *b : all) {
                 if (b->p) (void)hipFree(b->p);
                 *b = DevBuf();
@@ -1048,6 +1048,68 @@ int device_api(const F *d_in, size_t n, int tau, int emb, const F *thr, int T, c
                         d_lam, st);
 }
 
+// The recurrence plot of one (tau, emb, threshold) into d_out (N_c x N_c
+// elements of O, row-major), from the padded series dx.  A threshold that
+// never recurs (negative or NaN) gets bound -1, below every s.
+template <typename F, typename O>
+int rp_run(const F *dx, size_t n, int tau, int emb, F thr, int dist, O *d_out, hipStream_t st) {
+    Ctx &c = g_ctx;
+    const int64_t nc = (int64_t)n - (int64_t)(emb - 1) * tau;
+    F b = bound_of(thr, dist);
+    if (never_recurs(b)) b = F(-1);
+    const int64_t nblk = ((nc + rqa::BLK - 1) / rqa::BLK) * ((nc + rqa::WAVE - 1) / rqa::WAVE);
+    prof_begin("k_rp", st);
+    if (dist == 1) {
+        const unsigned g = grid_for(rqa::k_rp<F, 1, O>, nblk, c.ncu);
+        hipLaunchKernelGGL((rqa::k_rp<F, 1, O>), dim3(g), dim3(rqa::WG), 0, st, dx, nc, tau, emb, b, d_out);
+    } else {
+        const unsigned g = grid_for(rqa::k_rp<F, 2, O>, nblk, c.ncu);
+        hipLaunchKernelGGL((rqa::k_rp<F, 2, O>), dim3(g), dim3(rqa::WG), 0, st, dx, nc, tau, emb, b, d_out);
+    }
+    prof_end(st);
+    HIP_TRY(hipGetLastError());
+    prof_collect(st);
+    return E_OK;
+}
+
+template <typename F, typename O>
+void host_rp(O *output, const F *x, size_t n, int tau, int emb, F thr, int dist, int *error) {
+    if (!error) return;
+    *error = E_OK;
+    if (n == 0) *error = E_INVALID;
+    if (!output || !x) *error = E_NULL;
+    if (*error) return;
+    std::lock_guard<std::mutex> lk(g_mu);
+    int e = check_args(n, &tau, 1, &emb, 1, nullptr, 0, dist);
+    if (!e) e = init_ctx();
+    Ctx &c = g_ctx;
+    hipStream_t st = c.stream;
+    if (!e) e = upload_series(x, n, false, st);
+    const int64_t nc = (int64_t)n - (int64_t)(emb - 1) * tau;
+    const size_t bytes = e ? 0 : (size_t)nc * (size_t)nc * sizeof(O);
+    if (!e) e = ensure(c.out_rp, bytes);
+    if (!e) e = rp_run<F, O>((const F *)c.x.p, n, tau, emb, thr, dist, (O *)c.out_rp.p, st);
+    if (!e) {
+        hipError_t he = hipMemcpyAsync(output, c.out_rp.p, bytes, hipMemcpyDeviceToHost, st);
+        if (he == hipSuccess) he = hipStreamSynchronize(st);
+        if (he != hipSuccess) e = map_err(he);
+    }
+    *error = e;
+}
+
+template <typename F, typename O>
+int device_rp(const F *d_in, size_t n, int tau, int emb, F thr, int dist, O *d_out, void *stream) {
+    if (n == 0) return E_INVALID;
+    if (!d_in || !d_out) return E_NULL;
+    std::lock_guard<std::mutex> lk(g_mu);
+    int e = check_args(n, &tau, 1, &emb, 1, nullptr, 0, dist);
+    if (!e) e = init_ctx();
+    if (e) return e;
+    hipStream_t st = (hipStream_t)stream;
+    if ((e = upload_series(d_in, n, true, st))) return e;
+    return rp_run<F, O>((const F *)g_ctx.x.p, n, tau, emb, thr, dist, d_out, st);
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------- C ABI
@@ -1161,6 +1223,27 @@ int accrqa_amd_mx_stats_read(unsigned long long *counts) {
     HIP_TRY(hipDeviceSynchronize());
     HIP_TRY(hipMemcpy(counts, g_ctx.mxstats.p, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
     return E_OK;
+}
+
+void accrqa_amd_RP_f32(char *output, const float *input_data, size_t data_size, int tau, int emb, float threshold,
+                       int distance_type, int *error) {
+    host_rp<float, char>(output, input_data, data_size, tau, emb, threshold, distance_type, error);
+}
+void accrqa_amd_RP_f64(char *output, const double *input_data, size_t data_size, int tau, int emb,
+                       double threshold, int distance_type, int *error) {
+    host_rp<double, char>(output, input_data, data_size, tau, emb, threshold, distance_type, error);
+}
+void accrqa_amd_RP_f64_i32(int *output, const double *input_data, size_t data_size, int tau, int emb,
+                           double threshold, int distance_type, int *error) {
+    host_rp<double, int>(output, input_data, data_size, tau, emb, threshold, distance_type, error);
+}
+int accrqa_amd_rp_device_f32(const float *d_input, size_t data_size, int tau, int emb, float threshold,
+                             int distance_type, char *d_output, void *stream) {
+    return device_rp<float, char>(d_input, data_size, tau, emb, threshold, distance_type, d_output, stream);
+}
+int accrqa_amd_rp_device_f64(const double *d_input, size_t data_size, int tau, int emb, double threshold,
+                             int distance_type, char *d_output, void *stream) {
+    return device_rp<double, char>(d_input, data_size, tau, emb, threshold, distance_type, d_output, stream);
 }
 
 int accrqa_amd_mfma16_probe(const uint16_t *a, const uint16_t *b, const float *c, float *d) {

@@ -14,6 +14,7 @@
 #define ACCRQA_AMD_HPP
 
 #include <cstddef>
+#include <cstdint>
 
 enum Accrqa_Error {
     SUCCESS = 0,
@@ -71,5 +72,15 @@ void accrqa_RR(double *output, double *input_data, size_t data_size, int *tau_va
                int nEmbs, double *threshold_values, int nThresholds, Accrqa_Distance distance_type,
                Accrqa_CompPlatform comp_platform, Accrqa_Error *error);
 int accrqa_RR_output_size_in_elements(int nTaus, int nEmbs, int nThresholds);
+
+// Recurrence plot, the reference's three overloads (no platform argument:
+// the reference computes it on the CPU, this library on the AMD GPU).
+void accrqa_RP(char *output, float *input_data, size_t data_size, int tau, int emb, float threshold,
+               Accrqa_Distance distance_type, Accrqa_Error *error);
+void accrqa_RP(char *output, double *input_data, size_t data_size, int tau, int emb, double threshold,
+               Accrqa_Distance distance_type, Accrqa_Error *error);
+void accrqa_RP(int *output, double *input_data, size_t data_size, int tau, int emb, double threshold,
+               Accrqa_Distance distance_type, Accrqa_Error *error);
+int64_t accrqa_RP_output_size_in_elements(int64_t input_size, int64_t tau, int64_t emb);
 
 #endif

@@ -111,6 +111,31 @@ int accrqa_amd_rqa_device_f64(const double *d_input, size_t data_size, int tau, 
                               const double *threshold_values, int nThresholds, const int *lmin_values, int nL,
                               int distance_type, double *d_rr, double *d_det, double *d_lam, void *stream);
 
+/* Recurrence plot.  Replaces the three overloads of accrqa_RP
+ * (src/AccRQA_functions.cpp:996-1060, CPU only there:
+ * rqa_CPU_R_matrix_parallel, src/AccRQA_CPU_function.cpp:226-244):
+ * output[i*N_c + j] = 1 if the distance of rows i and j is within the
+ * threshold ((threshold - distance) >= 0 in the input type), else 0, for
+ * N_c = data_size - (emb - 1)*tau, row-major; char output for float and
+ * double input, int output for double input, as in the reference.  The
+ * output buffer holds N_c*N_c elements (accrqa_RP_output_size_in_elements
+ * gives N_c).  Errors as above; the reference validates nothing here, this
+ * returns 2 for tau < 0, emb < 1, N_c < 1 or a distance type other than 1/2. */
+void accrqa_amd_RP_f32(char *output, const float *input_data, size_t data_size, int tau, int emb, float threshold,
+                       int distance_type, int *error);
+void accrqa_amd_RP_f64(char *output, const double *input_data, size_t data_size, int tau, int emb,
+                       double threshold, int distance_type, int *error);
+void accrqa_amd_RP_f64_i32(int *output, const double *input_data, size_t data_size, int tau, int emb,
+                           double threshold, int distance_type, int *error);
+
+/* The same with the series and the plot in device memory (N_c*N_c bytes at
+ * d_output), launched on `stream` (the null stream when 0) and not
+ * synchronised.  Returns 0 or an Accrqa_Error code. */
+int accrqa_amd_rp_device_f32(const float *d_input, size_t data_size, int tau, int emb, float threshold,
+                             int distance_type, char *d_output, void *stream);
+int accrqa_amd_rp_device_f64(const double *d_input, size_t data_size, int tau, int emb, double threshold,
+                             int distance_type, char *d_output, void *stream);
+
 /* Form of the Euclidean distance (no counterpart in the reference, which has
  * one form per platform).  0 = automatic: the MFMA form for emb >= 8, exact
  * VALU sums otherwise; 1 = exact VALU sums only; 2 = MFMA form wherever it

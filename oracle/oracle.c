@@ -81,6 +81,26 @@ int oracle_num_threads(void) {
                                int dist) {                                                        \
         return r_elem_##SFX(x, i, j, thr, tau, emb, dist);                                        \
     }                                                                                             \
+    /* accrqa_RP, AccRQA_functions.cpp:996-1060 -> rqa_CPU_R_matrix_parallel,                      \
+       AccRQA_CPU_function.cpp:226-244: out[i N_c + j] = R(i, j), row-major, char or int */        \
+    int oracle_RP_u8_##SFX(uint8_t *out, const F *x, size_t n, F thr, int tau, int emb, int dist) {  \
+        int64_t nc = (int64_t)n - (int64_t)(emb - 1) * tau;                                       \
+        if (nc < 1 || tau < 0 || emb < 1) return ERR_INVALID_ARGUMENT;                            \
+        _Pragma("omp parallel for schedule(static)")                                              \
+        for (int64_t i = 0; i < nc; i++)                                                          \
+            for (int64_t j = 0; j < nc; j++)                                                      \
+                out[i * nc + j] = (uint8_t)r_elem_##SFX(x, i, j, thr, tau, emb, dist);            \
+        return ERR_SUCCESS;                                                                       \
+    }                                                                                             \
+    int oracle_RP_i32_##SFX(int32_t *out, const F *x, size_t n, F thr, int tau, int emb, int dist) { \
+        int64_t nc = (int64_t)n - (int64_t)(emb - 1) * tau;                                       \
+        if (nc < 1 || tau < 0 || emb < 1) return ERR_INVALID_ARGUMENT;                            \
+        _Pragma("omp parallel for schedule(static)")                                              \
+        for (int64_t i = 0; i < nc; i++)                                                          \
+            for (int64_t j = 0; j < nc; j++)                                                      \
+                out[i * nc + j] = (int32_t)r_elem_##SFX(x, i, j, thr, tau, emb, dist);            \
+        return ERR_SUCCESS;                                                                       \
+    }                                                                                             \
     /* rqa_CPU_RR_metric_ref_parallel, AccRQA_CPU_function.cpp:347-378 (count only) */            \
     int64_t oracle_rr_count_##SFX(const F *x, size_t n, F thr, int tau, int emb, int dist) {      \
         int64_t nc = (int64_t)n - (int64_t)(emb - 1) * tau;                                       \

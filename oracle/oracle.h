@@ -38,6 +38,12 @@ int oracle_R_element_f64(const double *x, int64_t i, int64_t j, double thr, int 
 
 /* Full-matrix recurrence count (AccRQA_CPU_function.cpp:347-378, without the divide). */
 int64_t oracle_rr_count_f32(const float *x, size_t n, float thr, int tau, int emb, int dist);
+/* accrqa_RP (AccRQA_functions.cpp:996-1060): the N_c x N_c recurrence plot,
+ * row-major, as bytes or ints; 0 or ERR_INVALID_ARGUMENT. */
+int oracle_RP_u8_f32(uint8_t *out, const float *x, size_t n, float thr, int tau, int emb, int dist);
+int oracle_RP_u8_f64(uint8_t *out, const double *x, size_t n, double thr, int tau, int emb, int dist);
+int oracle_RP_i32_f32(int32_t *out, const float *x, size_t n, float thr, int tau, int emb, int dist);
+int oracle_RP_i32_f64(int32_t *out, const double *x, size_t n, double thr, int tau, int emb, int dist);
 int64_t oracle_rr_count_f64(const double *x, size_t n, double thr, int tau, int emb, int dist);
 
 /* Diagonal line histogram h_D, size N_c+1, including the doubling and the

@@ -53,6 +53,10 @@ class Oracle:
             f = getattr(L, f"oracle_rr_count_{sfx}")
             f.argtypes = [_P, _SZ, fl, _I, _I, _I]
             f.restype = _I64
+            for ot in ("u8", "i32"):
+                f = getattr(L, f"oracle_RP_{ot}_{sfx}")
+                f.argtypes = [_P, _P, _SZ, fl, _I, _I, _I]
+                f.restype = _I
             for kind in ("det", "lam"):
                 f = getattr(L, f"oracle_{kind}_hist_{sfx}")
                 f.argtypes = [_P, _P, _SZ, fl, _I, _I, _I]
@@ -85,6 +89,16 @@ class Oracle:
     def rr_count(self, x, thr, tau, emb, dist):
         x = np.ascontiguousarray(x)
         return getattr(self.L, f"oracle_rr_count_{self._sfx(x)}")(_ptr(x), x.size, thr, tau, emb, dist)
+
+    def RP(self, x, tau, emb, thr, dist, out_dtype=np.int8):
+        """(N_c x N_c recurrence plot, error code): int8 (the reference's char
+        output) or int32."""
+        x = np.ascontiguousarray(x)
+        nc = max(nc_of(x.size, tau, emb), 0)
+        ot = "i32" if np.dtype(out_dtype) == np.int32 else "u8"
+        out = np.zeros((nc, nc), dtype=np.int32 if ot == "i32" else np.int8)
+        e = getattr(self.L, f"oracle_RP_{ot}_{self._sfx(x)}")(_ptr(out), _ptr(x), x.size, thr, tau, emb, dist)
+        return out, e
 
     def hist(self, kind, x, thr, tau, emb, dist):
         """Raw histogram h_D ('det') or h_V ('lam'), uint64[N_c+1]."""
@@ -175,6 +189,10 @@ class Reference:
             f = getattr(L, f"ref_r_element_{sfx}")
             f.argtypes = [_P, C.c_long, C.c_long, fl, _I, _I, _I]
             f.restype = _I
+        for name, fl in (("ref_rp_u8_f32", C.c_float), ("ref_rp_u8_f64", C.c_double), ("ref_rp_i32_f64", C.c_double)):
+            f = getattr(L, name)
+            f.argtypes = [_P, _P, C.c_long, _I, _I, fl, _I]
+            f.restype = _I
 
     @staticmethod
     def _sfx(x):
@@ -189,6 +207,16 @@ class Reference:
     def r_element(self, x, i, j, thr, tau, emb, dist):
         x = np.ascontiguousarray(x)
         return getattr(self.L, f"ref_r_element_{self._sfx(x)}")(_ptr(x), i, j, thr, tau, emb, dist)
+
+    def RP(self, x, tau, emb, thr, dist, out_dtype=np.int8):
+        """The reference's accrqa_RP: (N_c x N_c plot, error code); int32
+        output exists for fp64 input only, as in the reference."""
+        x = np.ascontiguousarray(x)
+        nc = nc_of(x.size, tau, emb)
+        ot = "i32" if np.dtype(out_dtype) == np.int32 else "u8"
+        out = np.zeros((nc, nc), dtype=np.int32 if ot == "i32" else np.int8)
+        e = getattr(self.L, f"ref_rp_{ot}_{self._sfx(x)}")(_ptr(out), _ptr(x), x.size, tau, emb, thr, dist)
+        return out, e
 
     def RR(self, x, tau, emb, thr, dist, platform=PLT_CPU):
         x = np.ascontiguousarray(x)

@@ -8,6 +8,18 @@
 // bit for bit.  Nothing here re-implements reference code: each function is a
 // one-line call into the reference template named in its comment.
 #include "AccRQA_CPU_function.hpp"
+#include "AccRQA_utilities_distance.hpp"
+#include "AccRQA_utilities_error.hpp"
+
+// The public RP overloads defined in src/AccRQA_functions.cpp:996-1060 (their
+// header, include/AccRQA_functions.hpp, needs the build system's generated
+// version macros, so the three signatures are declared here).
+void accrqa_RP(char *output, float *input_data, size_t data_size, int tau, int emb, float threshold,
+               Accrqa_Distance distance_type, Accrqa_Error *error);
+void accrqa_RP(char *output, double *input_data, size_t data_size, int tau, int emb, double threshold,
+               Accrqa_Distance distance_type, Accrqa_Error *error);
+void accrqa_RP(int *output, double *input_data, size_t data_size, int tau, int emb, double threshold,
+               Accrqa_Distance distance_type, Accrqa_Error *error);
 
 extern "C" {
 
@@ -33,6 +45,24 @@ int ref_r_element_f32(float *x, long i, long j, float thr, int tau, int emb, int
 }
 int ref_r_element_f64(double *x, long i, long j, double thr, int tau, int emb, int dist) {
     return R_matrix_element<double>(x, (size_t)i, (size_t)j, thr, tau, emb, dist);
+}
+
+// accrqa_RP, src/AccRQA_functions.cpp:996-1060 (the public API's three
+// overloads: char output for float and double input, int output for double)
+int ref_rp_u8_f32(char *out, float *x, long n, int tau, int emb, float thr, int dist) {
+    Accrqa_Error e = SUCCESS;
+    accrqa_RP(out, x, (size_t)n, tau, emb, thr, (Accrqa_Distance)dist, &e);
+    return (int)e;
+}
+int ref_rp_u8_f64(char *out, double *x, long n, int tau, int emb, double thr, int dist) {
+    Accrqa_Error e = SUCCESS;
+    accrqa_RP(out, x, (size_t)n, tau, emb, thr, (Accrqa_Distance)dist, &e);
+    return (int)e;
+}
+int ref_rp_i32_f64(int *out, double *x, long n, int tau, int emb, double thr, int dist) {
+    Accrqa_Error e = SUCCESS;
+    accrqa_RP(out, x, (size_t)n, tau, emb, thr, (Accrqa_Distance)dist, &e);
+    return (int)e;
 }
 
 }  // extern "C"

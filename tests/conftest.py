@@ -29,6 +29,24 @@ def golden():
 
 
 @pytest.fixture(scope="session")
+def golden_rp():
+    with open(os.path.join(GOLDEN, "golden_rp.json")) as f:
+        return json.load(f)
+
+
+def rp_case(case):
+    """(x, tau, emb, thr, dist, output dtype, expected N_c x N_c plot) of a
+    golden accrqa_RP case."""
+    dt = np.float64 if case["dtype"] == "f64" else np.float32
+    x = unhex(case["x"], dt)
+    thr = unhex([case["thr"]], dt)[0]
+    ot = np.int32 if case["out"] == "i32" else np.int8
+    nc = case["nc"]
+    bits = np.unpackbits(np.frombuffer(bytes.fromhex(case["bits"]), dtype=np.uint8))[: nc * nc]
+    return x, case["tau"], case["emb"], thr, case["dist"], ot, bits.reshape(nc, nc).astype(ot)
+
+
+@pytest.fixture(scope="session")
 def crqa_kat():
     with open(os.path.join(GOLDEN, "crqa_kat.json")) as f:
         return json.load(f)

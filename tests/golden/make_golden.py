@@ -113,6 +113,48 @@ def transcribe_crqa_kats():
         json.dump(kat, f, indent=1)
 
 
+def rp_record(R, name, x, tau, emb, thr, dist, out_dtype=np.int8):
+    """One accrqa_RP call: the plot's bits packed row-major (np.packbits of
+    plot != 0, hex) and its sum."""
+    dt = x.dtype
+    thr = dt.type(thr)
+    rp, e = R.RP(x, tau, emb, thr, dist, out_dtype)
+    assert e == 0, (name, e)
+    assert set(np.unique(rp)) <= {0, 1}, name
+    return {"name": name, "dtype": "f64" if dt == np.float64 else "f32", "x": hx(x), "tau": int(tau),
+            "emb": int(emb), "thr": float(thr).hex(), "dist": int(dist),
+            "out": "i32" if np.dtype(out_dtype) == np.int32 else "i8", "nc": int(rp.shape[0]),
+            "sum": int(rp.sum(dtype=np.int64)), "bits": np.packbits(rp.reshape(-1) != 0).tobytes().hex()}
+
+
+def rp_cases(R, app):
+    """Recurrence plots of accrqa_RP (its three overloads), both norms."""
+    rng = np.random.default_rng(20261203)
+    cases = []
+    a = app[:260]
+    for dist in (1, 2):
+        cases.append(rp_record(R, f"app_f32_d{dist}", a.astype(np.float32), 1, 3, 0.3, dist))
+        cases.append(rp_record(R, f"app_f64_d{dist}", a.astype(np.float64), 2, 4, 0.4, dist))
+        cases.append(rp_record(R, f"app_f64_i32_d{dist}", a.astype(np.float64), 1, 2, 0.25, dist, np.int32))
+    x = rng.standard_normal(220)
+    x[31] = np.nan
+    x[150] = np.inf
+    for dt in (np.float32, np.float64):
+        for thr in (-1.0, 0.0, 0.8, np.inf):
+            cases.append(rp_record(R, f"special_{np.dtype(dt).name}_{thr}", x.astype(dt), 1, 2, thr, 1))
+    ties = rng.integers(-3, 4, size=240).astype(np.float32)
+    cases.append(rp_record(R, "ties_f32", ties, 1, 3, float(np.sqrt(np.float32(5.0))), 1))
+    cases.append(rp_record(R, "ties_f32_tau0", ties, 0, 3, 2.0, 1))
+    cases.append(rp_record(R, "walk_f32_m16", np.cumsum(rng.standard_normal(300)).astype(np.float32), 1, 16,
+                           4.0, 1))
+    out = os.path.join(HERE, "golden_rp.json")
+    with open(out, "w") as f:
+        json.dump({"generator": "tests/golden/make_golden.py",
+                   "source": "reference accrqa_RP (src/AccRQA_functions.cpp:996-1060) compiled by oracle/Makefile",
+                   "cases": cases}, f, separators=(",", ":"))
+    print(f"wrote {len(cases)} RP cases to {out} ({os.path.getsize(out) / 1e3:.0f} kB)")
+
+
 def main():
     transcribe_crqa_kats()
     if not Reference.available():
@@ -183,6 +225,7 @@ def main():
                              "compiled by oracle/Makefile",
                    "cases": cases}, f, separators=(",", ":"))
     print(f"wrote {len(cases)} cases to {out} ({os.path.getsize(out) / 1e3:.0f} kB)")
+    rp_cases(R, app)
 
 
 if __name__ == "__main__":

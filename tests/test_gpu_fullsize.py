@@ -134,3 +134,31 @@ def test_config5_million(forms):
     assert_same(ex, mx, "config5 exact vs mfma")
     say("config5 counts")
     check_counts(amd, x, tau, emb, thr, 1, mx, "config5")
+
+
+def test_rp_large(forms):
+    """The recurrence plot at N_c = 24,000 (576 MB) on the default workload's
+    series: symmetric, ones on the diagonal, and its total equals the
+    recurrence count of the separate RR kernel for the same threshold, in
+    both norms."""
+    import torch
+    amd = forms
+    c = W.config3(m=16)
+    x = np.ascontiguousarray(c["x"][:24015])
+    thr = float(c["thr"][5])
+    dev = torch.device("cuda:0")
+    dx = torch.from_numpy(x).to(dev)
+    nc = x.size - 15
+    dout = torch.empty(nc * nc, dtype=torch.int8, device=dev)
+    for dist in (1, 2):
+        say("rp", nc, dist)
+        amd.rp_device(dx.data_ptr(), x.size, 1, 16, thr, dist, dout.data_ptr())
+        torch.cuda.synchronize()
+        rp = dout.view(nc, nc)
+        assert torch.equal(rp, rp.t())
+        assert bool((torch.diagonal(rp) == 1).all())
+        total = int(rp.sum(dtype=torch.int64).item())
+        rr = hists(amd, x, 1, 16, [thr], dist, which=1)["rr"]
+        assert total == int(rr[0]), (dist, total, int(rr[0]))
+    del dout
+    torch.cuda.empty_cache()

@@ -478,3 +478,77 @@ def test_mfma_decision_counters(mfma_form, oracle):
     assert exact < ranked // 100, (ranked, exact)
     for k, t in enumerate(thr):
         assert np.array_equal(got["lam"][k], oracle.hist("lam", x, np.float32(t), tau, m, 1)), t
+
+
+def test_rp_golden(amd, golden_rp):
+    """accrqa_RP on the GPU against the plots the compiled reference wrote."""
+    from conftest import rp_case
+    for case in golden_rp["cases"]:
+        say("rp golden", case["name"])
+        x, tau, emb, thr, dist, ot, want = rp_case(case)
+        got = amd.RP(x, tau, emb, thr, dist, ot)
+        assert got.dtype == want.dtype and np.array_equal(got, want), case["name"]
+
+
+def test_rp_against_oracle(amd, oracle):
+    """The GPU plot against the oracle's, bit for bit: both norms, fp32 and
+    fp64 (char and int output), N_c around the 32-row and 64-column block
+    edges, tau = 0, m = 1 and m = 20, NaN and inf samples, ties, and
+    thresholds that never recur."""
+    rng = np.random.default_rng(41)
+    cases = []
+    for n, tau, emb in ((31, 1, 1), (64, 1, 2), (97, 2, 3), (130, 0, 4), (700, 3, 20), (1025, 1, 5)):
+        cases.append((rng.standard_normal(n), tau, emb))
+    w = rng.standard_normal(400)
+    w[[7, 300]] = np.nan
+    w[100] = np.inf
+    cases.append((w, 1, 3))
+    cases.append((rng.integers(-3, 4, size=500).astype(np.float64), 2, 2))
+    for x64, tau, emb in cases:
+        for dt in (np.float32, np.float64):
+            x = x64.astype(dt)
+            for dist in (1, 2):
+                for thr in (0.0, 0.6, 2.0, -1.0):
+                    say("rp", x.size, tau, emb, dt.__name__, dist, thr)
+                    want, e = oracle.RP(x, tau, emb, dt(thr), dist)
+                    assert e == 0
+                    got = amd.RP(x, tau, emb, dt(thr), dist)
+                    assert np.array_equal(got, want), (x.size, tau, emb, dt, dist, thr)
+                    if dt == np.float64 and thr == 0.6:
+                        want32, _ = oracle.RP(x, tau, emb, dt(thr), dist, np.int32)
+                        assert np.array_equal(amd.RP(x, tau, emb, dt(thr), dist, np.int32), want32)
+
+
+def test_rp_cxx_device_and_errors(amd, oracle):
+    """The reference's mangled accrqa_RP overload (error preset to SUCCESS, as
+    the reference requires; a set error makes it do nothing), the device
+    entry point on torch buffers, and argument errors."""
+    import ctypes as C
+    import torch
+    L = amd.load()
+    rng = np.random.default_rng(43)
+    x = rng.standard_normal(600).astype(np.float32)
+    want, _ = oracle.RP(x, 2, 3, np.float32(0.7), 1)
+    f = getattr(L, "_Z9accrqa_RPPcPfmiif15Accrqa_DistanceP12Accrqa_Error")
+    f.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_int, C.c_float, C.c_int, C.c_void_p]
+    f.restype = None
+    out = np.full(want.shape, 7, np.int8)
+    err = C.c_int(0)
+    f(out.ctypes.data, x.ctypes.data, x.size, 2, 3, 0.7, 1, C.byref(err))
+    assert err.value == 0 and np.array_equal(out, want)
+    out[:] = 7
+    err = C.c_int(2)
+    f(out.ctypes.data, x.ctypes.data, x.size, 2, 3, 0.7, 1, C.byref(err))
+    assert err.value == 2 and np.all(out == 7)
+    dev = torch.device("cuda:0")
+    dx = torch.from_numpy(x).to(dev)
+    nc = want.shape[0]
+    dout = torch.empty(nc * nc, dtype=torch.int8, device=dev)
+    amd.rp_device(dx.data_ptr(), x.size, 2, 3, 0.7, 1, dout.data_ptr())
+    torch.cuda.synchronize()
+    assert np.array_equal(dout.cpu().numpy().reshape(nc, nc), want)
+    with pytest.raises(amd.AccrqaError):
+        amd.RP(x, 1, 700, 0.5, 1)
+    with pytest.raises(amd.AccrqaError):
+        amd.RP(x, 1, 3, 0.5, 3)
+

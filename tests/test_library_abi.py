@@ -23,6 +23,10 @@ CXX_SYMBOLS = [
     "_Z34accrqa_LAM_output_size_in_elementsiiii",
     "_Z33accrqa_RR_output_size_in_elementsiii",
     "_Z18accrqa_print_errorP12Accrqa_Error",
+    "_Z9accrqa_RPPcPfmiif15Accrqa_DistanceP12Accrqa_Error",
+    "_Z9accrqa_RPPcPdmiid15Accrqa_DistanceP12Accrqa_Error",
+    "_Z9accrqa_RPPiPdmiid15Accrqa_DistanceP12Accrqa_Error",
+    "_Z33accrqa_RP_output_size_in_elementslll",
 ]
 
 

@@ -216,3 +216,64 @@ def test_reference_library_agrees_when_present(oracle):
         a, _ = oracle.DET(x, [1, 2], [2], [2, 3], [0.4, 0.9], 1)
         b, _ = R.DET(x, [1, 2], [2], [2, 3], [0.4, 0.9], 1)
         assert same_bits(a, b)
+
+
+def test_golden_rp(oracle, golden_rp):
+    """accrqa_RP's plots, written by the compiled reference: the restatement
+    gives them bit for bit (both norms, char and int output, NaN/inf samples,
+    negative / zero / infinite thresholds, ties, tau = 0, m = 16)."""
+    from conftest import rp_case
+    assert len(golden_rp["cases"]) >= 15
+    for case in golden_rp["cases"]:
+        x, tau, emb, thr, dist, ot, want = rp_case(case)
+        got, e = oracle.RP(x, tau, emb, thr, dist, ot)
+        assert e == 0 and got.dtype == want.dtype, case["name"]
+        assert np.array_equal(got, want), case["name"]
+        assert int(got.sum(dtype=np.int64)) == case["sum"], case["name"]
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_rp_invariants(oracle, dt):
+    """Properties the plot must have whatever the arithmetic: symmetric
+    (RN(a - b) = -RN(b - a)), ones on the diagonal of finite rows for a
+    threshold >= 0 and zeros on NaN rows in the Euclidean norm (the maximal
+    norm skips NaN components, `if (d > mx)`, so its diagonal is all ones),
+    all zero for a negative or NaN threshold, total = the
+    recurrence count that the RR fixtures pin, and each entry = R(i, j)."""
+    rng = np.random.default_rng(23)
+    x = rng.standard_normal(180).astype(dt)
+    x[50] = np.nan
+    for dist in (1, 2):
+        for tau, emb, thr in ((1, 3, 0.7), (2, 2, 0.3), (0, 4, 0.1)):
+            rp, e = oracle.RP(x, tau, emb, dt(thr), dist)
+            assert e == 0
+            assert np.array_equal(rp, rp.T)
+            nc = rp.shape[0]
+            finite = np.array([np.all(np.isfinite(x[i:i + (emb - 1) * tau + 1:max(tau, 1)])) for i in range(nc)])
+            assert np.all(np.diag(rp)[finite] == 1)
+            assert np.all(np.diag(rp)[~finite] == (0 if dist == 1 else 1))
+            assert int(rp.sum(dtype=np.int64)) == oracle.rr_count(x, dt(thr), tau, emb, dist)
+            for i, j in rng.integers(0, nc, size=(40, 2)):
+                assert rp[i, j] == oracle.r_element(x, int(i), int(j), dt(thr), tau, emb, dist)
+        for thr in (-0.5, np.nan):
+            rp, e = oracle.RP(x, 1, 3, dt(thr), dist)
+            assert e == 0 and not rp.any()
+    rp, e = oracle.RP(x, 1, 200, dt(1.0), 1)
+    assert e == 2
+
+
+def test_reference_rp_agrees_when_present(oracle):
+    from pyoracle import Reference
+    if not Reference.available():
+        pytest.skip("oracle/_ref not built here (the reference tree is absent)")
+    R = Reference()
+    rng = np.random.default_rng(29)
+    for dt in (np.float32, np.float64):
+        x = rng.standard_normal(250).astype(dt)
+        for dist in (1, 2):
+            a, _ = oracle.RP(x, 2, 3, dt(0.9), dist)
+            b, _ = R.RP(x, 2, 3, dt(0.9), dist)
+            assert np.array_equal(a, b)
+    a, _ = oracle.RP(x, 1, 2, 0.5, 1, np.int32)
+    b, _ = R.RP(x, 1, 2, 0.5, 1, np.int32)
+    assert np.array_equal(a, b)
