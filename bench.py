@@ -72,10 +72,10 @@ def tile_store_bytes(nc):
 # form and scaled by the same counters' readings for a 1 GiB device copy
 # (FETCH_SIZE reads 0.5 GiB for it: x2.0; WRITE_SIZE 1 GiB: x1.0).
 MEASURED_TRAFFIC = {
-    ("ar1_m16", "k_lam_mx"): 2.0 * 0.9515e9 + 34.371e9,
+    ("ar1_m16", "k_lam_mx"): 2.0 * 0.9561e9 + 34.373e9,
     ("ar1_m16", "k_lam_vscan"): 2.0 * 17.281e9 + 5.266e9,
     ("ar1_m16", "k_lam_hscan"): 2.0 * 17.271e9 + 5.249e9,
-    ("ar1_m16", "k_det_scan"): 2.0 * 25.399e9 + 3.933e9,
+    ("ar1_m16", "k_det_scan"): 2.0 * 25.279e9 + 3.933e9,
 }
 
 
@@ -331,6 +331,49 @@ def config4_loop(accrqa_amd, stride=64):
             "path": "per-series loop through accrqa_amd.RR/DET/LAM (host arrays), every %dth series" % stride}
 
 
+def rp_bench(accrqa_amd, torch, x, thr, reps=3):
+    """accrqa_RP through the device entry point, series and plot in HBM, on a
+    65,536-row plot (4.3 GB of bytes) of the default workload's series, at
+    m = 16 and m = 3 (tau = 1, Euclidean).  k_rp computes every element of
+    the plot and writes one byte per element; the write rate is priced
+    against 8 TB/s.  Beside it, the reference's own accrqa_RP (char output,
+    OpenMP over rows) timed on a 6,000-row prefix."""
+    dev = torch.device("cuda:0")
+    st = torch.cuda.current_stream()
+    res = {}
+    for m in (16, 3):
+        nc = 65536
+        xs = np.ascontiguousarray(x[:nc + m - 1])
+        dx = torch.from_numpy(xs).to(dev)
+        dout = torch.empty(nc * nc, dtype=torch.int8, device=dev)
+        accrqa_amd.rp_device(dx.data_ptr(), xs.size, 1, m, thr, 1, dout.data_ptr(), st.cuda_stream)
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(st)
+        for _ in range(reps):
+            accrqa_amd.rp_device(dx.data_ptr(), xs.size, 1, m, thr, 1, dout.data_ptr(), st.cuda_stream)
+        b.record(st)
+        torch.cuda.synchronize()
+        t = a.elapsed_time(b) / reps / 1e3
+        el = float(nc) * nc
+        res[f"m{m}"] = {"n_c": nc, "ms_per_plot": t * 1e3, "elements_per_s": el / t,
+                        "write_GB_s": el / t / 1e9, "write_frac_of_8TB_s": el / t / 8e12,
+                        "ones": int(dout.sum(dtype=torch.int64).item())}
+        del dout, dx
+        torch.cuda.empty_cache()
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from pyoracle import Oracle, Reference
+    impl, kind = (Reference(), "reference") if Reference.available() else (Oracle(), "port")
+    ns = 6000 + 15
+    xs = np.ascontiguousarray(x[:ns])
+    t0 = time.perf_counter()
+    impl.RP(xs, 1, 16, np.float32(thr), 1)
+    t = time.perf_counter() - t0
+    res["cpu"] = {"kind": kind, "cores": Oracle().num_threads(), "elements_per_s": 6000.0 * 6000 / t,
+                  "sample": f"accrqa_RP, N_c = 6,000 prefix, m = 16, {t:.2f} s wall"}
+    return res
+
+
 def extras(accrqa_amd, L, torch, args, c, x, step):
     """Reported beside the headline, outside its timed region:
     * mfma_decisions: one MFMA-form step with the decision counters on:
@@ -342,7 +385,8 @@ def extras(accrqa_amd, L, torch, args, c, x, step):
       both forms (the MFMA form: FP16 hi/lo Gram of the series rounded to FP32, exact
       re-decision in FP64), outputs compared bit for bit;
     * config4_per_series: the batched configuration's throughput through the
-      public API, one series at a time (config4_loop)."""
+      public API, one series at a time (config4_loop);
+    * rp: the recurrence plot (rp_bench)."""
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream(dev)
     out = {}
@@ -392,6 +436,7 @@ def extras(accrqa_amd, L, torch, args, c, x, step):
     mm, vm, om = timed(*args64, accrqa_amd.FORM_MFMA, np.float64)
     mis = int(sum(int((u.view(np.uint64) != v.view(np.uint64)).sum()) for u, v in zip(oe, om)))
     out["config4_per_series"] = config4_loop(accrqa_amd)
+    out["rp"] = rp_bench(accrqa_amd, torch, x, float(c["thr"][5]))
     out["fp64_input"] = {"exact_ms_per_step": me, "exact_value": ve, "mfma_ms_per_step": mm, "mfma_value": vm,
                          "mfma_vs_exact_speedup": me / mm, "mismatches_mfma_vs_exact": mis,
                          "forms": {"exact": "exact-order FP64 sums on VALU",
