@@ -16,7 +16,7 @@ import os
 
 import numpy as np
 
-__all__ = ["RR", "DET", "LAM", "RP", "rp_device", "histograms", "rqa_device", "set_form", "set_split", "AccrqaError", "library_path",
+__all__ = ["RR", "DET", "LAM", "RP", "rp_device", "RR_target", "histograms", "rqa_device", "set_form", "set_split", "AccrqaError", "library_path",
            "load", "PLT_AMD_GPU", "FORM_AUTO", "FORM_EXACT", "FORM_MFMA", "SPLIT_AUTO", "SPLIT_NEVER",
            "DST_EUCLIDEAN", "DST_MAXIMAL"]
 
@@ -317,3 +317,6 @@ def rqa_device(d_input, n, tau, emb, threshold_values, lmin_values, distance_typ
                                                    C.c_void_p(stream or None))
     if e != 0:
         raise AccrqaError(e, L.accrqa_amd_last_error().decode())
+
+
+from .rr_target import RR_target  # noqa: E402

@@ -244,3 +244,36 @@ class Reference:
                                       _ptr(lm), lm.size, _ptr(thr), thr.size, dist, calc_entr, platform,
                                       C.byref(err))
         return out, err.value
+
+
+def rr_target(oracle, x, tau, emb, target_RR, dist, epsilon=0.01, max_iter=20, threshold_min=0,
+              threshold_max=10):
+    """Restatement of the reference's RR_target (src/accrqa/accrqa.py:470-594):
+    one RR evaluation per bisection step, sequentially, through the oracle's
+    RR.  Returns (threshold, RR at the last evaluated threshold, the list of
+    thresholds evaluated in the loop, in order)."""
+    def rr(thr):
+        out, e = oracle.RR(x, [tau], [emb], np.array(thr, dtype=np.float32).astype(x.dtype), dist)
+        assert e == 0
+        return out.reshape(-1)
+
+    threshold_mid = (threshold_max - threshold_min) / 2.0 + threshold_min
+    v = rr([threshold_min, threshold_mid, threshold_max])
+    low_RR, hgh_RR = v[0], v[2]
+    seen = []
+    current_RR = None
+    for _ in range(1, max_iter + 1):
+        threshold_mid = (threshold_max - threshold_min) / 2.0 + threshold_min
+        seen.append(threshold_mid)
+        current_RR = rr([threshold_mid])[0]
+        if abs(current_RR - target_RR) < epsilon:
+            return threshold_mid, current_RR, seen
+        if (hgh_RR - low_RR) < epsilon:
+            return threshold_mid, current_RR, seen
+        if current_RR < target_RR:
+            threshold_min = threshold_mid
+        else:
+            threshold_max = threshold_mid
+    threshold_mid = (threshold_max - threshold_min) / 2.0 + threshold_min
+    return threshold_mid, current_RR, seen
+

@@ -552,3 +552,20 @@ def test_rp_cxx_device_and_errors(amd, oracle):
     with pytest.raises(amd.AccrqaError):
         amd.RP(x, 1, 3, 0.5, 3)
 
+
+
+def test_rr_target_gpu(amd, oracle):
+    """RR_target on the GPU (speculative bisection, several thresholds per
+    pass) against the oracle's sequential restatement of the reference's
+    loop: the same threshold and the same RR bits, both norms, for searches
+    that stop by epsilon, by the interval test and by max_iter."""
+    from pyoracle import rr_target
+    rng = np.random.default_rng(71)
+    x = np.cumsum(rng.standard_normal(2500)).astype(np.float32)
+    x = (x - x.mean()) / x.std()
+    for target, dist, eps, it, lo, hi in ((0.1, 1, 0.01, 20, 0, 10), (0.3, 2, 1e-4, 20, 0, 10),
+                                          (0.05, 1, 1e-8, 12, 0, 4), (0.2, 1, 0.01, 20, 6, 10)):
+        say("rr_target", target, dist, eps)
+        want_t, want_rr, _ = rr_target(oracle, x, 3, 4, target, dist, eps, it, lo, hi)
+        got_t, got_rr = amd.RR_target(x, 3, 4, target, dist, eps, it, lo, hi)
+        assert got_t == want_t and got_rr.tobytes() == want_rr.tobytes(), (target, dist, eps)

@@ -277,3 +277,25 @@ def test_reference_rp_agrees_when_present(oracle):
     a, _ = oracle.RP(x, 1, 2, 0.5, 1, np.int32)
     b, _ = R.RP(x, 1, 2, 0.5, 1, np.int32)
     assert np.array_equal(a, b)
+
+
+def test_rr_target_restatement(oracle):
+    """The bisection restatement: a returned threshold inside epsilon really
+    has that RR; an interval whose bounds give the same RR returns at the
+    first midpoint; running out of iterations returns the midpoint of the
+    last interval; each step halves the interval toward the target (RR is
+    monotone in the threshold)."""
+    from pyoracle import rr_target
+    rng = np.random.default_rng(67)
+    x = rng.standard_normal(400).astype(np.float32)
+    t, r, seen = rr_target(oracle, x, 1, 3, 0.2, 1, 0.005)
+    out, _ = oracle.RR(x, [1], [3], [np.float32(t)], 1)
+    assert out.reshape(-1)[0] == r and abs(r - 0.2) < 0.005
+    assert seen[0] == 5.0 and all(abs(b - a) <= 5.0 / 2 ** k for k, (a, b) in enumerate(zip(seen, seen[1:])))
+    t, r, seen = rr_target(oracle, x, 1, 3, 0.2, 1, 0.01, 20, 20.0, 30.0)
+    assert seen == [25.0] and t == 25.0 and r == 1.0
+    t, r, seen = rr_target(oracle, x, 1, 3, 0.2, 1, 1e-9, 3)
+    assert seen == [5.0, 2.5, 1.25] and t in (0.625, 1.875)
+    for a, b in zip(seen, seen[1:]):
+        ra, _ = oracle.RR(x, [1], [3], [np.float32(a)], 1)
+        assert (b < a) == (ra.reshape(-1)[0] >= 0.2)
