@@ -374,6 +374,25 @@ def rp_bench(accrqa_amd, torch, x, thr, reps=3):
     return res
 
 
+def rr_target_bench(accrqa_amd, x):
+    """RR_target on a 65,536-row prefix of the default series (tau = 1,
+    m = 16, target RR 0.05, epsilon 1e-5, so that the search runs all 20
+    steps): the speculative bisection (depth 5) against depth 1, which is the
+    reference's one RR pass per step, on the same GPU RR."""
+    xs = np.ascontiguousarray(x[:65536 + 15])
+    res = {"n_c": 65536, "m": 16, "target_RR": 0.05, "epsilon": 1e-5}
+    out = {}
+    for depth in (5, 1):
+        accrqa_amd.RR_target(xs, 1, 16, 0.05, 1, 1e-5, 20, 0, 10, depth=depth)
+        t0 = time.perf_counter()
+        thr, rr = accrqa_amd.RR_target(xs, 1, 16, 0.05, 1, 1e-5, 20, 0, 10, depth=depth)
+        out[depth] = (time.perf_counter() - t0, thr, float(rr))
+    res.update({"seconds_speculative": out[5][0], "seconds_sequential": out[1][0],
+                "speedup": out[1][0] / out[5][0], "same_result": out[5][1:] == out[1][1:],
+                "threshold": out[5][1], "rr": out[5][2]})
+    return res
+
+
 def extras(accrqa_amd, L, torch, args, c, x, step):
     """Reported beside the headline, outside its timed region:
     * mfma_decisions: one MFMA-form step with the decision counters on:
@@ -386,7 +405,8 @@ def extras(accrqa_amd, L, torch, args, c, x, step):
       re-decision in FP64), outputs compared bit for bit;
     * config4_per_series: the batched configuration's throughput through the
       public API, one series at a time (config4_loop);
-    * rp: the recurrence plot (rp_bench)."""
+    * rp: the recurrence plot (rp_bench);
+    * rr_target: the threshold search (rr_target_bench)."""
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream(dev)
     out = {}
@@ -437,6 +457,7 @@ def extras(accrqa_amd, L, torch, args, c, x, step):
     mis = int(sum(int((u.view(np.uint64) != v.view(np.uint64)).sum()) for u, v in zip(oe, om)))
     out["config4_per_series"] = config4_loop(accrqa_amd)
     out["rp"] = rp_bench(accrqa_amd, torch, x, float(c["thr"][5]))
+    out["rr_target"] = rr_target_bench(accrqa_amd, x)
     out["fp64_input"] = {"exact_ms_per_step": me, "exact_value": ve, "mfma_ms_per_step": mm, "mfma_value": vm,
                          "mfma_vs_exact_speedup": me / mm, "mismatches_mfma_vs_exact": mis,
                          "forms": {"exact": "exact-order FP64 sums on VALU",
