@@ -16,7 +16,7 @@ import os
 
 import numpy as np
 
-__all__ = ["RR", "DET", "LAM", "RP", "rp_device", "RR_target", "histograms", "rqa_device", "set_form", "set_split", "AccrqaError", "library_path",
+__all__ = ["RR", "DET", "LAM", "RP", "rp_device", "RR_target", "rqa_batch", "histograms", "rqa_device", "set_form", "set_split", "AccrqaError", "library_path",
            "load", "PLT_AMD_GPU", "FORM_AUTO", "FORM_EXACT", "FORM_MFMA", "SPLIT_AUTO", "SPLIT_NEVER",
            "DST_EUCLIDEAN", "DST_MAXIMAL"]
 
@@ -93,6 +93,10 @@ def load():
         f = getattr(L, f"accrqa_amd_rp_device_{sfx}")
         f.argtypes = [P, SZ, I, I, fl, I, P, P]
         f.restype = I
+    for sfx in ("f32", "f64"):
+        f = getattr(L, f"accrqa_amd_rqa_batch_{sfx}")
+        f.argtypes = [P, P, I, P, I, P, I, P, I, P, I, I, P, P, P, P]
+        f.restype = None
     L.accrqa_amd_RP_f64_i32.argtypes = [P, P, SZ, I, I, C.c_double, I, P]
     L.accrqa_amd_RP_f64_i32.restype = None
     L.accrqa_amd_set_form.argtypes = [I]
@@ -209,6 +213,48 @@ def RR(input_data, tau_values, emb_values, threshold_values, distance_type="eucl
     err = C.c_int(-1)
     getattr(L, f"accrqa_amd_RR_{sfx}")(_p(out), _p(x), x.size, _p(tau), tau.size, _p(emb), emb.size, _p(thr),
                                        thr.size, _dist(distance_type), C.byref(err))
+    _raise(err)
+    return out
+
+
+def rqa_batch(series, tau_values, emb_values, lmin_values, threshold_values, distance_type="euclidean",
+              want=("RR", "DET", "LAM")):
+    """RR / DET / LAM of many series in one call.  series: a list of 1-D
+    arrays of one dtype; threshold_values: [nseries][T] (or one list of T for
+    every series).  Returns {"RR": [nseries, nTaus, nEmbs, T],
+    "DET": [nseries, nTaus, nEmbs, T, nL, 5], "LAM": [nseries, nTaus, nEmbs,
+    nL, T, 5]} for the measures in `want`: the single-series calls' outputs,
+    stacked."""
+    L = load()
+    xs = [np.ascontiguousarray(s) for s in series]
+    dt = xs[0].dtype
+    if any(s.dtype != dt for s in xs):
+        raise TypeError("rqa_batch: every series must have the same dtype")
+    sfx = _sfx(dt)
+    cat = np.ascontiguousarray(np.concatenate(xs))
+    offs = np.zeros(len(xs) + 1, dtype=np.uint64)
+    offs[1:] = np.cumsum([s.size for s in xs])
+    tau, emb, lm = _ints(tau_values), _ints(emb_values), _ints(lmin_values)
+    thr = np.asarray(threshold_values, dtype=dt)
+    if thr.ndim == 1:
+        thr = np.broadcast_to(thr, (len(xs), thr.size))
+    thr = np.ascontiguousarray(thr)
+    if thr.shape[0] != len(xs):
+        raise ValueError("rqa_batch: one row of thresholds per series")
+    T, ns = thr.shape[1], len(xs)
+    out = {}
+    if "RR" in want:
+        out["RR"] = np.zeros((ns, tau.size, emb.size, T), dtype=dt)
+    if "DET" in want:
+        out["DET"] = np.zeros((ns, tau.size, emb.size, T, lm.size, 5), dtype=dt)
+    if "LAM" in want:
+        out["LAM"] = np.zeros((ns, tau.size, emb.size, lm.size, T, 5), dtype=dt)
+    err = C.c_int(-1)
+    getattr(L, f"accrqa_amd_rqa_batch_{sfx}")(_p(cat), _p(offs), ns, _p(tau), tau.size, _p(emb), emb.size, _p(lm),
+                                              lm.size, _p(thr), T, _dist(distance_type),
+                                              _p(out["RR"]) if "RR" in out else None,
+                                              _p(out["DET"]) if "DET" in out else None,
+                                              _p(out["LAM"]) if "LAM" in out else None, C.byref(err))
     _raise(err)
     return out
 

@@ -68,8 +68,8 @@ Ctx g_ctx;
 
 // Work-item tables kept for the last few (nc, seg_len): a call with several
 // (tau, m) blocks, or a loop over many series of a few lengths, alternates
-// between them, and rebuilding costs a host pass, uploads and a stream
-// synchronisation.  Least recently used entry replaced.
+// between them, and rebuilding costs a host pass and five uploads.  Least
+// recently used entry replaced, its new tables uploaded in stream order.
 struct ItemSet {
     int64_t nc = -1;
     int L = -1;
@@ -515,12 +515,14 @@ int build_items(int64_t nc, hipStream_t st) {
         for (size_t i = 0; i < v.size(); i++) it[i] = v[i].second;
         int e2;
         if ((e2 = ensure(buf, std::max<size_t>(it.size(), 1) * sizeof(int2)))) return e2;
-        if (!it.empty()) HIP_TRY(hipMemcpy(buf.p, it.data(), it.size() * sizeof(int2), hipMemcpyHostToDevice));
+        // ordered on the engine's stream: after the queued kernels that may
+        // still read the entry's previous tables, before those of this block
+        // (the runtime stages pageable host memory before it returns)
+        if (!it.empty())
+            HIP_TRY(hipMemcpyAsync(buf.p, it.data(), it.size() * sizeof(int2), hipMemcpyHostToDevice, st));
         n = (int64_t)it.size();
         return E_OK;
     };
-    // the entry replaced may still be read by queued kernels
-    HIP_TRY(hipStreamSynchronize(st));
     S.nc = -1;
     if ((e = upload(rr, S.rr, S.n_rr))) return e;
     if ((e = upload(det, S.det, S.n_det))) return e;
@@ -1048,6 +1050,89 @@ int device_api(const F *d_in, size_t n, int tau, int emb, const F *thr, int T, c
                         d_lam, st);
 }
 
+// Many series in one call: every series uploaded once (each followed by its
+// own PAD zeros, which the kernels' windows read past its end), every
+// (series, tau, emb) block computing RR, DET and LAM together from one pass
+// over its pairs into its part of device outputs, and one copy back.
+// Thresholds are per series ([nseries][T]); outputs are the per-series
+// reference layouts stacked: rr [s][a][b][T], det [s][a][b][T][nl][5],
+// lam [s][a][b][nl][T][5].
+template <typename F>
+void host_batch(const F *series, const size_t *offsets, int nseries, const int *tau, int ntau, const int *emb,
+                int nemb, const int *lm, int nl, const F *thr, int T, int dist, F *rr_out, F *det_out, F *lam_out,
+                int *error) {
+    if (!error) return;
+    *error = E_OK;
+    const bool want_l = det_out || lam_out;
+    if (nseries <= 0 || ntau <= 0 || nemb <= 0 || T <= 0 || (want_l && nl <= 0)) *error = E_INVALID;
+    if (!series || !offsets || !tau || !emb || !thr || (want_l && !lm) || (!rr_out && !det_out && !lam_out))
+        *error = E_NULL;
+    if (*error) return;
+    int e = E_OK;
+    size_t total = 0;
+    for (int k = 0; k < nseries && !e; k++) {
+        if (offsets[k + 1] <= offsets[k]) e = E_INVALID;
+        if (!e) e = check_args(offsets[k + 1] - offsets[k], tau, ntau, emb, nemb, lm, want_l ? nl : 0, dist);
+        total += offsets[k + 1] - offsets[k] + PAD;
+    }
+    if (e) {
+        *error = e;
+        return;
+    }
+    std::lock_guard<std::mutex> lk(g_mu);
+    if ((e = init_ctx())) {
+        *error = e;
+        return;
+    }
+    Ctx &c = g_ctx;
+    hipStream_t st = c.stream;
+    std::vector<F> host(total, F(0));
+    std::vector<size_t> poff(nseries);
+    size_t p = 0;
+    for (int k = 0; k < nseries; k++) {
+        poff[k] = p;
+        std::copy(series + offsets[k], series + offsets[k + 1], host.begin() + (ptrdiff_t)p);
+        p += offsets[k + 1] - offsets[k] + PAD;
+    }
+    const size_t blocks = (size_t)nseries * ntau * nemb;
+    const size_t per_rr = (size_t)T, per_dl = (size_t)T * nl * 5;
+    if ((e = ensure(c.x, total * sizeof(F))) || (e = ensure(c.out_rr, blocks * per_rr * sizeof(F))) ||
+        (e = ensure(c.out_det, blocks * (want_l ? per_dl : 1) * sizeof(F))) ||
+        (e = ensure(c.out_lam, blocks * (want_l ? per_dl : 1) * sizeof(F)))) {
+        *error = e;
+        return;
+    }
+    hipError_t he = hipMemcpyAsync(c.x.p, host.data(), total * sizeof(F), hipMemcpyHostToDevice, st);
+    if (he == hipSuccess) he = hipStreamSynchronize(st);
+    if (he != hipSuccess) {
+        *error = map_err(he);
+        return;
+    }
+    for (int k = 0; k < nseries && !e; k++)
+        for (int a = 0; a < ntau && !e; a++)
+            for (int b = 0; b < nemb && !e; b++) {
+                const size_t blk = ((size_t)k * ntau + a) * nemb + b;
+                e = run_block<F>((const F *)c.x.p + poff[k], offsets[k + 1] - offsets[k], tau[a], emb[b],
+                                 thr + (size_t)k * T, T, want_l ? lm : nullptr, want_l ? nl : 0, dist,
+                                 rr_out ? (F *)c.out_rr.p + blk * per_rr : nullptr,
+                                 det_out ? (F *)c.out_det.p + blk * per_dl : nullptr,
+                                 lam_out ? (F *)c.out_lam.p + blk * per_dl : nullptr, st);
+            }
+    if (e) {
+        (void)hipStreamSynchronize(st);
+        *error = e;
+        return;
+    }
+    he = hipSuccess;
+    if (rr_out) he = hipMemcpyAsync(rr_out, c.out_rr.p, blocks * per_rr * sizeof(F), hipMemcpyDeviceToHost, st);
+    if (he == hipSuccess && det_out)
+        he = hipMemcpyAsync(det_out, c.out_det.p, blocks * per_dl * sizeof(F), hipMemcpyDeviceToHost, st);
+    if (he == hipSuccess && lam_out)
+        he = hipMemcpyAsync(lam_out, c.out_lam.p, blocks * per_dl * sizeof(F), hipMemcpyDeviceToHost, st);
+    if (he == hipSuccess) he = hipStreamSynchronize(st);
+    if (he != hipSuccess) *error = map_err(he);
+}
+
 // The recurrence plot of one (tau, emb, threshold) into d_out (N_c x N_c
 // elements of O, row-major), from the padded series dx.  A threshold that
 // never recurs (negative or NaN) gets bound -1, below every s.
@@ -1244,6 +1329,21 @@ int accrqa_amd_rp_device_f32(const float *d_input, size_t data_size, int tau, in
 int accrqa_amd_rp_device_f64(const double *d_input, size_t data_size, int tau, int emb, double threshold,
                              int distance_type, char *d_output, void *stream) {
     return device_rp<double, char>(d_input, data_size, tau, emb, threshold, distance_type, d_output, stream);
+}
+
+void accrqa_amd_rqa_batch_f32(const float *series, const size_t *offsets, int nSeries, const int *tau_values,
+                              int nTaus, const int *emb_values, int nEmbs, const int *lmin_values, int nL,
+                              const float *threshold_values, int nThresholds, int distance_type, float *rr_out,
+                              float *det_out, float *lam_out, int *error) {
+    host_batch<float>(series, offsets, nSeries, tau_values, nTaus, emb_values, nEmbs, lmin_values, nL,
+                      threshold_values, nThresholds, distance_type, rr_out, det_out, lam_out, error);
+}
+void accrqa_amd_rqa_batch_f64(const double *series, const size_t *offsets, int nSeries, const int *tau_values,
+                              int nTaus, const int *emb_values, int nEmbs, const int *lmin_values, int nL,
+                              const double *threshold_values, int nThresholds, int distance_type, double *rr_out,
+                              double *det_out, double *lam_out, int *error) {
+    host_batch<double>(series, offsets, nSeries, tau_values, nTaus, emb_values, nEmbs, lmin_values, nL,
+                       threshold_values, nThresholds, distance_type, rr_out, det_out, lam_out, error);
 }
 
 int accrqa_amd_mfma16_probe(const uint16_t *a, const uint16_t *b, const float *c, float *d) {

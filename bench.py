@@ -331,6 +331,26 @@ def config4_loop(accrqa_amd, stride=64):
             "path": "per-series loop through accrqa_amd.RR/DET/LAM (host arrays), every %dth series" % stride}
 
 
+def config4_batched(accrqa_amd, stride=64):
+    """The batched configuration on the same every-stride-th sample as
+    config4_loop, through rqa_batch: one call per norm for all the sampled
+    series, each (series, tau, m) block computing RR, DET and LAM from one
+    pass; RR is checked against the per-series API for the first series."""
+    from accrqa_amd import workloads as W
+    c = W.config4(sample=range(0, 4096, stride))
+    xs = c["series"]
+    thr = np.array([[r * s.std() for r in c["thr_rel"]] for s in xs], dtype=np.float32)
+    accrqa_amd.rqa_batch(xs[:2], c["tau"], c["emb"], c["lmin"], thr[:2], 1)
+    t0 = time.perf_counter()
+    outs = [accrqa_amd.rqa_batch(xs, c["tau"], c["emb"], c["lmin"], thr, d) for d in c["norms"]]
+    dt = time.perf_counter() - t0
+    same = bool(np.array_equal(outs[0]["RR"][0], accrqa_amd.RR(xs[0], c["tau"], c["emb"], thr[0], 1)))
+    k = len(xs)
+    return {"sample_series": k, "seconds": dt, "series_per_s": k / dt, "full_config_seconds_est": dt * 4096 / k,
+            "rr_equals_single_series_api": same,
+            "path": "accrqa_amd.rqa_batch, one call per norm over every %dth series" % stride}
+
+
 def rp_bench(accrqa_amd, torch, x, thr, reps=3):
     """accrqa_RP through the device entry point, series and plot in HBM, on a
     65,536-row plot (4.3 GB of bytes) of the default workload's series, at
@@ -405,6 +425,8 @@ def extras(accrqa_amd, L, torch, args, c, x, step):
       re-decision in FP64), outputs compared bit for bit;
     * config4_per_series: the batched configuration's throughput through the
       public API, one series at a time (config4_loop);
+    * config4_batched: the same sample through the batched API
+      (config4_batched);
     * rp: the recurrence plot (rp_bench);
     * rr_target: the threshold search (rr_target_bench)."""
     dev = torch.device("cuda:0")
@@ -456,6 +478,7 @@ def extras(accrqa_amd, L, torch, args, c, x, step):
     mm, vm, om = timed(*args64, accrqa_amd.FORM_MFMA, np.float64)
     mis = int(sum(int((u.view(np.uint64) != v.view(np.uint64)).sum()) for u, v in zip(oe, om)))
     out["config4_per_series"] = config4_loop(accrqa_amd)
+    out["config4_batched"] = config4_batched(accrqa_amd)
     out["rp"] = rp_bench(accrqa_amd, torch, x, float(c["thr"][5]))
     out["rr_target"] = rr_target_bench(accrqa_amd, x)
     out["fp64_input"] = {"exact_ms_per_step": me, "exact_value": ve, "mfma_ms_per_step": mm, "mfma_value": vm,

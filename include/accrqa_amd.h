@@ -103,13 +103,39 @@ void accrqa_amd_histograms_f64(const double *input_data, size_t data_size, int t
  *   d_rr [T], d_det [T][nL][5], d_lam [nL][T][5]   (any may be NULL to skip it).
  * stream: a hipStream_t (NULL = default stream).  Returns the error code.
  * The call is asynchronous with respect to the host except for the small
- * argument uploads; synchronise the stream before reading the outputs. */
+ * argument uploads; synchronise the stream before reading the outputs.
+ * Calls share the library's work buffers, so calls on different streams
+ * must be ordered by the caller (synchronise one before the next). */
 int accrqa_amd_rqa_device_f32(const float *d_input, size_t data_size, int tau, int emb,
                               const float *threshold_values, int nThresholds, const int *lmin_values, int nL,
                               int distance_type, float *d_rr, float *d_det, float *d_lam, void *stream);
 int accrqa_amd_rqa_device_f64(const double *d_input, size_t data_size, int tau, int emb,
                               const double *threshold_values, int nThresholds, const int *lmin_values, int nL,
                               int distance_type, double *d_rr, double *d_det, double *d_lam, void *stream);
+
+/* Many series in one call (no counterpart in the reference, whose API takes
+ * one series per call: the batched configuration's form).  Series k is
+ * series[offsets[k] .. offsets[k+1]) (offsets has nSeries + 1 entries,
+ * increasing); threshold_values is [nSeries][nThresholds] (thresholds per
+ * series).  Outputs are host arrays in the per-series layouts of
+ * accrqa_amd_RR / DET / LAM stacked over the series:
+ *   rr_out  [nSeries][nTaus][nEmbs][nThresholds]
+ *   det_out [nSeries][nTaus][nEmbs][nThresholds][nL][5]
+ *   lam_out [nSeries][nTaus][nEmbs][nL][nThresholds][5]
+ * Any output may be NULL to skip it (not all three).  Each (series, tau,
+ * emb) block computes the requested measures from one pass over its pairs;
+ * the series go up in one copy and the outputs come back in one copy each.
+ * Results are those of the single-series calls, bit for bit.  Errors as
+ * above; a bad argument for any series fails the whole call before any
+ * work. */
+void accrqa_amd_rqa_batch_f32(const float *series, const size_t *offsets, int nSeries, const int *tau_values,
+                              int nTaus, const int *emb_values, int nEmbs, const int *lmin_values, int nL,
+                              const float *threshold_values, int nThresholds, int distance_type, float *rr_out,
+                              float *det_out, float *lam_out, int *error);
+void accrqa_amd_rqa_batch_f64(const double *series, const size_t *offsets, int nSeries, const int *tau_values,
+                              int nTaus, const int *emb_values, int nEmbs, const int *lmin_values, int nL,
+                              const double *threshold_values, int nThresholds, int distance_type, double *rr_out,
+                              double *det_out, double *lam_out, int *error);
 
 /* Recurrence plot.  Replaces the three overloads of accrqa_RP
  * (src/AccRQA_functions.cpp:996-1060, CPU only there:

@@ -569,3 +569,36 @@ def test_rr_target_gpu(amd, oracle):
         want_t, want_rr, _ = rr_target(oracle, x, 3, 4, target, dist, eps, it, lo, hi)
         got_t, got_rr = amd.RR_target(x, 3, 4, target, dist, eps, it, lo, hi)
         assert got_t == want_t and got_rr.tobytes() == want_rr.tobytes(), (target, dist, eps)
+
+
+def test_rqa_batch_matches_single_series(amd, oracle):
+    """The batched call against the single-series calls, bit for bit: series
+    of different lengths (one shorter than a 64-column band, one spanning
+    several segments), thresholds per series (one never recurs), several
+    (tau, emb), both norms, fp32 and fp64, every measure subset; one series
+    also against the oracle directly."""
+    rng = np.random.default_rng(73)
+    lens = [50, 300, 777, 1500, 2100, 64, 5000]
+    taus, embs, lm = [1, 3], [2, 5], [2, 3]
+    for dt in (np.float32, np.float64):
+        xs = [np.cumsum(rng.standard_normal(n)).astype(dt) for n in lens]
+        thr = np.array([[0.1 * s.std(), 0.3 * s.std(), -1.0, 0.7 * s.std()] for s in xs], dtype=dt)
+        for dist in (1, 2):
+            say("batch", dt.__name__, dist)
+            got = amd.rqa_batch(xs, taus, embs, lm, thr, dist)
+            for k, s in enumerate(xs):
+                rr = amd.RR(s, taus, embs, thr[k], dist)
+                det = amd.DET(s, taus, embs, lm, thr[k], dist)
+                lam = amd.LAM(s, taus, embs, lm, thr[k], dist)
+                assert same_bits(got["RR"][k], rr), (dt, dist, k, "RR")
+                assert same_bits(got["DET"][k], det), (dt, dist, k, "DET")
+                assert same_bits(got["LAM"][k], lam), (dt, dist, k, "LAM")
+            only = amd.rqa_batch(xs, taus, embs, lm, thr, dist, want=("RR",))
+            assert same_bits(only["RR"], got["RR"])
+            only = amd.rqa_batch(xs, taus, embs, lm, thr, dist, want=("LAM",))
+            assert same_bits(only["LAM"], got["LAM"])
+        rr_o, e = oracle.RR(xs[2], taus, embs, thr[2], 1)
+        assert e == 0
+        assert same_bits(amd.rqa_batch(xs, taus, embs, lm, thr, 1, want=("RR",))["RR"][2].reshape(-1), rr_o)
+    with pytest.raises(amd.AccrqaError):
+        amd.rqa_batch([np.zeros(10, np.float32), np.zeros(3, np.float32)], [1], [5], [2], [0.1], 1)
