@@ -1412,6 +1412,81 @@ __global__ __launch_bounds__(WG) RQA_WPE_MX(SCAN) void k_lam_mx(const F *__restr
     }
 }
 
+// ---------------------------------------------------------------- k_rp_mx
+// The recurrence plot in the MFMA form (fp32, Euclidean, one threshold): the
+// decisions of k_lam_mx's tiles -- hi/lo Gram, byte table, second stage and
+// exact-order recheck, with the one bound's rank 0 meaning recurrent -- written
+// straight to the row-major plot.  Work item = one block of 32 rows x 64
+// columns, one per wave (both Gram tiles); element e of lane (col, h) of tile
+// p is row 8(e>>2) + 4h + (e&3), column 32p + col, so each store writes two
+// runs of 32 consecutive bytes.  Windows read up to x[N + 62] and nrm[N_c + 62]
+// (both padded with zeros).
+__global__ __launch_bounds__(WG) RQA_WPE_MX(false) void k_rp_mx(const float *__restrict__ x,
+                                                               const float *__restrict__ nrm,
+                                                               const unsigned int *__restrict__ nmax_bits, int64_t nc,
+                                                               int tau, int m, float bound,
+                                                               const uint8_t *__restrict__ lut, MxParams P,
+                                                               char *__restrict__ out) {
+    __shared__ uint8_t lut_s[NBX_LAM];
+    __shared__ float bnd_s[NBND];
+    __shared__ MxWin<2> win[WAVES];
+    for (int e = threadIdx.x; e < P.nb / 4; e += WG) ((uint32_t *)lut_s)[e] = ((const uint32_t *)lut)[e];
+    if (threadIdx.x < NBND) bnd_s[threadIdx.x] = threadIdx.x == 0 ? bound : __int_as_float(0x7F800000);
+    __syncthreads();
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wave = wave_id();
+    const int col = lane & 31;
+    const int h = lane >> 5;
+    const int rspan = 32 + (m - 1) * tau;   // <= MXW (checked by the host)
+    const int cspan = 64 + (m - 1) * tau;
+    MxWin<2> &Wn = win[wave];
+    float *rw = Wn.rw, *cw = Wn.cw;
+    mx_win_init(Wn, lane);
+    const float *pa[8];
+    gram_addrs(Wn, col, h, m, tau, pa);
+    const MxScale Z = mx_scale(nmax_bits, P);
+    const int64_t nbands = (nc + WAVE - 1) / WAVE;
+    const int64_t nblk = ((nc + BLK - 1) / BLK) * nbands;
+    for (int64_t b = (int64_t)blockIdx.x * WAVES + wave; b < nblk; b += (int64_t)gridDim.x * WAVES) {
+        const int64_t I = b / nbands, J = b - I * nbands;
+        const int64_t i0 = I * BLK, j0 = J * WAVE;
+        wave_sync();
+        for (int e = lane; e < cspan; e += WAVE) cw[e] = hl_word(x[j0 + e] * Z.s);
+        for (int e = lane; e < rspan; e += WAVE) rw[e] = hl_word(x[i0 + e] * Z.s);
+        wave_sync();
+        float ni[16];
+        tile_row_norms(nrm, i0, h, ni);
+#pragma unroll
+        for (int p = 0; p < 2; p++) {
+            const floatx16 acc = p == 0 ? gram_tile<2, 0>(Wn, pa, col, h, m, tau) : gram_tile<2, 1>(Wn, pa, col, h, m, tau);
+            uint32_t Rq[4], fm, flw;
+            tile_ranks(acc, ni, nrm[j0 + 32 * p + col], lut_s, P, Z, bnd_s, Rq, fm, flw);
+            const int64_t j = j0 + 32 * p + col;
+            const bool jv = j < nc;
+            // exact re-decision of the ambiguous elements by their own lane
+            uint32_t f = jv ? fm : 0u;
+            while (f) {
+                const uint32_t e = __builtin_ctz(f);
+                f &= f - 1u;
+                const int64_t i = i0 + 8 * (e >> 2) + 4 * h + (e & 3);
+                if (i < nc) {
+                    const float sx = exact_sqdist<float>(x, i, j, tau, m);
+                    const uint32_t rk = rank_direct<float, 1>(sx, bnd_s, 1);
+                    const uint32_t sh = 8 * (e >> 2), c = e & 3;
+                    Rq[c] = (Rq[c] & ~(0xFFu << sh)) | (rk << sh);
+                }
+            }
+            if (jv) {
+#pragma unroll
+                for (int e = 0; e < 16; e++) {
+                    const int64_t i = i0 + 8 * (e >> 2) + 4 * h + (e & 3);
+                    if (i < nc) out[i * nc + j] = ((Rq[e & 3] >> (8 * (e >> 2))) & 0xFFu) == 0u ? 1 : 0;
+                }
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------- k_lam_vscan
 // The column part of LAM in the split form, read back from the rank tiles
 // (the MFMA column kernel then only ranks and stores).  Work item = (band J,

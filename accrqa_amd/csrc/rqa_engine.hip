@@ -1141,8 +1141,38 @@ int rp_run(const F *dx, size_t n, int tau, int emb, F thr, int dist, O *d_out, h
     Ctx &c = g_ctx;
     const int64_t nc = (int64_t)n - (int64_t)(emb - 1) * tau;
     F b = bound_of(thr, dist);
-    if (never_recurs(b)) b = F(-1);
+    const bool never = never_recurs(b);
+    if (never) b = F(-1);
     const int64_t nblk = ((nc + rqa::BLK - 1) / rqa::BLK) * ((nc + rqa::WAVE - 1) / rqa::WAVE);
+    if constexpr (sizeof(F) == 4 && sizeof(O) == 1) {
+        if (!never && dist == 1 && use_mfma<F>(1, emb, tau)) {
+            // MFMA form: norms, the one-bound byte table, then the plot
+            int e;
+            if ((e = ensure(c.nrm, ((size_t)nc + 2 * rqa::WAVE) * sizeof(float)))) return e;
+            if ((e = ensure(c.nmax, 256))) return e;
+            if ((e = ensure(c.lut8, rqa::NBX_LAM))) return e;
+            fill_zero((unsigned long long *)c.nmax.p, 1, st);
+            const int64_t nthr = nc + 2 * rqa::WAVE;
+            hipLaunchKernelGGL((rqa::k_norms<F>), dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, st, dx, nc, tau,
+                               emb, (float *)c.nrm.p, (unsigned int *)c.nmax.p);
+            HIP_TRY(hipGetLastError());
+            rqa::Bounds<float, 4> Bf;
+            for (int t = 0; t < 4; t++) Bf.s[t] = (float)b;
+            const rqa::MxParams MP = mx_params(Bf.s, 1, emb, rqa::NBX_LAM);
+            hipLaunchKernelGGL((rqa::k_mx_lut<4>), dim3((MP.nb + 255) / 256), dim3(256), 0, st, Bf, 1, MP,
+                               (const unsigned int *)c.nmax.p, (uint8_t *)c.lut8.p);
+            HIP_TRY(hipGetLastError());
+            const unsigned g = grid_for(rqa::k_rp_mx, nblk, c.ncu);
+            prof_begin("k_rp_mx", st);
+            hipLaunchKernelGGL(rqa::k_rp_mx, dim3(g), dim3(rqa::WG), 0, st, (const float *)dx, (const float *)c.nrm.p,
+                               (const unsigned int *)c.nmax.p, nc, tau, emb, (float)b, (const uint8_t *)c.lut8.p, MP,
+                               (char *)d_out);
+            prof_end(st);
+            HIP_TRY(hipGetLastError());
+            prof_collect(st);
+            return E_OK;
+        }
+    }
     prof_begin("k_rp", st);
     if (dist == 1) {
         const unsigned g = grid_for(rqa::k_rp<F, 1, O>, nblk, c.ncu);

@@ -602,3 +602,32 @@ def test_rqa_batch_matches_single_series(amd, oracle):
         assert same_bits(amd.rqa_batch(xs, taus, embs, lm, thr, 1, want=("RR",))["RR"][2].reshape(-1), rr_o)
     with pytest.raises(amd.AccrqaError):
         amd.rqa_batch([np.zeros(10, np.float32), np.zeros(3, np.float32)], [1], [5], [2], [0.1], 1)
+
+
+def test_rp_mfma_form(amd, oracle):
+    """The recurrence plot through the MFMA form (hi/lo Gram, one-bound
+    table, exact recheck) against the oracle, bit for bit, also with the form
+    forced for m < 8 and off: walks, ties on attainable distances, a spike
+    10^6 times the rest, a NaN sample (every pair re-decided exactly), tau = 0,
+    m = 33, N_c at the block edges."""
+    rng = np.random.default_rng(79)
+    w = np.cumsum(rng.standard_normal(700)).astype(np.float32)
+    w = (w - w.mean()) / w.std()
+    spike = (1e-3 * rng.standard_normal(500)).astype(np.float32)
+    spike[222] = 1e3
+    nan = rng.standard_normal(400).astype(np.float32)
+    nan[99] = np.nan
+    ties = rng.integers(-3, 4, size=450).astype(np.float32)
+    cases = [(w, 1, 16, 1.5), (w[:96 + 15], 1, 16, 2.0), (w[:161], 2, 9, 0.8), (ties, 1, 8, float(np.sqrt(np.float32(6)))),
+             (spike, 1, 9, 4e-3), (nan, 1, 10, 2.5), (w, 0, 12, 0.3), (w, 2, 33, 4.0), (w, 1, 3, 0.4)]
+    try:
+        for form in (amd.FORM_MFMA, amd.FORM_AUTO, amd.FORM_EXACT):
+            amd.set_form(form)
+            for x, tau, emb, thr in cases:
+                say("rp mfma", form, x.size, tau, emb, thr)
+                want, e = oracle.RP(x, tau, emb, np.float32(thr), 1)
+                assert e == 0
+                got = amd.RP(x, tau, emb, np.float32(thr), 1)
+                assert np.array_equal(got, want), (form, x.size, tau, emb, thr)
+    finally:
+        amd.set_form(amd.FORM_AUTO)
