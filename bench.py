@@ -32,12 +32,14 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# MI355X: 256 CUs x 64 FP32 lanes x 2.4 GHz.  The decision arithmetic is
-# exact-order sub/mul/add/compare (no FMA) and packed-FP32 instruction
-# selection is disabled in this build, so one FP32 op per lane per clock is
-# the ceiling.  The published 157.3 TFLOP/s counts an FMA as 2 and packed
-# dual-issue FP32 as 2.
-VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
+# MI355X FP32 VALU: two wave64 instructions per CU per clock (128 lanes x
+# 256 CUs x 2.4 GHz = 78.6 T lane-ops/s), the rate an add/mul probe with
+# VGPR operands measures (120.8 lane-ops per CU per clock); the published
+# 157.3 TFLOP/s is that rate with an FMA counted as 2.  The decision
+# arithmetic is exact-order sub/mul/add/compare (no FMA, no packed FP32), so
+# one op per lane-slot is the ceiling.  An instruction with an SGPR operand
+# issues at half that rate (the same probe: 62 lane-ops per CU per clock).
+VALU_PEAK_TOPS = 256 * 128 * 2.4e9 / 1e12
 PUBLISHED_FP32_TFLOPS = 157.3
 PUBLISHED_FP16_TFLOPS = 2500.0   # dense FP16 MFMA
 
@@ -109,7 +111,7 @@ def kernel_rooflines(kern, nc, emb, dist, workload):
             w = pairs * (3 * emb if dist == 1 else 2 * emb)
             out[k] = {"bound": "valu", "achieved": w / t / 1e12, "peak": VALU_PEAK_TOPS, "unit": "TFLOP/s",
                       "work_per_launch": w, "work_unit": "FLOP (exact-order sum per distinct pair)",
-                      "peak_note": "FP32 VALU non-FMA issue rate (sub/mul/add in order, no FMA)"}
+                      "peak_note": "FP32 VALU non-FMA issue rate, VGPR operands (sub/mul/add in order, no FMA)"}
         elif k in ("k_lam_vscan", "k_lam_hscan", "k_det_scan"):
             w = float(tile_store_bytes(nc))
             out[k] = {"bound": "hbm", "achieved": w / t / 1e9, "peak": 8000.0, "unit": "GB/s",
