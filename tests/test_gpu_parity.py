@@ -631,3 +631,33 @@ def test_rp_mfma_form(amd, oracle):
                 assert np.array_equal(got, want), (form, x.size, tau, emb, thr)
     finally:
         amd.set_form(amd.FORM_AUTO)
+
+
+def test_compat_module_on_gpu(amd, oracle):
+    """accrqa_amd.compat (the reference's Python interface) on the GPU: its
+    cubes are the oracle's flat outputs in the reference's shapes, bit for
+    bit, and RP / RR_target go through."""
+    import accrqa_amd.compat as accrqa
+    rng = np.random.default_rng(89)
+    x = np.cumsum(rng.standard_normal(900)).astype(np.float32)
+    x = ((x - x.mean()) / x.std()).astype(np.float32)
+    for name in ("euclidean", "maximal"):
+        d = accrqa.accrqaDistance(name)
+        did = d.get_distance_id()
+        rr = accrqa.RR(x, [1, 3], [2, 4], [0.2, 0.5], d, tidy_data=False)
+        rr_o, _ = oracle.RR(x, [1, 3], [2, 4], [0.2, 0.5], did)
+        assert same_bits(rr.reshape(-1), rr_o.reshape(-1))
+        for kind in ("DET", "LAM"):
+            got = getattr(accrqa, kind)(x, [1, 3], [2, 4], [2, 3], [0.2, 0.5], d, tidy_data=False)
+            exp, _ = getattr(oracle, kind)(x, [1, 3], [2, 4], [2, 3], [0.2, 0.5], did)
+            exp = np.asarray(exp).reshape(-1, 5)
+            g = got.reshape(-1, 5)
+            for k in (0, 1, 2, 4):
+                assert same_bits(g[:, k].copy(), exp[:, k].copy()), (name, kind, k)
+        tidy = accrqa.DET(x, [1], [2], [2], [0.3], d)
+        assert len(tidy) == 1 and tidy["Lmin"].iloc[0] == 2
+    rp = accrqa.RP(x[:300], 1, 3, 0.4, accrqa.accrqaDistance("euclidean"))
+    rp_o, _ = oracle.RP(x[:300], 1, 3, np.float32(0.4), 1)
+    assert np.array_equal(rp, rp_o)
+    t, r = accrqa.RR_target(x, 1, 3, 0.1, accrqa.accrqaDistance("euclidean"))
+    assert abs(r - 0.1) < 0.01 or t > 0
