@@ -185,8 +185,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the sharded form across GPUs "
-              "is described in DESIGN.md but not built.", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); a run sharded across GPUs "
+              "is not built.", file=sys.stderr)
         sys.exit(2)
 
     import torch
