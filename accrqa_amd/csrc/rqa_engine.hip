@@ -45,7 +45,9 @@ std::string g_last_error;
 struct DevBuf {
     void *p = nullptr;
     size_t cap = 0;
+    uint64_t gen = 0;   // allocation number (a new one for every hipMalloc)
 };
+uint64_t g_gen = 0;
 
 struct Ctx {
     int device = -1;
@@ -132,6 +134,25 @@ int ensure(DevBuf &b, size_t bytes) {
     }
     HIP_TRY(hipMalloc(&b.p, bytes));
     b.cap = bytes;
+    b.gen = ++g_gen;
+    return E_OK;
+}
+
+// A small read-only device table uploaded only when its bytes change: the
+// (tau, emb) blocks of one call, and the calls of a loop over series, reuse
+// the same threshold permutation, l_min list and bucket tables.  Kept per
+// allocation, so a reallocated buffer is always filled again.
+struct Uploaded {
+    uint64_t gen = 0;
+    std::vector<char> bytes;
+};
+Uploaded g_up_perm, g_up_lmins, g_up_lut;
+
+int upload_once(const DevBuf &b, Uploaded &u, const void *src, size_t bytes, hipStream_t st) {
+    if (u.gen == b.gen && u.bytes.size() == bytes && std::memcmp(u.bytes.data(), src, bytes) == 0) return E_OK;
+    HIP_TRY(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, st));
+    u.gen = b.gen;
+    u.bytes.assign((const char *)src, (const char *)src + bytes);
     return E_OK;
 }
 
@@ -853,11 +874,11 @@ int run_block(const F *dx, size_t n, int tau, int emb, const F *thr, int T, cons
     if ((e = ensure(c.lmins, (size_t)std::max(nl, 1) * sizeof(int)))) return e;
     if ((e = ensure(c.lut, luts.size() * sizeof(rqa::LutEntry<F>)))) return e;
     if ((e = ensure(c.counts, 64 * sizeof(unsigned long long)))) return e;
-    HIP_TRY(hipMemcpyAsync(c.perm.p, order.data(), (size_t)T * sizeof(int), hipMemcpyHostToDevice, st));
-    if (nl > 0) HIP_TRY(hipMemcpyAsync(c.lmins.p, lmins, (size_t)nl * sizeof(int), hipMemcpyHostToDevice, st));
-    if (ngroups > 0)
-        HIP_TRY(hipMemcpyAsync(c.lut.p, luts.data(), (size_t)ngroups * rqa::NB_MAX * sizeof(rqa::LutEntry<F>),
-                               hipMemcpyHostToDevice, st));
+    if ((e = upload_once(c.perm, g_up_perm, order.data(), (size_t)T * sizeof(int), st))) return e;
+    if (nl > 0 && (e = upload_once(c.lmins, g_up_lmins, lmins, (size_t)nl * sizeof(int), st))) return e;
+    if (ngroups > 0 && (e = upload_once(c.lut, g_up_lut, luts.data(),
+                                        (size_t)ngroups * rqa::NB_MAX * sizeof(rqa::LutEntry<F>), st)))
+        return e;
     if ((e = build_items(nc, st))) return e;
     if (use_mfma<F>(dist, emb, tau) && ngroups > 0) {
         if ((e = ensure(c.nrm, ((size_t)nc + 2 * rqa::WAVE) * sizeof(float)))) return e;
