@@ -53,6 +53,11 @@ constexpr int WAVES = WG / WAVE;
 #ifndef RQA_WPE_MX
 #define RQA_WPE_MX(scan) __attribute__((amdgpu_waves_per_eu((scan) ? 3 : 4)))
 #endif
+// The tile scans up to 16 thresholds: 7 waves per SIMD for the LAM scans
+// (<= 73 VGPRs) and 6 for k_det_scan (<= 85), one more than the compiler's
+// own choice (measured: -0.8 ms each on the default workload); 32
+// thresholds keep the occupancy their registers give.
+#define RQA_WPE_SCAN(tg, n, n32) __attribute__((amdgpu_waves_per_eu((tg) == 32 ? (n32) : (n))))
 constexpr int DET_WIN = 992;       // LDS samples per wave for the k_det lane window
 
 template <typename F, int TG>
@@ -543,7 +548,7 @@ constexpr int DTILES = WAVES + 2;    // tiles of one block of a strip group
 constexpr int DRECS = DTILES * WAVE; // their records (tile lanes)
 
 template <int TG>
-__global__ __launch_bounds__(WG) void k_det_scan(const uint8_t *__restrict__ tiles, int64_t nbands, int64_t nc, int nt,
+__global__ __launch_bounds__(WG) RQA_WPE_SCAN(TG, 6, 3) void k_det_scan(const uint8_t *__restrict__ tiles, int64_t nbands, int64_t nc, int nt,
                                                  int seg_len, int nseg, const int2 *__restrict__ items,
                                                  int64_t nitems, unsigned long long *__restrict__ hist,
                                                  uint2 *__restrict__ carry, uint32_t *__restrict__ priv_base) {
@@ -650,7 +655,7 @@ __global__ __launch_bounds__(WG) void k_det_scan(const uint8_t *__restrict__ til
 // of column band Jc) of tiles (2J', Jc) and (2J'+1, Jc); the wave copies them
 // to LDS and lane l reads byte (l%32) of each record of its row block.
 template <int TG>
-__global__ __launch_bounds__(WG) void k_lam_hscan(const uint8_t *__restrict__ tiles, int64_t nbands, int64_t nc, int nt,
+__global__ __launch_bounds__(WG) RQA_WPE_SCAN(TG, 7, 4) void k_lam_hscan(const uint8_t *__restrict__ tiles, int64_t nbands, int64_t nc, int nt,
                                                   int seg_len, int nseg, const int2 *__restrict__ items,
                                                   int64_t nitems, unsigned long long *__restrict__ hist,
                                                   uint2 *__restrict__ carry, uint32_t *__restrict__ priv_base) {
@@ -1496,7 +1501,7 @@ __global__ __launch_bounds__(WG) RQA_WPE_MX(false) void k_rp_mx(const float *__r
 // exchange between lanes.  The next block's record is in flight while a
 // block is scanned.
 template <int TG>
-__global__ __launch_bounds__(WG) void k_lam_vscan(const uint8_t *__restrict__ tiles, int64_t nbands, int64_t nc,
+__global__ __launch_bounds__(WG) RQA_WPE_SCAN(TG, 7, 4) void k_lam_vscan(const uint8_t *__restrict__ tiles, int64_t nbands, int64_t nc,
                                                   int nt, int seg_len, LamWork W,
                                                   unsigned long long *__restrict__ hist, uint2 *__restrict__ carry,
                                                   uint32_t *__restrict__ priv_base) {
