@@ -44,18 +44,21 @@ constexpr int WAVE = 64;
 constexpr int BLK = 32;            // steps per decision word
 constexpr int WG = 256;            // threads per workgroup (4 waves)
 constexpr int WAVES = WG / WAVE;
-#ifndef RQA_WPE_LAM
-#define RQA_WPE_LAM
-#endif
+// k_lam, fp32 Euclidean up to 16 thresholds: 4 waves per SIMD (128 VGPRs, a
+// few bytes spilled) instead of the compiler's 3 (measured: 61.8 -> 56.0 ms
+// on the default workload); the other instantiations keep what their
+// registers give (1 = no minimum).
+#define RQA_WPE_LAM(F, NORM, TG) \
+    __attribute__((amdgpu_waves_per_eu((sizeof(F) == 4 && (NORM) == 1 && (TG) <= 16) ? 4 : 1)))
 // k_lam_mx, k_det_mx: three waves per SIMD with the line scan (168 VGPRs for
 // 16 thresholds, no spills; left to itself the compiler takes a few registers
 // more and halves that), four without it (the LDS allows four workgroups).
 #ifndef RQA_WPE_MX
 #define RQA_WPE_MX(scan) __attribute__((amdgpu_waves_per_eu((scan) ? 3 : 4)))
 #endif
-// The tile scans up to 16 thresholds: 7 waves per SIMD for the LAM scans
-// (<= 73 VGPRs) and 6 for k_det_scan (<= 85), one more than the compiler's
-// own choice (measured: -0.8 ms each on the default workload); 32
+// The tile scans up to 16 thresholds: 7 waves per SIMD (<= 73 VGPRs), one
+// or two more than the compiler's own choice (measured on the default
+// workload: -0.8 ms for each LAM scan, -1.9 ms for k_det_scan); 32
 // thresholds keep the occupancy their registers give.
 #define RQA_WPE_SCAN(tg, n, n32) __attribute__((amdgpu_waves_per_eu((tg) == 32 ? (n32) : (n))))
 constexpr int DET_WIN = 992;       // LDS samples per wave for the k_det lane window
@@ -358,7 +361,7 @@ struct LamWork {
 // The row window x[i0 + u + k*tau] (u < 32) is uniform across the wave and
 // comes through scalar loads; x[j + k*tau] is one vector load per k and block.
 template <typename F, int NORM, int TG>
-__global__ __launch_bounds__(WG) RQA_WPE_LAM void k_lam(const F *__restrict__ x, int64_t nc, int tau, int m, Bounds<F, TG> S,
+__global__ __launch_bounds__(WG) RQA_WPE_LAM(F, NORM, TG) void k_lam(const F *__restrict__ x, int64_t nc, int tau, int m, Bounds<F, TG> S,
                                             int nt, const LutEntry<F> *__restrict__ lut, LutParams P, int seg_len,
                                             LamWork W, unsigned long long *__restrict__ hist,
                                             uint2 *__restrict__ carry, uint32_t *__restrict__ priv_base,
@@ -548,7 +551,7 @@ constexpr int DTILES = WAVES + 2;    // tiles of one block of a strip group
 constexpr int DRECS = DTILES * WAVE; // their records (tile lanes)
 
 template <int TG>
-__global__ __launch_bounds__(WG) RQA_WPE_SCAN(TG, 6, 3) void k_det_scan(const uint8_t *__restrict__ tiles, int64_t nbands, int64_t nc, int nt,
+__global__ __launch_bounds__(WG) RQA_WPE_SCAN(TG, 7, 3) void k_det_scan(const uint8_t *__restrict__ tiles, int64_t nbands, int64_t nc, int nt,
                                                  int seg_len, int nseg, const int2 *__restrict__ items,
                                                  int64_t nitems, unsigned long long *__restrict__ hist,
                                                  uint2 *__restrict__ carry, uint32_t *__restrict__ priv_base) {
