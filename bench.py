@@ -74,10 +74,10 @@ def tile_store_bytes(nc):
 # form and scaled by the same counters' readings for a 1 GiB device copy
 # (FETCH_SIZE reads 0.5 GiB for it: x2.0; WRITE_SIZE 1 GiB: x1.0).
 MEASURED_TRAFFIC = {
-    ("ar1_m16", "k_lam_mx"): 2.0 * 0.9554e9 + 34.373e9,
+    ("ar1_m16", "k_lam_mx"): 2.0 * 0.9557e9 + 34.373e9,
     ("ar1_m16", "k_lam_vscan"): 2.0 * 17.306e9 + 5.311e9,
     ("ar1_m16", "k_lam_hscan"): 2.0 * 17.287e9 + 5.291e9,
-    ("ar1_m16", "k_det_scan"): 2.0 * 25.418e9 + 3.974e9,
+    ("ar1_m16", "k_det_scan"): 2.0 * 25.418e9 + 4.015e9,
 }
 
 
