@@ -97,6 +97,12 @@ def load():
         f = getattr(L, f"accrqa_amd_rqa_batch_{sfx}")
         f.argtypes = [P, P, I, P, I, P, I, P, I, P, I, I, P, P, P, P]
         f.restype = None
+    L.accrqa_amd_fp64_mismatch_f32.argtypes = [P, SZ, I, I, P, I, I, P, P]
+    L.accrqa_amd_fp64_mismatch_f32.restype = I
+    L.accrqa_amd_profile.argtypes = [I]
+    L.accrqa_amd_profile.restype = None
+    L.accrqa_amd_profile_read.argtypes = [C.c_char_p, P, P]
+    L.accrqa_amd_profile_read.restype = I
     L.accrqa_amd_RP_f64_i32.argtypes = [P, P, SZ, I, I, C.c_double, I, P]
     L.accrqa_amd_RP_f64_i32.restype = None
     L.accrqa_amd_set_form.argtypes = [I]
@@ -172,6 +178,21 @@ def mfma16_probe(a, b, c):
     if e:
         raise AccrqaError(e, "mfma16_probe")
     return d
+
+
+def fp64_mismatches(d_input, n, tau, emb, threshold_values, distance_type="euclidean", stream=0):
+    """Diagnostic for fp32 series in device memory (d_input: device pointer):
+    per threshold, the pairs of the full matrix whose fp32 decision (the
+    reference's arithmetic) differs from the decision in fp64 on the same
+    samples.  Returns a uint64 array in the thresholds' order."""
+    L = load()
+    thr = np.ascontiguousarray(np.asarray(threshold_values, dtype=np.float32).reshape(-1))
+    out = np.zeros(thr.size, dtype=np.uint64)
+    e = L.accrqa_amd_fp64_mismatch_f32(C.c_void_p(d_input), n, int(tau), int(emb), _p(thr), thr.size,
+                                       _dist(distance_type), _p(out), C.c_void_p(stream or None))
+    if e != 0:
+        raise AccrqaError(e, L.accrqa_amd_last_error().decode())
+    return out
 
 
 def _dist(d):

@@ -2,6 +2,10 @@
 
     python -m accrqa_amd.build          # product library + oracle (+ oracle/_ref where the reference exists)
 
+oracle/_ref also receives the reference library with the AMD platform branch
+(oracle/ref_amd.py), linked against the product library, for the integration
+test; it is test infrastructure like the rest of oracle/.
+
 The product library is compiled with hipcc for gfx950 only, with floating-point
 contraction off (the exact decision path relies on one rounding per operation)
 and packed-FP32 instruction selection disabled.
@@ -14,7 +18,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libaccrqa_amd.so")
-SOURCES = [os.path.join(HERE, "csrc", "rqa_engine.hip"), os.path.join(HERE, "csrc", "accrqa_api.cpp")]
+SOURCES = [os.path.join(HERE, "csrc", n) for n in ("rqa_engine.hip", "rqa_batch.hip", "accrqa_api.cpp")]
+OBJDIR = os.path.join(HERE, "_build")
 REF_TREE = "/root/reference"
 
 HIPCC_FLAGS = [
@@ -22,7 +27,6 @@ HIPCC_FLAGS = [
     "-O3",
     "-std=c++17",
     "-fPIC",
-    "-shared",
     "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     # MFMA results in ordinary VGPRs: the element stage reads every result once,
@@ -48,8 +52,19 @@ def _run(cmd, cwd=None):
 
 
 def build_library(out=LIB):
+    """Each translation unit to an object (in parallel), then one link."""
+    from concurrent.futures import ThreadPoolExecutor
+    os.makedirs(OBJDIR, exist_ok=True)
+    inc = ["-I" + os.path.join(ROOT, "include")]
+
+    def compile_one(src):
+        obj = os.path.join(OBJDIR, os.path.basename(src) + ".o")
+        _run([hipcc()] + HIPCC_FLAGS + inc + ["-c", src, "-o", obj])
+        return obj
+    with ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
+        objs = list(ex.map(compile_one, SOURCES))
     tmp = out + ".tmp"
-    _run([hipcc()] + HIPCC_FLAGS + ["-I" + os.path.join(ROOT, "include")] + SOURCES + ["-o", tmp])
+    _run([hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC"] + objs + ["-o", tmp])
     os.replace(tmp, out)
     return out
 
@@ -60,9 +75,19 @@ def build_oracle():
         _run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref", f"REF={REF_TREE}"])
 
 
+def build_reference_integration():
+    """The reference library with the AMD platform branch (oracle/ref_amd.py),
+    linked against the library just built; only where the reference exists."""
+    if os.path.isdir(os.path.join(REF_TREE, "src")):
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import ref_amd
+        ref_amd.build(REF_TREE)
+
+
 def build():
     build_oracle()
     build_library()
+    build_reference_integration()
 
 
 if __name__ == "__main__":

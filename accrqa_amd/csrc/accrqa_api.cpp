@@ -13,12 +13,23 @@
 
 namespace {
 
-bool platform_ok(Accrqa_CompPlatform p, Accrqa_Error *error) {
-    if (p == PLT_AMD_GPU) return true;
-    std::printf("WARNING: this library implements PLT_AMD_GPU only; use the reference library for platform %d.\n",
-                (int)p);
-    if (error) *error = ERR_INVALID_ARGUMENT;
-    return false;
+// Platform dispatch of this library.  The reference runs PLT_NV_GPU without
+// CUDA on its default platform silently, and any value other than PLT_CPU /
+// PLT_NV_GPU on its default platform after a warning
+// (src/AccRQA_functions.cpp:525-539).  Here the only platform is the AMD GPU,
+// whose results equal PLT_CPU's bit for bit, so every value is served there:
+// PLT_AMD_GPU and PLT_NV_GPU silently, PLT_CPU with a one-time note (the
+// caller asked for the host), other values with the reference's warning.
+void serve_platform(Accrqa_CompPlatform p) {
+    static bool cpu_noted = false;
+    if (p == PLT_AMD_GPU || p == PLT_NV_GPU) return;
+    if (p == PLT_CPU) {
+        if (!cpu_noted)
+            std::printf("NOTE: libaccrqa_amd computes PLT_CPU requests on the AMD GPU (same results as the CPU path).\n");
+        cpu_noted = true;
+        return;
+    }
+    std::printf("WARNING: Unknown compute platform. Defaulting to AMD GPU.\n");
 }
 
 }  // namespace
@@ -46,7 +57,7 @@ void accrqa_print_error(Accrqa_Error *error) {
     void NAME(T *output, T *input_data, size_t data_size, int *tau_values, int nTaus, int *emb_values, int nEmbs, \
               int *l_values, int nL, T *threshold_values, int nThresholds, Accrqa_Distance distance_type,        \
               int calc_ENTR, Accrqa_CompPlatform comp_platform, Accrqa_Error *error) {                           \
-        if (!platform_ok(comp_platform, error)) return;                                                          \
+        serve_platform(comp_platform);                                                                           \
         int e = 0;                                                                                               \
         NAME##_amd_##SFX(output, input_data, data_size, tau_values, nTaus, emb_values, nEmbs, l_values, nL,      \
                          threshold_values, nThresholds, (int)distance_type, calc_ENTR, &e);                      \
@@ -68,7 +79,7 @@ ACCRQA_DL(accrqa_LAM, double, f64)
 void accrqa_RR(float *output, float *input_data, size_t data_size, int *tau_values, int nTaus, int *emb_values,
                int nEmbs, float *threshold_values, int nThresholds, Accrqa_Distance distance_type,
                Accrqa_CompPlatform comp_platform, Accrqa_Error *error) {
-    if (!platform_ok(comp_platform, error)) return;
+    serve_platform(comp_platform);
     int e = 0;
     accrqa_amd_RR_f32(output, input_data, data_size, tau_values, nTaus, emb_values, nEmbs, threshold_values,
                       nThresholds, (int)distance_type, &e);
@@ -78,7 +89,7 @@ void accrqa_RR(float *output, float *input_data, size_t data_size, int *tau_valu
 void accrqa_RR(double *output, double *input_data, size_t data_size, int *tau_values, int nTaus, int *emb_values,
                int nEmbs, double *threshold_values, int nThresholds, Accrqa_Distance distance_type,
                Accrqa_CompPlatform comp_platform, Accrqa_Error *error) {
-    if (!platform_ok(comp_platform, error)) return;
+    serve_platform(comp_platform);
     int e = 0;
     accrqa_amd_RR_f64(output, input_data, data_size, tau_values, nTaus, emb_values, nEmbs, threshold_values,
                       nThresholds, (int)distance_type, &e);

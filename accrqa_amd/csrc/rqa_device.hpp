@@ -36,6 +36,10 @@
 
 #include "rqa_bits.hpp"
 
+// A translation unit that only needs the device helpers defines
+// RQA_HELPERS_ONLY: the non-template kernels below (one definition each, in
+// the engine's own unit) are then left out.
+
 #pragma clang fp contract(off)
 
 namespace rqa {
@@ -853,6 +857,64 @@ __global__ __launch_bounds__(WG) void k_rr(const F *__restrict__ x, int64_t nc, 
         if (acc[e]) glob_add(&counts[e], acc[e]);
 }
 
+// ---------------------------------------------------------------- k_fp64_diag
+// Diagnostic for fp32 input (no counterpart in the reference): for each
+// threshold, the pairs of the full matrix whose decision in the reference's
+// fp32 arithmetic differs from the decision in true fp64 arithmetic on the
+// same fp32 samples, i.e. [s32 <= B32_t] != [s64 <= B64_t] with s in the
+// reference's order in each precision and B the exact bound of each
+// precision.  Work items as k_rr (band of 64 columns, row segment, rows up to
+// the band's last column), weight 2 off the diagonal.
+constexpr int DIAG_MAXT = 32;
+
+template <int NORM>
+__global__ __launch_bounds__(WG) void k_fp64_diag(const float *__restrict__ x, int64_t nc, int tau, int m,
+                                                  Bounds<float, DIAG_MAXT> B32, Bounds<double, DIAG_MAXT> B64,
+                                                  int nt, int seg_len, const int2 *__restrict__ items, int64_t nitems,
+                                                  unsigned long long *__restrict__ counts) {
+    __shared__ unsigned long long acc[DIAG_MAXT];
+    for (int e = threadIdx.x; e < DIAG_MAXT; e += WG) acc[e] = 0;
+    __syncthreads();
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wave = wave_id();
+    for (int64_t item = (int64_t)blockIdx.x * WAVES + wave; item < nitems; item += (int64_t)gridDim.x * WAVES) {
+        const int2 it = items[item];
+        const int64_t j0 = (int64_t)it.x * WAVE;
+        const int64_t j = j0 + lane;
+        const bool jv = j < nc;
+        const int64_t rows = (j0 + WAVE < nc ? j0 + WAVE : nc);
+        const int64_t ib = (int64_t)it.y * seg_len;
+        const int64_t ie = ib + seg_len < rows ? ib + seg_len : rows;
+        uint32_t c[DIAG_MAXT];
+#pragma unroll
+        for (int t = 0; t < DIAG_MAXT; t++) c[t] = 0;
+        for (int64_t i = ib; i < ie; i++) {
+            float s32 = 0.f;
+            double s64 = 0.0;
+            for (int k = 0; k < m; k++) {
+                const float a = x[i + (int64_t)k * tau], b = x[j + (int64_t)k * tau];
+                s32 = dim_step<NORM>(s32, a, b);
+                s64 = dim_step<NORM>(s64, (double)a, (double)b);
+            }
+            const uint32_t w = !jv || i > j ? 0u : (i == j ? 1u : 2u);
+#pragma unroll
+            for (int t = 0; t < DIAG_MAXT; t++)
+                if (t < nt) c[t] += ((s32 <= B32.s[t]) != (s64 <= B64.s[t])) ? w : 0u;
+        }
+#pragma unroll
+        for (int t = 0; t < DIAG_MAXT; t++) {
+            if (t < nt) {
+                unsigned long long v = c[t];
+                for (int off = WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off);
+                if (lane == 0 && v) atomicAdd(&acc[t], v);
+            }
+        }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < nt; e += WG)
+        if (acc[e]) glob_add(&counts[e], acc[e]);
+}
+
 // ================================================================ MFMA form
 // Euclidean norm, fp32 input.  The squared distance is approximated as
 //   s~ = (n_i + n_j) - 2 g_ij,   g = x_i . x_j on MFMA,
@@ -1112,6 +1174,7 @@ __device__ __forceinline__ floatx16 mfma16(halfx8 a, halfx8 b, floatx16 c) {
 // wave): a = A[32][16], b = B[16][32] as fp16 bits, row-major; c = the
 // accumulator C[32][32] in, d = C + A B out.  Lanes supply the operands as
 // gram_tile does (dimension 8h + t in slot t) and hold C in its layout.
+#ifndef RQA_HELPERS_ONLY
 __global__ void k_mfma16_probe(const uint16_t *__restrict__ a, const uint16_t *__restrict__ b,
                                const float *__restrict__ c, float *__restrict__ d) {
     const int lane = threadIdx.x & (WAVE - 1), col = lane & 31, h = lane >> 5;
@@ -1128,6 +1191,7 @@ __global__ void k_mfma16_probe(const uint16_t *__restrict__ a, const uint16_t *_
 #pragma unroll
     for (int e = 0; e < 16; e++) d[(8 * (e >> 2) + 4 * h + (e & 3)) * 32 + col] = acc[e];
 }
+#endif
 
 // One 32x32 Gram tile (column tile p) in scaled units: G[row][32p + col]
 // over the m dimensions.  Order of accumulation: l.h and h.l of every chunk,
@@ -1429,6 +1493,7 @@ __global__ __launch_bounds__(WG) RQA_WPE_MX(SCAN) void k_lam_mx(const F *__restr
 // p is row 8(e>>2) + 4h + (e&3), column 32p + col, so each store writes two
 // runs of 32 consecutive bytes.  Windows read up to x[N + 62] and nrm[N_c + 62]
 // (both padded with zeros).
+#ifndef RQA_HELPERS_ONLY
 __global__ __launch_bounds__(WG) RQA_WPE_MX(false) void k_rp_mx(const float *__restrict__ x,
                                                                const float *__restrict__ nrm,
                                                                const unsigned int *__restrict__ nmax_bits, int64_t nc,
@@ -1494,6 +1559,7 @@ __global__ __launch_bounds__(WG) RQA_WPE_MX(false) void k_rp_mx(const float *__r
         }
     }
 }
+#endif
 
 // ---------------------------------------------------------------- k_lam_vscan
 // The column part of LAM in the split form, read back from the rank tiles
@@ -1720,6 +1786,7 @@ __global__ __launch_bounds__(WG) RQA_WPE_MX(true) void k_det_mx(const float *__r
 constexpr int STT = 256;
 constexpr int SCAP = 4096;
 
+#ifndef RQA_HELPERS_ONLY
 __global__ __launch_bounds__(STT) void k_stitch(const uint2 *__restrict__ carry, int nt, int64_t nseq, int64_t nc,
                                                 int is_det, int split, int seg_len, int nseg, int cseg,
                                                 unsigned long long *__restrict__ hist) {
@@ -1766,6 +1833,7 @@ __global__ __launch_bounds__(STT) void k_stitch(const uint2 *__restrict__ carry,
     for (int e = threadIdx.x; e < SCAP; e += STT)
         if (sh[e] && e <= nc) glob_add(&h[e], sh[e]);
 }
+#endif
 
 // ---------------------------------------------------------------- k_finalize
 // One workgroup per threshold.  Turns a line histogram into the reference's
@@ -1783,6 +1851,7 @@ constexpr int FIN = 1024;
 // below nc of each threshold, as tops[t] = bin + 1 (0 if none; tops zeroed
 // before the launch).
 constexpr int HPT = 256;
+#ifndef RQA_HELPERS_ONLY
 __global__ __launch_bounds__(HPT) void k_hist_prep(unsigned long long *__restrict__ hist, int64_t nc, int is_det,
                                                    unsigned long long *__restrict__ tops) {
     const int t = blockIdx.y;
@@ -1806,6 +1875,7 @@ __global__ __launch_bounds__(HPT) void k_hist_prep(unsigned long long *__restric
     }
     if (threadIdx.x == 0 && red[0]) atomicMax(&tops[t], red[0]);
 }
+#endif
 
 template <typename F>
 __global__ __launch_bounds__(FIN) void k_finalize(unsigned long long *__restrict__ hist, int64_t nc, int nt,

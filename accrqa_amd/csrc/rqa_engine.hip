@@ -27,46 +27,41 @@
 #include <vector>
 
 #include "accrqa_amd.h"
+#include "engine.hpp"
 #include "rqa_device.hpp"
 
-namespace {
-
-enum : int {
-    E_OK = 0,
-    E_INVALID = 2,
-    E_NULL = 4,
-    E_NO_DEVICE = 8,
-    E_NO_MEMORY = 9,
-    E_DEVICE = 10,
-};
+namespace eng {
 
 std::string g_last_error;
 
-struct DevBuf {
-    void *p = nullptr;
-    size_t cap = 0;
-    uint64_t gen = 0;   // allocation number (a new one for every hipMalloc)
-};
 uint64_t g_gen = 0;
-
-struct Ctx {
-    int device = -1;
-    int ncu = 0;
-    hipStream_t stream = nullptr;
-    DevBuf x, hist, carry, metric, scan, counts, perm, lmins, lut, nrm, nmax, lut8, out_rr, out_det, out_lam, priv,
-        tiles, dump, tops, mxstats, probe, out_rp;
-    // work-item tables of the current (nc, seg_len): views of an ItemSet
-    // entry (owned there, never freed through these)
-    DevBuf items_rr, items_det, items_dets, items_lamv, items_lamh;
-    // Raw-result sinks of accrqa_amd_histograms_*: when set, launch_group
-    // copies each group's counts / h_D rows / h_V rows there, indexed by the
-    // caller's threshold order.  Null for every other entry point.
-    unsigned long long *dump_rr = nullptr, *dump_det = nullptr, *dump_lam = nullptr;
-    int64_t n_items_rr = 0, n_items_det = 0, n_items_dets = 0, n_items_lamv = 0, n_items_lamh = 0;
-};
 
 std::mutex g_mu;
 Ctx g_ctx;
+
+// Pinned host staging for the small uploads of a call (threshold
+// permutation, l_min lists, bucket tables, work-item tables): copies from
+// pinned memory are asynchronous, so the bytes must stay as they are until
+// the copies reading them have run.  The arena is filled front to back and
+// only reset once every copy issued from it is known complete: the previous
+// call's completion event (which, through the ordering below, follows every
+// earlier call) and the current call's stream.
+struct Staging {
+    char *p = nullptr;
+    size_t cap = 0, used = 0;
+} g_stage;
+
+// Calls are serialised by g_mu.  Consecutive calls on different streams (the
+// host entry points use the library's own stream, the device entry points
+// the caller's) are ordered on the device as well: each call's stream waits
+// for the completion event of the previous call, whose stream waited for the
+// one before, so the work buffers shared by all calls are never written
+// while an earlier call may still read them.
+struct CallOrder {
+    hipEvent_t done = nullptr;
+    hipStream_t last = nullptr;
+    bool any = false;
+} g_order;
 
 // Work-item tables kept for the last few (nc, seg_len): a call with several
 // (tau, m) blocks, or a loop over many series of a few lengths, alternates
@@ -115,18 +110,13 @@ int map_err(hipError_t e) {
     return E_DEVICE;
 }
 
-#define HIP_TRY(expr)                             \
-    do {                                          \
-        hipError_t e_ = (expr);                   \
-        if (e_ != hipSuccess) return map_err(e_); \
-    } while (0)
-
 int ensure(DevBuf &b, size_t bytes) {
     if (bytes == 0) bytes = 256;
     if (b.cap >= bytes) return E_OK;
     if (b.p) {
-        // kernels queued by earlier blocks of the same call may still read it
-        if (g_ctx.stream) HIP_TRY(hipStreamSynchronize(g_ctx.stream));
+        // kernels queued by this call or an earlier one, on any stream, may
+        // still read it (a rare path: buffers only grow)
+        HIP_TRY(hipDeviceSynchronize());
         hipError_t e = hipFree(b.p);
         b.p = nullptr;
         b.cap = 0;
@@ -148,11 +138,53 @@ struct Uploaded {
 };
 Uploaded g_up_perm, g_up_lmins, g_up_lut;
 
+// Copies `bytes` from host memory to device memory `dst` in stream order,
+// through the pinned staging arena (the source may be released on return).
+int stage_upload(void *dst, const void *src, size_t bytes, hipStream_t st) {
+    if (!bytes) return E_OK;
+    const size_t need = (bytes + 255) & ~(size_t)255;
+    Staging &S = g_stage;
+    if (S.used + need > S.cap) {
+        // every copy issued from the arena must have run before it is reused
+        if (g_order.any) HIP_TRY(hipEventSynchronize(g_order.done));
+        HIP_TRY(hipStreamSynchronize(st));
+        S.used = 0;
+        if (need > S.cap) {
+            if (S.p) HIP_TRY(hipHostFree(S.p));
+            S.p = nullptr;
+            S.cap = 0;
+            const size_t cap = std::max<size_t>(need, (size_t)8 << 20);
+            HIP_TRY(hipHostMalloc((void **)&S.p, cap, hipHostMallocDefault));
+            S.cap = cap;
+        }
+    }
+    char *h = S.p + S.used;
+    std::memcpy(h, src, bytes);
+    S.used += need;
+    HIP_TRY(hipMemcpyAsync(dst, h, bytes, hipMemcpyHostToDevice, st));
+    return E_OK;
+}
+
 int upload_once(const DevBuf &b, Uploaded &u, const void *src, size_t bytes, hipStream_t st) {
     if (u.gen == b.gen && u.bytes.size() == bytes && std::memcmp(u.bytes.data(), src, bytes) == 0) return E_OK;
-    HIP_TRY(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, st));
+    int e = stage_upload(b.p, src, bytes, st);
+    if (e) return e;
     u.gen = b.gen;
     u.bytes.assign((const char *)src, (const char *)src + bytes);
+    return E_OK;
+}
+
+// Start and end of every entry point that queues device work on `st`.
+int begin_call(hipStream_t st) {
+    if (!g_order.done) HIP_TRY(hipEventCreateWithFlags(&g_order.done, hipEventDisableTiming));
+    if (g_order.any && g_order.last != st) HIP_TRY(hipStreamWaitEvent(st, g_order.done, 0));
+    return E_OK;
+}
+
+int end_call(hipStream_t st) {
+    HIP_TRY(hipEventRecord(g_order.done, st));
+    g_order.last = st;
+    g_order.any = true;
     return E_OK;
 }
 
@@ -175,7 +207,7 @@ int init_ctx() {
                              &g_ctx.lut,     &g_ctx.nrm,        &g_ctx.nmax,       &g_ctx.lut8,
                              &g_ctx.out_rr,  &g_ctx.out_det,    &g_ctx.out_lam,    &g_ctx.priv,
                              &g_ctx.tiles,   &g_ctx.dump,       &g_ctx.tops,
-                             &g_ctx.mxstats, &g_ctx.probe, &g_ctx.out_rp};
+                             &g_ctx.mxstats, &g_ctx.probe, &g_ctx.out_rp, &g_ctx.bhist, &g_ctx.bdesc};
             for (DevBuf *b : all) {
                 if (b->p) (void)hipFree(b->p);
                 *b = DevBuf();
@@ -186,6 +218,9 @@ int init_ctx() {
                 s = ItemSet();
             }
             g_ctx.items_rr = g_ctx.items_det = g_ctx.items_dets = g_ctx.items_lamv = g_ctx.items_lamh = DevBuf();
+            if (g_order.done) (void)hipEventSynchronize(g_order.done);
+            if (g_order.done) (void)hipEventDestroy(g_order.done);
+            g_order = CallOrder();
             if (g_ctx.stream) (void)hipStreamDestroy(g_ctx.stream);
             g_ctx.stream = nullptr;
             (void)hipSetDevice(cur);
@@ -197,46 +232,6 @@ int init_ctx() {
         g_ctx.device = dev;
     }
     return E_OK;
-}
-
-// ---------------------------------------------------------------- thresholds
-// Euclidean: largest s >= 0 with (eps - RN(sqrt(s))) >= 0 in F, so that the
-// GPU decision [s <= S] equals the reference's (eps - sqrt(s)) >= 0
-// (AccRQA_CPU_function.cpp:22,45).  Values that never recur map to -1.
-template <typename F>
-bool euclid_ok(F eps, F s) {
-    F d = std::sqrt(s);
-    F diff = eps - d;
-    return diff >= F(0);
-}
-
-template <typename F>
-F euclid_bound(F eps) {
-    const F big = std::numeric_limits<F>::max();
-    const F inf = std::numeric_limits<F>::infinity();
-    if (std::isnan(eps) || eps < F(0)) return F(-1);
-    if (eps == inf) return big;
-    if (eps == F(0)) return F(0);   // +0 or -0: only s == 0 recurs
-    double e2 = (double)eps * (double)eps;
-    F s = e2 > (double)big ? big : (F)e2;
-    while (s < big && euclid_ok(eps, std::nextafter(s, inf))) s = std::nextafter(s, inf);
-    while (s > F(0) && !euclid_ok(eps, s)) s = std::nextafter(s, -inf);
-    return s;
-}
-
-template <typename F>
-F bound_of(F eps, int dist) {
-    if (dist == 1) return euclid_bound(eps);
-    // maximal norm: R = [(eps - d) >= 0] = [d <= eps] except eps = d = +inf
-    if (eps == std::numeric_limits<F>::infinity()) return std::numeric_limits<F>::max();
-    return eps;   // NaN compares false, negative eps never recurs
-}
-
-// A threshold whose bound is negative or NaN never recurs; it skips the pair
-// kernels (its histogram is empty).  -0 behaves as +0.
-template <typename F>
-bool never_recurs(F b) {
-    return std::isnan(b) || b < F(0);
 }
 
 int tg_for(int nt) { return nt <= 4 ? 4 : (nt <= 8 ? 8 : (nt <= 16 ? 16 : 32)); }
@@ -350,8 +345,9 @@ int dump_rows(const unsigned long long *src, int nt, size_t hs, const int *d_per
 
 // ---------------------------------------------------------------- profiling
 // Optional per-kernel timing with HIP events recorded on the launch stream
-// (accrqa_amd_profile_*).  Off by default; when on, run_block synchronises
-// its stream and folds the event pairs into per-kernel totals.
+// (accrqa_amd_profile_*).  Off by default.  The event pairs are folded into
+// per-kernel totals when the totals are read (or when many are pending), so a
+// timed loop of calls is not synchronised per call.
 struct ProfEntry {
     const char *name;
     hipEvent_t a, b;
@@ -391,11 +387,11 @@ void prof_end(hipStream_t st) {
     g_prof.cur = nullptr;
 }
 
-void prof_collect(hipStream_t st) {
-    if (!g_prof.on || g_prof.pending.empty()) return;
-    (void)hipStreamSynchronize(st);
+// Waits for the pending event pairs and adds them to the totals.
+void prof_fold() {
     for (ProfEntry &p : g_prof.pending) {
         float ms = 0.f;
+        (void)hipEventSynchronize(p.b);
         (void)hipEventElapsedTime(&ms, p.a, p.b);
         bool found = false;
         for (auto &kv : g_prof.acc)
@@ -411,7 +407,10 @@ void prof_collect(hipStream_t st) {
     g_prof.pending.clear();
 }
 
-constexpr int PAD = 128;   // zero samples after N (kernel windows read up to N + 95)
+// End of a call: fold only when many pairs are pending (bounded event use).
+void prof_collect(hipStream_t) {
+    if (g_prof.pending.size() > 16384) prof_fold();
+}
 
 // Segment length: enough full-column (band, segment) items for about sixteen
 // per resident wave of a full GPU (65536 items; the split LAM and DET passes
@@ -426,30 +425,6 @@ int seg_len_for(int64_t nc) {
     L = std::max<int64_t>(L, 256);
     L = (L + rqa::BLK - 1) / rqa::BLK * rqa::BLK;
     return (int)std::min<int64_t>(L, 1 << 30);
-}
-
-// Persistent grid: as many workgroups as can be resident at once (from the
-// kernel's register and LDS use), or fewer if there are fewer items.
-template <typename Kern>
-unsigned grid_for(Kern kern, int64_t nitems, int ncu) {
-    static std::map<const void *, int> per_cu;
-    const void *key = reinterpret_cast<const void *>(kern);
-    auto it = per_cu.find(key);
-    int nb = 0;
-    if (it == per_cu.end()) {
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, key, rqa::WG, 0) != hipSuccess || nb < 1) nb = 1;
-        per_cu[key] = nb;
-    } else {
-        nb = it->second;
-    }
-    const int64_t wgs = (nitems + rqa::WAVES - 1) / rqa::WAVES;
-    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(wgs, (int64_t)nb * ncu));
-}
-
-// Persistent grid for a kernel whose work items belong to whole workgroups.
-template <typename Kern>
-unsigned grid_for_wg(Kern kern, int64_t nitems, int ncu) {
-    return grid_for(kern, nitems * rqa::WAVES, ncu);
 }
 
 // Workgroup-private long-run bins for a launch of `grid` workgroups.
@@ -536,11 +511,9 @@ int build_items(int64_t nc, hipStream_t st) {
         for (size_t i = 0; i < v.size(); i++) it[i] = v[i].second;
         int e2;
         if ((e2 = ensure(buf, std::max<size_t>(it.size(), 1) * sizeof(int2)))) return e2;
-        // ordered on the engine's stream: after the queued kernels that may
+        // ordered on the call's stream: after the queued kernels that may
         // still read the entry's previous tables, before those of this block
-        // (the runtime stages pageable host memory before it returns)
-        if (!it.empty())
-            HIP_TRY(hipMemcpyAsync(buf.p, it.data(), it.size() * sizeof(int2), hipMemcpyHostToDevice, st));
+        if ((e2 = stage_upload(buf.p, it.data(), it.size() * sizeof(int2), st))) return e2;
         n = (int64_t)it.size();
         return E_OK;
     };
@@ -968,7 +941,8 @@ void host_api(int kind, F *output, const F *x, size_t n, const int *tau, int nta
     }
     Ctx &c = g_ctx;
     hipStream_t st = c.stream;
-    if ((e = upload_series(x, n, false, st))) {
+    CallScope call(st);
+    if ((e = call.err) || (e = upload_series(x, n, false, st))) {
         *error = e;
         return;
     }
@@ -1009,13 +983,17 @@ void host_hist(const F *x, size_t n, int tau, int emb, const F *thr, int T, int 
     std::lock_guard<std::mutex> lk(g_mu);
     int e = check_args(n, &tau, 1, &emb, 1, nullptr, 0, dist);
     if (!e) e = init_ctx();
-    if (!e) e = upload_series(x, n, false, g_ctx.stream);
     if (e) {
         *error = e;
         return;
     }
     Ctx &c = g_ctx;
     hipStream_t st = c.stream;
+    CallScope call(st);
+    if ((e = call.err) || (e = upload_series(x, n, false, st))) {
+        *error = e;
+        return;
+    }
     const int64_t nc = (int64_t)n - (int64_t)(emb - 1) * tau;
     const size_t hs = (size_t)nc + 1;
     // The histograms come from the same kernels and launches as RR/DET/LAM,
@@ -1066,92 +1044,59 @@ int device_api(const F *d_in, size_t n, int tau, int emb, const F *thr, int T, c
     if (!e) e = init_ctx();
     if (e) return e;
     hipStream_t st = (hipStream_t)stream;
-    if ((e = upload_series(d_in, n, true, st))) return e;
+    CallScope call(st);
+    if ((e = call.err) || (e = upload_series(d_in, n, true, st))) return e;
     return run_block<F>((const F *)g_ctx.x.p, n, tau, emb, thr, T, lm, (d_det || d_lam) ? nl : 0, dist, d_rr, d_det,
                         d_lam, st);
 }
 
-// Many series in one call: every series uploaded once (each followed by its
-// own PAD zeros, which the kernels' windows read past its end), every
-// (series, tau, emb) block computing RR, DET and LAM together from one pass
-// over its pairs into its part of device outputs, and one copy back.
-// Thresholds are per series ([nseries][T]); outputs are the per-series
-// reference layouts stacked: rr [s][a][b][T], det [s][a][b][T][nl][5],
-// lam [s][a][b][nl][T][5].
-template <typename F>
-void host_batch(const F *series, const size_t *offsets, int nseries, const int *tau, int ntau, const int *emb,
-                int nemb, const int *lm, int nl, const F *thr, int T, int dist, F *rr_out, F *det_out, F *lam_out,
-                int *error) {
-    if (!error) return;
-    *error = E_OK;
-    const bool want_l = det_out || lam_out;
-    if (nseries <= 0 || ntau <= 0 || nemb <= 0 || T <= 0 || (want_l && nl <= 0)) *error = E_INVALID;
-    if (!series || !offsets || !tau || !emb || !thr || (want_l && !lm) || (!rr_out && !det_out && !lam_out))
-        *error = E_NULL;
-    if (*error) return;
-    int e = E_OK;
-    size_t total = 0;
-    for (int k = 0; k < nseries && !e; k++) {
-        if (offsets[k + 1] <= offsets[k]) e = E_INVALID;
-        if (!e) e = check_args(offsets[k + 1] - offsets[k], tau, ntau, emb, nemb, lm, want_l ? nl : 0, dist);
-        total += offsets[k + 1] - offsets[k] + PAD;
-    }
-    if (e) {
-        *error = e;
-        return;
-    }
+// fp32 decisions against true fp64 decisions on the same samples (diagnostic):
+// counts[t] = pairs of the full matrix whose decisions differ, in the
+// caller's threshold order.
+int device_fp64_diag(const float *d_in, size_t n, int tau, int emb, const float *thr, int T, int dist,
+                     unsigned long long *counts, void *stream) {
+    if (n == 0 || T <= 0) return E_INVALID;
+    if (!d_in || !thr || !counts) return E_NULL;
     std::lock_guard<std::mutex> lk(g_mu);
-    if ((e = init_ctx())) {
-        *error = e;
-        return;
-    }
+    int e = check_args(n, &tau, 1, &emb, 1, nullptr, 0, dist);
+    if (!e) e = init_ctx();
+    if (e) return e;
+    hipStream_t st = (hipStream_t)stream;
+    CallScope call(st);
+    if ((e = call.err) || (e = upload_series(d_in, n, true, st))) return e;
     Ctx &c = g_ctx;
-    hipStream_t st = c.stream;
-    std::vector<F> host(total, F(0));
-    std::vector<size_t> poff(nseries);
-    size_t p = 0;
-    for (int k = 0; k < nseries; k++) {
-        poff[k] = p;
-        std::copy(series + offsets[k], series + offsets[k + 1], host.begin() + (ptrdiff_t)p);
-        p += offsets[k + 1] - offsets[k] + PAD;
+    const int64_t nc = (int64_t)n - (int64_t)(emb - 1) * tau;
+    if ((e = build_items(nc, st))) return e;
+    if ((e = ensure(c.counts, std::max<size_t>(64, (size_t)T) * sizeof(unsigned long long)))) return e;
+    unsigned long long *d_counts = (unsigned long long *)c.counts.p;
+    fill_zero(d_counts, (size_t)T, st);
+    const int L = seg_len_for(nc);
+    for (int t0 = 0; t0 < T; t0 += rqa::DIAG_MAXT) {
+        const int nt = std::min(rqa::DIAG_MAXT, T - t0);
+        rqa::Bounds<float, rqa::DIAG_MAXT> B32;
+        rqa::Bounds<double, rqa::DIAG_MAXT> B64;
+        for (int t = 0; t < rqa::DIAG_MAXT; t++) {
+            const float e32 = thr[t0 + std::min(t, nt - 1)];
+            const float b32 = bound_of<float>(e32, dist);
+            const double b64 = bound_of<double>((double)e32, dist);
+            B32.s[t] = never_recurs(b32) ? -1.0f : b32;
+            B64.s[t] = never_recurs(b64) ? -1.0 : b64;
+        }
+        const int64_t ni = c.n_items_rr;
+        if (dist == 1) {
+            const unsigned g = grid_for(rqa::k_fp64_diag<1>, ni, c.ncu);
+            hipLaunchKernelGGL(rqa::k_fp64_diag<1>, dim3(g), dim3(rqa::WG), 0, st, (const float *)c.x.p, nc, tau, emb,
+                               B32, B64, nt, L, (const int2 *)c.items_rr.p, ni, d_counts + t0);
+        } else {
+            const unsigned g = grid_for(rqa::k_fp64_diag<2>, ni, c.ncu);
+            hipLaunchKernelGGL(rqa::k_fp64_diag<2>, dim3(g), dim3(rqa::WG), 0, st, (const float *)c.x.p, nc, tau, emb,
+                               B32, B64, nt, L, (const int2 *)c.items_rr.p, ni, d_counts + t0);
+        }
+        HIP_TRY(hipGetLastError());
     }
-    const size_t blocks = (size_t)nseries * ntau * nemb;
-    const size_t per_rr = (size_t)T, per_dl = (size_t)T * nl * 5;
-    if ((e = ensure(c.x, total * sizeof(F))) || (e = ensure(c.out_rr, blocks * per_rr * sizeof(F))) ||
-        (e = ensure(c.out_det, blocks * (want_l ? per_dl : 1) * sizeof(F))) ||
-        (e = ensure(c.out_lam, blocks * (want_l ? per_dl : 1) * sizeof(F)))) {
-        *error = e;
-        return;
-    }
-    hipError_t he = hipMemcpyAsync(c.x.p, host.data(), total * sizeof(F), hipMemcpyHostToDevice, st);
-    if (he == hipSuccess) he = hipStreamSynchronize(st);
-    if (he != hipSuccess) {
-        *error = map_err(he);
-        return;
-    }
-    for (int k = 0; k < nseries && !e; k++)
-        for (int a = 0; a < ntau && !e; a++)
-            for (int b = 0; b < nemb && !e; b++) {
-                const size_t blk = ((size_t)k * ntau + a) * nemb + b;
-                e = run_block<F>((const F *)c.x.p + poff[k], offsets[k + 1] - offsets[k], tau[a], emb[b],
-                                 thr + (size_t)k * T, T, want_l ? lm : nullptr, want_l ? nl : 0, dist,
-                                 rr_out ? (F *)c.out_rr.p + blk * per_rr : nullptr,
-                                 det_out ? (F *)c.out_det.p + blk * per_dl : nullptr,
-                                 lam_out ? (F *)c.out_lam.p + blk * per_dl : nullptr, st);
-            }
-    if (e) {
-        (void)hipStreamSynchronize(st);
-        *error = e;
-        return;
-    }
-    he = hipSuccess;
-    if (rr_out) he = hipMemcpyAsync(rr_out, c.out_rr.p, blocks * per_rr * sizeof(F), hipMemcpyDeviceToHost, st);
-    if (he == hipSuccess && det_out)
-        he = hipMemcpyAsync(det_out, c.out_det.p, blocks * per_dl * sizeof(F), hipMemcpyDeviceToHost, st);
-    if (he == hipSuccess && lam_out)
-        he = hipMemcpyAsync(lam_out, c.out_lam.p, blocks * per_dl * sizeof(F), hipMemcpyDeviceToHost, st);
-    if (he == hipSuccess) he = hipStreamSynchronize(st);
-    if (he != hipSuccess) *error = map_err(he);
+    HIP_TRY(hipMemcpyAsync(counts, d_counts, (size_t)T * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return E_OK;
 }
 
 // The recurrence plot of one (tau, emb, threshold) into d_out (N_c x N_c
@@ -1218,8 +1163,14 @@ void host_rp(O *output, const F *x, size_t n, int tau, int emb, F thr, int dist,
     std::lock_guard<std::mutex> lk(g_mu);
     int e = check_args(n, &tau, 1, &emb, 1, nullptr, 0, dist);
     if (!e) e = init_ctx();
+    if (e) {
+        *error = e;
+        return;
+    }
     Ctx &c = g_ctx;
     hipStream_t st = c.stream;
+    CallScope call(st);
+    e = call.err;
     if (!e) e = upload_series(x, n, false, st);
     const int64_t nc = (int64_t)n - (int64_t)(emb - 1) * tau;
     const size_t bytes = e ? 0 : (size_t)nc * (size_t)nc * sizeof(O);
@@ -1242,11 +1193,23 @@ int device_rp(const F *d_in, size_t n, int tau, int emb, F thr, int dist, O *d_o
     if (!e) e = init_ctx();
     if (e) return e;
     hipStream_t st = (hipStream_t)stream;
-    if ((e = upload_series(d_in, n, true, st))) return e;
+    CallScope call(st);
+    if ((e = call.err) || (e = upload_series(d_in, n, true, st))) return e;
     return rp_run<F, O>((const F *)g_ctx.x.p, n, tau, emb, thr, dist, d_out, st);
 }
 
-}  // namespace
+// The padded copy and the block launcher serve the batched engine's
+// long series as well (rqa_batch.hip).
+template int upload_series<float>(const float *, size_t, bool, hipStream_t);
+template int upload_series<double>(const double *, size_t, bool, hipStream_t);
+template int run_block<float>(const float *, size_t, int, int, const float *, int, const int *, int, int, float *,
+                              float *, float *, hipStream_t);
+template int run_block<double>(const double *, size_t, int, int, const double *, int, const int *, int, int,
+                               double *, double *, double *, hipStream_t);
+
+}  // namespace eng
+
+using namespace eng;
 
 // ---------------------------------------------------------------- C ABI
 extern "C" {
@@ -1324,6 +1287,13 @@ int accrqa_amd_rqa_device_f64(const double *d_input, size_t data_size, int tau, 
                               distance_type, d_rr, d_det, d_lam, stream);
 }
 
+int accrqa_amd_fp64_mismatch_f32(const float *d_input, size_t data_size, int tau, int emb,
+                                 const float *threshold_values, int nThresholds, int distance_type,
+                                 unsigned long long *mismatches, void *stream) {
+    return device_fp64_diag(d_input, data_size, tau, emb, threshold_values, nThresholds, distance_type, mismatches,
+                            stream);
+}
+
 const char *accrqa_amd_last_error(void) { return g_last_error.c_str(); }
 
 int accrqa_amd_set_form(int form) {
@@ -1344,6 +1314,8 @@ int accrqa_amd_mx_stats(int enable) {
     std::lock_guard<std::mutex> lk(g_mu);
     int e = init_ctx();
     if (e) return e;
+    CallScope call(g_ctx.stream);
+    if ((e = call.err)) return e;
     if ((e = ensure(g_ctx.mxstats, 3 * sizeof(unsigned long long)))) return e;
     fill_zero((unsigned long long *)g_ctx.mxstats.p, 3, g_ctx.stream);
     HIP_TRY(hipStreamSynchronize(g_ctx.stream));
@@ -1382,21 +1354,6 @@ int accrqa_amd_rp_device_f64(const double *d_input, size_t data_size, int tau, i
     return device_rp<double, char>(d_input, data_size, tau, emb, threshold, distance_type, d_output, stream);
 }
 
-void accrqa_amd_rqa_batch_f32(const float *series, const size_t *offsets, int nSeries, const int *tau_values,
-                              int nTaus, const int *emb_values, int nEmbs, const int *lmin_values, int nL,
-                              const float *threshold_values, int nThresholds, int distance_type, float *rr_out,
-                              float *det_out, float *lam_out, int *error) {
-    host_batch<float>(series, offsets, nSeries, tau_values, nTaus, emb_values, nEmbs, lmin_values, nL,
-                      threshold_values, nThresholds, distance_type, rr_out, det_out, lam_out, error);
-}
-void accrqa_amd_rqa_batch_f64(const double *series, const size_t *offsets, int nSeries, const int *tau_values,
-                              int nTaus, const int *emb_values, int nEmbs, const int *lmin_values, int nL,
-                              const double *threshold_values, int nThresholds, int distance_type, double *rr_out,
-                              double *det_out, double *lam_out, int *error) {
-    host_batch<double>(series, offsets, nSeries, tau_values, nTaus, emb_values, nEmbs, lmin_values, nL,
-                       threshold_values, nThresholds, distance_type, rr_out, det_out, lam_out, error);
-}
-
 int accrqa_amd_mfma16_probe(const uint16_t *a, const uint16_t *b, const float *c, float *d) {
     if (!a || !b || !c || !d) return E_NULL;
     std::lock_guard<std::mutex> lk(g_mu);
@@ -1406,6 +1363,8 @@ int accrqa_amd_mfma16_probe(const uint16_t *a, const uint16_t *b, const float *c
     if ((e = ensure(g_ctx.probe, 10240))) return e;
     char *p = (char *)g_ctx.probe.p;
     hipStream_t st = g_ctx.stream;
+    CallScope call(st);
+    if ((e = call.err)) return e;
     HIP_TRY(hipMemcpyAsync(p, a, 1024, hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(p + 1024, b, 1024, hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(p + 2048, c, 4096, hipMemcpyHostToDevice, st));
@@ -1424,11 +1383,13 @@ void accrqa_amd_profile(int enable) {
 
 void accrqa_amd_profile_reset(void) {
     std::lock_guard<std::mutex> lk(g_mu);
+    prof_fold();
     g_prof.acc.clear();
 }
 
 int accrqa_amd_profile_read(const char *kernel, double *total_ms, long long *launches) {
     std::lock_guard<std::mutex> lk(g_mu);
+    prof_fold();
     for (auto &kv : g_prof.acc)
         if (kernel && kv.first == kernel) {
             if (total_ms) *total_ms = kv.second.first;

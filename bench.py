@@ -12,22 +12,41 @@ Default workload: AR(1) phi = 0.95 plus a 0.3-amplitude sine of period 1,000,
 N = 262,144, z-normalised, fp32, tau = 1, m = 16, 16 thresholds
 linspace(0.1, 1.6) * sqrt(m), Euclidean norm, l_min = 2.
 
-The JSON line also carries
-  roofline     -- the dominant kernel of the reported form, timed with HIP
-                  events on its own stream during the timed steps, against the
-                  peak of the unit that bounds its algorithmic work (MFMA for
-                  the Gram of k_lam_mx, HBM for the tile scans); per-kernel
-                  figures of every form under forms.*.rooflines;
-  cpu_baseline -- the reference CPU path (oracle/_ref, OpenMP on this host's
-                  cores) on a bounded prefix of the same series.
-"""
-import argparse
-import json
-import os
-import sys
-import time
+Timing: W warm-up steps, then K steps between two stream synchronisations
+with nothing else in the loop (ms_per_step, value); then the same K steps
+again with the library's per-kernel HIP events on (an event pair around each
+launch on its own stream, read back after the loop), which give each
+kernel's average launch duration.
 
-import numpy as np
+The JSON line also carries
+  roofline     -- the dominant kernel of the reported form (the largest share
+                  of the step), its algorithmic work per launch over its
+                  average launch duration against the published peak of the
+                  unit that work runs on: k_lam_mx, the MFMA form's Gram and
+                  ranks, is priced at 2m FP16 FLOP per distinct pair
+                  (N_c(N_c+1)/2 per launch) against 2.5 PFLOP/s; the exact
+                  form's k_lam at 3m FP32 operations per distinct pair against
+                  the VALU; the tile scans at the bytes of the rank tiles they
+                  read against 8 TB/s.  Every kernel's figures are under
+                  forms.*.rooflines;
+  cpu_baseline -- the reference CPU path (oracle/_ref, OpenMP over this
+                  host's cores, bound spread) on a bounded prefix of the same
+                  series: one warm-up, the median of three timed runs.
+"""
+import os
+
+# The CPU baseline (OpenMP) is bound spread over the host's cores; set before
+# any library that starts an OpenMP runtime is loaded.
+os.environ.setdefault("OMP_PROC_BIND", "spread")
+os.environ.setdefault("OMP_PLACES", "cores")
+
+import argparse  # noqa: E402
+import json  # noqa: E402
+import statistics  # noqa: E402
+import sys  # noqa: E402
+import time  # noqa: E402
+
+import numpy as np  # noqa: E402
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -69,29 +88,31 @@ def tile_store_bytes(nc):
     return (nrb * nb - (nrb // 2) * ((nrb - 1) // 2)) * 2048
 
 
-# HBM bytes per launch of the kernels on the default workload, measured with
+# HBM bytes per launch of the kernels on the default workload (value, source):
 # rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE over one step of the MFMA
-# form and scaled by the same counters' readings for a 1 GiB device copy
+# form, scaled by the same counters' readings for a 1 GiB device copy
 # (FETCH_SIZE reads 0.5 GiB for it: x2.0; WRITE_SIZE 1 GiB: x1.0).
+_TRAFFIC_SRC = "profiles/ar1_m16_traffic.md"
 MEASURED_TRAFFIC = {
-    ("ar1_m16", "k_lam_mx"): 2.0 * 0.9557e9 + 34.373e9,
-    ("ar1_m16", "k_lam_vscan"): 2.0 * 17.306e9 + 5.311e9,
-    ("ar1_m16", "k_lam_hscan"): 2.0 * 17.287e9 + 5.291e9,
-    ("ar1_m16", "k_det_scan"): 2.0 * 25.418e9 + 4.015e9,
+    ("ar1_m16", "k_lam_mx"): (2.0 * 0.9557e9 + 34.373e9, _TRAFFIC_SRC),
+    ("ar1_m16", "k_lam_vscan"): (2.0 * 17.306e9 + 5.311e9, _TRAFFIC_SRC),
+    ("ar1_m16", "k_lam_hscan"): (2.0 * 17.287e9 + 5.291e9, _TRAFFIC_SRC),
+    ("ar1_m16", "k_det_scan"): (2.0 * 25.418e9 + 4.015e9, _TRAFFIC_SRC),
 }
 
 
-def kernel_rooflines(kern, nc, emb, dist, workload):
+def kernel_rooflines(kern, nc, emb, dist, workload, tile_store):
     """Roofline figures of the main kernels from their average launch times.
-    Algorithmic work per launch:
-      k_lam_mx      one write of the rank-tile store (its HBM floor); beside
-                    it the FP16 MFMA work of the Gram: h.h, h.l and l.h
-                    products of the N_c(N_c+1)/2 distinct pairs it ranks
-                    (upper triangle and diagonal), 3 x 2m FLOP each;
-      k_lam         the same pairs' exact-order distances, 3m (Euclidean) or 2m
-                    (maximal) VALU FLOP each;
-      k_lam_vscan,  one read of the rank-tile store (one byte per pair of the
-      k_lam_hscan,  upper triangle), from HBM.
+    Algorithmic work per launch (the distinct pairs are the N_c(N_c+1)/2 of
+    the upper triangle and the diagonal; each is decided once per launch):
+      k_lam_mx     2m FP16 MFMA FLOP per distinct pair (the Gram x_i . x_j)
+                   against the dense FP16 peak; beside it the hi/lo form's
+                   3 x 2m issued FLOP and the bytes of rank tiles it writes;
+      k_lam        3m (Euclidean) or 2m (maximal) exact-order FP32 VALU
+                   operations per distinct pair against the VALU issue rate
+                   (VALU_PEAK_TOPS);
+      k_lam_vscan, one read of the rank-tile store (the form's intermediate,
+      k_lam_hscan, one byte per distinct pair) against 8 TB/s.
       k_det_scan"""
     pairs = nc * (nc + 1) / 2.0
     out = {}
@@ -100,19 +121,21 @@ def kernel_rooflines(kern, nc, emb, dist, workload):
         if t <= 0:
             continue
         if k == "k_lam_mx":
-            w = float(tile_store_bytes(nc))
-            g = pairs * 3 * 2 * emb
-            out[k] = {"bound": "hbm", "achieved": w / t / 1e9, "peak": 8000.0, "unit": "GB/s",
-                      "work_per_launch": w, "work_unit": "bytes (one write of the rank-tile store)",
-                      "mfma": {"achieved": g / t / 1e12, "peak": PUBLISHED_FP16_TFLOPS, "unit": "TFLOP/s",
-                               "frac": g / t / 1e12 / PUBLISHED_FP16_TFLOPS,
-                               "work_unit": "FP16 MFMA FLOP (h.h, h.l, l.h: 3 x 2m per distinct pair)"}}
+            g = pairs * 2 * emb
+            out[k] = {"bound": "mfma", "achieved": g / t / 1e12, "peak": PUBLISHED_FP16_TFLOPS, "unit": "TFLOP/s",
+                      "work_per_launch": g, "work_unit": "FP16 MFMA FLOP, 2m per distinct pair",
+                      "issued_hilo": {"flop_per_launch": 3 * g, "achieved": 3 * g / t / 1e12,
+                                      "frac": 3 * g / t / 1e12 / PUBLISHED_FP16_TFLOPS,
+                                      "note": "h.h + h.l + l.h products issued (3 x 2m per distinct pair)"}}
+            if tile_store:
+                w = float(tile_store_bytes(nc))
+                out[k]["tile_store_write"] = {"bytes": w, "GB_s": w / t / 1e9, "frac_of_8TB_s": w / t / 8e12}
         elif k == "k_lam":
             w = pairs * (3 * emb if dist == 1 else 2 * emb)
             out[k] = {"bound": "valu", "achieved": w / t / 1e12, "peak": VALU_PEAK_TOPS, "unit": "TFLOP/s",
-                      "work_per_launch": w, "work_unit": "FLOP (exact-order sum per distinct pair)",
+                      "work_per_launch": w, "work_unit": "FP32 VALU operations (exact-order sum per distinct pair)",
                       "peak_note": "FP32 VALU non-FMA issue rate, VGPR operands (sub/mul/add in order, no FMA)"}
-        elif k in ("k_lam_vscan", "k_lam_hscan", "k_det_scan"):
+        elif k in ("k_lam_vscan", "k_lam_hscan", "k_det_scan") and tile_store:
             w = float(tile_store_bytes(nc))
             out[k] = {"bound": "hbm", "achieved": w / t / 1e9, "peak": 8000.0, "unit": "GB/s",
                       "work_per_launch": w, "work_unit": "bytes (one read of the rank-tile store)"}
@@ -121,15 +144,31 @@ def kernel_rooflines(kern, nc, emb, dist, workload):
         r = out[k]
         r["frac"] = r["achieved"] / r["peak"]
         r["avg_ms"] = t * 1e3
-        r["traffic"] = MEASURED_TRAFFIC.get((workload, k))
-        if r["traffic"] is not None:
-            r["traffic_note"] = "HBM bytes per launch, rocprofv3 FETCH_SIZE+WRITE_SIZE, copy-calibrated"
+        r["launches"] = v["launches"]
+        tr = MEASURED_TRAFFIC.get((workload, k))
+        r["traffic"] = tr[0] if tr else None
+        if tr:
+            r["traffic_note"] = ("HBM bytes per launch: rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by the "
+                                 "counters' reading for a device copy of known size; " + tr[1])
     return out
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(x, tau, emb, thr, lmin, dist, budget_s):
-    """Time the reference CPU path (or the oracle if the reference build is
-    absent) on a prefix of the series; returns the JSON object."""
+    """Times the reference CPU path (or the oracle where the reference build
+    is absent) on a prefix of the series sized so that one RR+DET+LAM run
+    takes about budget_s / 4 seconds: one warm-up run, then the median of
+    three timed runs.  Returns the JSON object."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from pyoracle import Oracle, Reference
     O = Oracle()
@@ -147,16 +186,18 @@ def cpu_baseline(x, tau, emb, thr, lmin, dist, budget_s):
         impl.LAM(xs, [tau], [emb], lmin, thr, dist)
         return time.perf_counter() - t0
 
+    per_run = budget_s / 4.0
     ns = min(len(x), 1024 + (emb - 1) * tau)
     t = run(ns)
-    while t < 0.5 and ns < len(x):
+    while t < 0.3 and ns < len(x):
         ns = min(len(x), ns * 2)
         t = run(ns)
     if ns < len(x):
         nc0 = ns - (emb - 1) * tau
-        target_nc = int(nc0 * (budget_s / max(t, 1e-3)) ** 0.5)
-        ns = min(len(x), target_nc + (emb - 1) * tau)
-        t = run(ns)
+        ns = min(len(x), int(nc0 * (per_run / max(t, 1e-3)) ** 0.5) + (emb - 1) * tau)
+    run(ns)   # warm-up at the final size
+    times = [run(ns) for _ in range(3)]
+    t = statistics.median(times)
     nc = ns - (emb - 1) * tau
     return {
         "value": float(nc) * nc * T / t,
@@ -164,7 +205,11 @@ def cpu_baseline(x, tau, emb, thr, lmin, dist, budget_s):
         "cores": O.num_threads(),
         "kind": kind,
         "sample": (f"prefix of {ns} samples (N_c={nc}) of the same series, RR+DET+LAM via accrqa_RR/DET/LAM "
-                   f"PLT_CPU, {T} thresholds, {t:.2f} s wall"),
+                   f"PLT_CPU, {T} thresholds; median of 3 runs after a warm-up: "
+                   + ", ".join(f"{v:.2f}" for v in times) + " s"),
+        "cpu_model": cpu_model(),
+        "nproc": os.cpu_count(),
+        "omp": {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OMP_PROC_BIND", "OMP_PLACES")},
     }
 
 
@@ -174,7 +219,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="ar1_m16", choices=["ar1_m16", "lorenz", "logistic"])
-    ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
+    ap.add_argument("--cpu-budget", type=float, default=16.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--form", default="both", choices=["both", "exact", "mfma"],
                     help="Euclidean distance form(s) to time; 'both' reports the MFMA form as value")
@@ -224,9 +269,7 @@ def main():
         accrqa_amd.set_form(form)
         for _ in range(args.warmup):
             step()
-        torch.cuda.synchronize()
-        L.accrqa_amd_profile_reset()
-        L.accrqa_amd_profile(1)
+        # the timed steps: nothing but the calls between two synchronisations
         ev0 = torch.cuda.Event(enable_timing=True)
         ev1 = torch.cuda.Event(enable_timing=True)
         torch.cuda.synchronize()
@@ -237,8 +280,18 @@ def main():
         ev1.record(stream)
         torch.cuda.synchronize()
         wall = time.perf_counter() - w0
-        L.accrqa_amd_profile(0)
         ms = max(ev0.elapsed_time(ev1), wall * 1e3) / args.steps
+        outs = (d_rr.cpu().numpy().copy(), d_det.cpu().numpy().copy(), d_lam.cpu().numpy().copy())
+        # the same steps with an event pair around every launch, on its stream
+        L.accrqa_amd_profile_reset()
+        L.accrqa_amd_profile(1)
+        torch.cuda.synchronize()
+        w0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
+        wall_ev = time.perf_counter() - w0
+        L.accrqa_amd_profile(0)
         kern = {}
         for k in ("k_lam", "k_lam_mx", "k_lam_vscan", "k_lam_hscan", "k_det", "k_det_mx", "k_det_scan", "k_rr",
                   "k_stitch", "k_finalize"):
@@ -246,8 +299,7 @@ def main():
             L.accrqa_amd_profile_read(k.encode(), C.byref(t), C.byref(n))
             if n.value:
                 kern[k] = {"ms_total": t.value, "launches": n.value}
-        outs = (d_rr.cpu().numpy().copy(), d_det.cpu().numpy().copy(), d_lam.cpu().numpy().copy())
-        return ms, kern, outs
+        return ms, kern, outs, wall_ev * 1e3 / args.steps
 
     forms = {"exact": accrqa_amd.FORM_EXACT, "mfma": accrqa_amd.FORM_MFMA}
     todo = ["exact", "mfma"] if args.form == "both" else [args.form]
@@ -256,18 +308,20 @@ def main():
     res = {f: measure(forms[f]) for f in todo}
     accrqa_amd.set_form(accrqa_amd.FORM_AUTO)
     head = "mfma" if "mfma" in res else "exact"
-    ms_step, kern, outs = res[head]
+    ms_step, kern, outs, _ = res[head]
     mismatches = None
     if "mfma" in res and "exact" in res:
         a, b = res["exact"][2], res["mfma"][2]
         mismatches = int(sum(int((u.view(np.uint32) != v.view(np.uint32)).sum()) for u, v in zip(a, b)))
 
     value = float(nc) * nc * T / (ms_step / 1e3)
+    tiles = any(k in kern for k in ("k_lam_vscan", "k_lam_hscan", "k_det_scan"))   # the split form ran
     form_report = {}
-    for f, (ms_f, kern_f, _) in res.items():
+    for f, (ms_f, kern_f, _, ms_ev) in res.items():
         form_report[f] = {"value": float(nc) * nc * T / (ms_f / 1e3), "ms_per_step": ms_f,
+                          "ms_per_step_with_kernel_events": ms_ev,
                           "kernels_ms_per_step": {k: v["ms_total"] / args.steps for k, v in kern_f.items()}}
-    rooflines = {f: kernel_rooflines(res[f][1], nc, emb, dist, args.workload) for f in res}
+    rooflines = {f: kernel_rooflines(res[f][1], nc, emb, dist, args.workload, tiles) for f in res}
     for f in res:
         form_report[f]["rooflines"] = rooflines[f]
     # the dominant kernel of the reported form: the largest share of the step
@@ -298,6 +352,9 @@ def main():
         "mfma_vs_exact_speedup": (form_report["exact"]["ms_per_step"] / form_report["mfma"]["ms_per_step"]
                                   if "mfma" in form_report and "exact" in form_report else None),
         "mismatches_mfma_vs_exact": mismatches,
+        "mismatches_note": ("MFMA+recheck form against the exact-order form in the input precision (the "
+                            "reference's own decisions), every output bit; the fp32 decisions against true "
+                            "fp64 ones are the labelled diagnostic fp32_vs_true_fp64"),
         "rr_min_max": [float(rr.min()), float(rr.max())],
     }
     if not args.no_extras and "mfma" in res:
@@ -305,6 +362,42 @@ def main():
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(x, tau, emb, thr, lmin, dist, args.cpu_budget)
     print(json.dumps(out), flush=True)
+
+
+def config5_step(accrqa_amd, torch, steps=2):
+    """The million-sample configuration (Roessler x, N = 1,048,576, tau 1,
+    m 10, 32 thresholds, Euclidean, l_min 2): one RR+DET+LAM call through the
+    device entry point in the default form (the rank-tile store, 550 GB, does
+    not fit, so DET recomputes its pairs); one warm-up, then the mean of
+    `steps` calls."""
+    from accrqa_amd import workloads as W
+    t0 = time.perf_counter()
+    c = W.config5()
+    gen_s = time.perf_counter() - t0
+    x = np.ascontiguousarray(c["x"], dtype=np.float32)
+    tau, emb, thr, lmin = c["tau"][0], c["emb"][0], [float(v) for v in c["thr"]], c["lmin"]
+    T, nl = len(thr), len(lmin)
+    dev = torch.device("cuda:0")
+    st = torch.cuda.current_stream(dev)
+    dx = torch.from_numpy(x).to(dev)
+    o = [torch.zeros(n, dtype=torch.float32, device=dev) for n in (T, T * nl * 5, nl * T * 5)]
+
+    def one():
+        accrqa_amd.rqa_device(dx.data_ptr(), x.size, tau, emb, thr, lmin, 1, o[0].data_ptr(), o[1].data_ptr(),
+                              o[2].data_ptr(), stream=st.cuda_stream, dtype=np.float32)
+    one()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        one()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    nc = x.size - (emb - 1) * tau
+    del dx, o
+    torch.cuda.empty_cache()
+    return {"seconds_per_step": dt, "pair_evals_per_s": float(nc) * nc * T / dt, "n_c": nc, "thresholds": T,
+            "generator_seconds": gen_s,
+            "path": "accrqa_amd_rqa_device_f32, RR+DET+LAM, default form (direct: tile store does not fit)"}
 
 
 def config4_loop(accrqa_amd, stride=64):
@@ -333,13 +426,18 @@ def config4_loop(accrqa_amd, stride=64):
             "path": "per-series loop through accrqa_amd.RR/DET/LAM (host arrays), every %dth series" % stride}
 
 
-def config4_batched(accrqa_amd, stride=64):
-    """The batched configuration on the same every-stride-th sample as
-    config4_loop, through rqa_batch: one call per norm for all the sampled
-    series, each (series, tau, m) block computing RR, DET and LAM from one
-    pass; RR is checked against the per-series API for the first series."""
+def config4_batched(accrqa_amd):
+    """The full batched configuration (4,096 series, tau {1, 2, 4} x m {2, 3,
+    5} x 8 thresholds at {0.05 .. 0.40} of each series' std, l_min 2) through
+    rqa_batch: one call per norm for all series; every block with
+    N_c <= 65,535 (here all) through the persistent batched kernels.  Timed:
+    the two calls after a warm-up call on two series; the series are made
+    (and held in host memory) beforehand.  RR is checked against the
+    per-series API for the first series."""
     from accrqa_amd import workloads as W
-    c = W.config4(sample=range(0, 4096, stride))
+    t0 = time.perf_counter()
+    c = W.config4()
+    gen_s = time.perf_counter() - t0
     xs = c["series"]
     thr = np.array([[r * s.std() for r in c["thr_rel"]] for s in xs], dtype=np.float32)
     accrqa_amd.rqa_batch(xs[:2], c["tau"], c["emb"], c["lmin"], thr[:2], 1)
@@ -347,10 +445,16 @@ def config4_batched(accrqa_amd, stride=64):
     outs = [accrqa_amd.rqa_batch(xs, c["tau"], c["emb"], c["lmin"], thr, d) for d in c["norms"]]
     dt = time.perf_counter() - t0
     same = bool(np.array_equal(outs[0]["RR"][0], accrqa_amd.RR(xs[0], c["tau"], c["emb"], thr[0], 1)))
+    pairs = 0
+    for s in xs:
+        for tau in c["tau"]:
+            for m in c["emb"]:
+                nc = s.size - (m - 1) * tau
+                pairs += nc * nc * len(c["thr_rel"]) * len(c["norms"])
     k = len(xs)
-    return {"sample_series": k, "seconds": dt, "series_per_s": k / dt, "full_config_seconds_est": dt * 4096 / k,
-            "rr_equals_single_series_api": same,
-            "path": "accrqa_amd.rqa_batch, one call per norm over every %dth series" % stride}
+    return {"series": k, "seconds": dt, "series_per_s": k / dt, "pair_evals_per_s": pairs / dt,
+            "generator_seconds": gen_s, "rr_equals_single_series_api": same,
+            "path": "accrqa_amd.rqa_batch, one call per norm over all 4,096 series (persistent batched kernels)"}
 
 
 def rp_bench(accrqa_amd, torch, x, thr, reps=3):
@@ -427,10 +531,12 @@ def extras(accrqa_amd, L, torch, args, c, x, step):
       re-decision in FP64), outputs compared bit for bit;
     * config4_per_series: the batched configuration's throughput through the
       public API, one series at a time (config4_loop);
-    * config4_batched: the same sample through the batched API
-      (config4_batched);
+    * config4_batched: the whole batched configuration through the batched
+      API (config4_batched);
     * rp: the recurrence plot (rp_bench);
-    * rr_target: the threshold search (rr_target_bench)."""
+    * rr_target: the threshold search (rr_target_bench);
+    * fp32_vs_true_fp64: the fp32 decisions against true fp64 ones;
+    * config5: the million-sample configuration (config5_step)."""
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream(dev)
     out = {}
@@ -479,6 +585,24 @@ def extras(accrqa_amd, L, torch, args, c, x, step):
     me, ve, oe = timed(*args64, accrqa_amd.FORM_EXACT, np.float64)
     mm, vm, om = timed(*args64, accrqa_amd.FORM_MFMA, np.float64)
     mis = int(sum(int((u.view(np.uint64) != v.view(np.uint64)).sum()) for u, v in zip(oe, om)))
+    # the reference decides fp32 input in fp32; how many of its decisions a
+    # true fp64 evaluation of the same samples would flip (diagnostic)
+    t0 = time.perf_counter()
+    dx = torch.from_numpy(x).to(dev)
+    flips = accrqa_amd.fp64_mismatches(dx.data_ptr(), x.size, c["tau"][0], c["emb"][0], c["thr"], 1,
+                                       stream.cuda_stream)
+    del dx
+    nc = x.size - (c["emb"][0] - 1) * c["tau"][0]
+    out["fp32_vs_true_fp64"] = {
+        "per_threshold": [int(v) for v in flips], "total": int(flips.sum()),
+        "frac_of_decisions": float(flips.sum()) / (float(nc) * nc * len(c["thr"])),
+        "seconds": time.perf_counter() - t0,
+        "note": ("diagnostic, not a gate: pairs of the full matrix whose decision in the reference's fp32 "
+                 "arithmetic differs from a true fp64 evaluation of the same fp32 samples and thresholds "
+                 "(k_fp64_diag); the MFMA+recheck form reproduces the fp32 decisions exactly "
+                 "(mismatches_mfma_vs_exact), and for fp64 input it matches the exact fp64 form "
+                 "(fp64_input.mismatches_mfma_vs_exact)")}
+    out["config5"] = config5_step(accrqa_amd, torch)
     out["config4_per_series"] = config4_loop(accrqa_amd)
     out["config4_batched"] = config4_batched(accrqa_amd)
     out["rp"] = rp_bench(accrqa_amd, torch, x, float(c["thr"][5]))

@@ -7,9 +7,13 @@
 // against the reference compiles and links unchanged (the mangled symbols are
 // identical).  The only addition is PLT_AMD_GPU = 2048 in Accrqa_CompPlatform.
 //
-// This library implements PLT_AMD_GPU.  The CPU platform stays in the
-// reference library: a call with any other platform value prints a warning
-// and returns ERR_INVALID_ARGUMENT instead of silently computing elsewhere.
+// This library computes on the AMD GPU only.  Like the reference, which runs
+// a platform it does not have on its default platform (silently for
+// PLT_NV_GPU without CUDA, with a warning for unknown values;
+// src/AccRQA_functions.cpp:525-539), it serves every platform value there:
+// PLT_AMD_GPU and PLT_NV_GPU silently, PLT_CPU after a one-time note (the
+// results equal the reference CPU path's bit for bit), unknown values after
+// the reference's warning.
 #ifndef ACCRQA_AMD_HPP
 #define ACCRQA_AMD_HPP
 

@@ -102,10 +102,11 @@ void accrqa_amd_histograms_f64(const double *input_data, size_t data_size, int t
  * Outputs are device arrays in the reference layout of one (tau, emb) block:
  *   d_rr [T], d_det [T][nL][5], d_lam [nL][T][5]   (any may be NULL to skip it).
  * stream: a hipStream_t (NULL = default stream).  Returns the error code.
- * The call is asynchronous with respect to the host except for the small
- * argument uploads; synchronise the stream before reading the outputs.
- * Calls share the library's work buffers, so calls on different streams
- * must be ordered by the caller (synchronise one before the next). */
+ * The call is asynchronous with respect to the host; synchronise the stream
+ * before reading the outputs.  Calls share the library's work buffers: the
+ * library orders every call after the previous one on the device (the
+ * stream of a call waits for the previous call's completion event when the
+ * streams differ), so calls on several streams serialise rather than race. */
 int accrqa_amd_rqa_device_f32(const float *d_input, size_t data_size, int tau, int emb,
                               const float *threshold_values, int nThresholds, const int *lmin_values, int nL,
                               int distance_type, float *d_rr, float *d_det, float *d_lam, void *stream);
@@ -202,6 +203,18 @@ int accrqa_amd_mx_stats_read(unsigned long long *counts);
  * v_mfma_f32_32x32x16_f16 with the operands in the Gram kernels' lane layout.
  * Returns 0 or an Accrqa_Error code. */
 int accrqa_amd_mfma16_probe(const uint16_t *a, const uint16_t *b, const float *c, float *d);
+
+/* Decisions in fp32 against true fp64 (diagnostic, no counterpart in the
+ * reference): for fp32 series d_input (device memory) and each threshold,
+ * mismatches[t] = the number of pairs (i, j) of the full N_c x N_c matrix
+ * whose decision in the reference's fp32 arithmetic ((eps - dist) >= 0 with
+ * dist summed in order in fp32) differs from the decision in fp64 arithmetic
+ * on the same samples and threshold.  mismatches is a host array of
+ * nThresholds entries; the call returns when they are written.  Returns 0 or
+ * an Accrqa_Error code. */
+int accrqa_amd_fp64_mismatch_f32(const float *d_input, size_t data_size, int tau, int emb,
+                                 const float *threshold_values, int nThresholds, int distance_type,
+                                 unsigned long long *mismatches, void *stream);
 
 /* Kernel timing (tracing aid, no counterpart in the reference): when enabled,
  * every kernel launch is bracketed by HIP events on its own stream and the

@@ -1,4 +1,5 @@
 import json
+import math
 import os
 import sys
 
@@ -78,3 +79,46 @@ def same_bits(a, b):
     a = np.asarray(a)
     b = np.asarray(b)
     return a.dtype == b.dtype and a.shape == b.shape and np.array_equal(a.view(np.uint8), b.view(np.uint8))
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+# ENTR is a sum of p ln p in fp64 in both implementations, in different
+# orders and with different log implementations; stored in the input type.
+ENTR_RTOL = {np.float32: 1e-6, np.float64: 1e-12}
+
+
+def assert_measures(got, exp, dt, what):
+    """got/exp: [..., 5] arrays of one measure tuple layout.  DET/LAM, L/TT,
+    Lmax/TTmax and RR bit for bit; ENTR within ENTR_RTOL relative."""
+    got = np.asarray(got).reshape(-1, 5)
+    exp = np.asarray(exp).reshape(-1, 5)
+    for k in (0, 1, 2, 4):
+        assert same_bits(got[:, k].copy(), exp[:, k].copy()), (what, k, got[:, k], exp[:, k])
+    g, e = got[:, 3].astype(np.float64), exp[:, 3].astype(np.float64)
+    tol = ENTR_RTOL[dt]
+    for a, b in zip(g, e):
+        if math.isnan(b):
+            assert math.isnan(a), (what, a, b)
+        else:
+            assert abs(a - b) <= tol * max(abs(b), 1e-300), (what, a, b)
+
+
+def check_case(amd, oracle, x, taus, embs, lmins, thrs, dist, what):
+    """RR, DET and LAM of the GPU against the oracle's."""
+    dt = x.dtype.type
+    rr = amd.RR(x, taus, embs, thrs, dist)
+    rr_o, e = oracle.RR(x, taus, embs, thrs, dist)
+    assert e == 0
+    assert same_bits(rr.reshape(-1), rr_o), (what, "RR", rr.reshape(-1), rr_o)
+    det = amd.DET(x, taus, embs, lmins, thrs, dist)
+    det_o, e = oracle.DET(x, taus, embs, lmins, thrs, dist)
+    assert e == 0
+    assert_measures(det, det_o, dt, what + " DET")
+    lam = amd.LAM(x, taus, embs, lmins, thrs, dist)
+    lam_o, e = oracle.LAM(x, taus, embs, lmins, thrs, dist)
+    assert e == 0
+    assert_measures(lam, lam_o, dt, what + " LAM")

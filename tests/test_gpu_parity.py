@@ -7,17 +7,12 @@ divided exactly as the reference divides); ENTR within 1e-6 relative for fp32
 outputs and 1e-12 relative for fp64 outputs (the GPU sums p*log(p) in a
 different order with the device log).
 """
-import math
-import sys
-
 import numpy as np
 import pytest
 
-from conftest import case_arrays, dense, same_bits, unhex
+from conftest import assert_measures, case_arrays, check_case, dense, same_bits, say, unhex
 
 pytestmark = pytest.mark.gpu
-
-ENTR_RTOL = {np.float32: 1e-6, np.float64: 1e-12}
 
 
 @pytest.fixture(scope="module")
@@ -25,41 +20,6 @@ def amd():
     import accrqa_amd
     accrqa_amd.load()
     return accrqa_amd
-
-
-def say(*a):
-    print(*a, flush=True)
-    sys.stdout.flush()
-
-
-def assert_measures(got, exp, dt, what):
-    """got/exp: [..., 5] arrays of one measure tuple layout."""
-    got = np.asarray(got).reshape(-1, 5)
-    exp = np.asarray(exp).reshape(-1, 5)
-    for k in (0, 1, 2, 4):
-        assert same_bits(got[:, k].copy(), exp[:, k].copy()), (what, k, got[:, k], exp[:, k])
-    g, e = got[:, 3].astype(np.float64), exp[:, 3].astype(np.float64)
-    tol = ENTR_RTOL[dt]
-    for a, b in zip(g, e):
-        if math.isnan(b):
-            assert math.isnan(a), (what, a, b)
-        else:
-            assert abs(a - b) <= tol * max(abs(b), 1e-300) or abs(a - b) <= 4 * np.finfo(dt).eps * max(abs(b), 1), \
-                (what, a, b)
-
-
-def check_case(amd, oracle, x, taus, embs, lmins, thrs, dist, what):
-    dt = x.dtype.type
-    rr = amd.RR(x, taus, embs, thrs, dist)
-    rr_o, e = oracle.RR(x, taus, embs, thrs, dist)
-    assert e == 0
-    assert same_bits(rr.reshape(-1), rr_o), (what, "RR", rr.reshape(-1), rr_o)
-    det = amd.DET(x, taus, embs, lmins, thrs, dist)
-    det_o, e = oracle.DET(x, taus, embs, lmins, thrs, dist)
-    assert_measures(det, det_o, dt, what + " DET")
-    lam = amd.LAM(x, taus, embs, lmins, thrs, dist)
-    lam_o, e = oracle.LAM(x, taus, embs, lmins, thrs, dist)
-    assert_measures(lam, lam_o, dt, what + " LAM")
 
 
 # ----------------------------------------------------------------- golden fixtures
@@ -244,8 +204,9 @@ def test_argument_errors_on_gpu(amd):
 
 
 def test_cxx_api_platform_dispatch(amd, oracle):
-    """The reference's mangled C++ entry point with PLT_AMD_GPU runs on the GPU;
-    other platform values are refused with ERR_INVALID_ARGUMENT."""
+    """The reference's mangled C++ entry point runs on the GPU with PLT_AMD_GPU
+    and with every other platform value (the reference's fallback to its
+    default platform), giving the oracle's results."""
     import ctypes as C
     L = amd.load()
     f = getattr(L, "_Z10accrqa_DETPfS_mPiiS0_iS0_iS_i15Accrqa_Distancei19Accrqa_CompPlatformP12Accrqa_Error")
@@ -266,8 +227,14 @@ def test_cxx_api_platform_dispatch(amd, oracle):
     assert err.value == 0
     exp, _ = oracle.DET(x, [2], [3], [2, 3], [0.5, 1.0], 1)
     assert_measures(out, exp, np.float32, "cxx DET")
-    f(*args, 1, C.byref(err))
-    assert err.value == 2
+    # PLT_CPU, PLT_NV_GPU and an unknown value are served on the GPU, as the
+    # reference serves platforms it lacks on its default one
+    for plt in (1, 1024, 7):
+        out[:] = 0
+        err.value = -1
+        f(*args, plt, C.byref(err))
+        assert err.value == 0, plt
+        assert_measures(out, exp, np.float32, f"cxx DET platform {plt}")
 
 
 def test_device_entry_point_matches_host(amd, oracle):
@@ -572,11 +539,13 @@ def test_rr_target_gpu(amd, oracle):
 
 
 def test_rqa_batch_matches_single_series(amd, oracle):
-    """The batched call against the single-series calls, bit for bit: series
-    of different lengths (one shorter than a 64-column band, one spanning
-    several segments), thresholds per series (one never recurs), several
-    (tau, emb), both norms, fp32 and fp64, every measure subset; one series
-    also against the oracle directly."""
+    """The batched call (persistent batched kernels) against the single-series
+    calls (the large-N kernels): RR bit for bit, DET/LAM to the parity bar
+    (ENTR, summed in another order, within 1e-6 / 1e-12 relative); series of
+    different lengths (one shorter than a 64-column band, one spanning several
+    segments of the large-N path), thresholds per series (one never recurs),
+    several (tau, emb), both norms, fp32 and fp64, every measure subset; one
+    series also against the oracle directly."""
     rng = np.random.default_rng(73)
     lens = [50, 300, 777, 1500, 2100, 64, 5000]
     taus, embs, lm = [1, 3], [2, 5], [2, 3]
@@ -591,8 +560,8 @@ def test_rqa_batch_matches_single_series(amd, oracle):
                 det = amd.DET(s, taus, embs, lm, thr[k], dist)
                 lam = amd.LAM(s, taus, embs, lm, thr[k], dist)
                 assert same_bits(got["RR"][k], rr), (dt, dist, k, "RR")
-                assert same_bits(got["DET"][k], det), (dt, dist, k, "DET")
-                assert same_bits(got["LAM"][k], lam), (dt, dist, k, "LAM")
+                assert_measures(got["DET"][k], det, dt, f"batch DET {dt.__name__} {dist} {k}")
+                assert_measures(got["LAM"][k], lam, dt, f"batch LAM {dt.__name__} {dist} {k}")
             only = amd.rqa_batch(xs, taus, embs, lm, thr, dist, want=("RR",))
             assert same_bits(only["RR"], got["RR"])
             only = amd.rqa_batch(xs, taus, embs, lm, thr, dist, want=("LAM",))
@@ -659,5 +628,31 @@ def test_compat_module_on_gpu(amd, oracle):
     rp = accrqa.RP(x[:300], 1, 3, 0.4, accrqa.accrqaDistance("euclidean"))
     rp_o, _ = oracle.RP(x[:300], 1, 3, np.float32(0.4), 1)
     assert np.array_equal(rp, rp_o)
+    from pyoracle import rr_target
     t, r = accrqa.RR_target(x, 1, 3, 0.1, accrqa.accrqaDistance("euclidean"))
-    assert abs(r - 0.1) < 0.01 or t > 0
+    want_t, want_rr, _ = rr_target(oracle, x, 1, 3, 0.1, 1, 0.01, 20, 0, 10)
+    assert t == want_t and np.float32(r).tobytes() == want_rr.tobytes(), (t, r, want_t, want_rr)
+
+
+def test_fp64_mismatch_diagnostic(amd, oracle):
+    """The fp32-vs-true-fp64 decision counter against the oracle's two plots:
+    the reference's fp32 decisions and its fp64 decisions on the same fp32
+    samples and thresholds, differing pairs counted over the full matrix.
+    Thirds of integers are not fp32 values, so many distances land within
+    rounding of the thresholds (both norms, ties and near ties)."""
+    import torch
+    rng = np.random.default_rng(5)
+    x = (rng.integers(-9, 10, size=900) / 3.0).astype(np.float32)
+    m = 3
+    dx = torch.from_numpy(x).cuda()
+    for dist in (1, 2):
+        thr = [np.float32(np.sqrt(k) / 3.0) if dist == 1 else np.float32(k / 3.0) for k in (1, 2, 3, 5, 8, 13)]
+        got = amd.fp64_mismatches(dx.data_ptr(), x.size, 1, m, thr, dist)
+        want = []
+        for t in thr:
+            a, _ = oracle.RP(x, 1, m, t, dist)
+            b, _ = oracle.RP(x.astype(np.float64), 1, m, np.float64(t), dist)
+            want.append(int((a != b).sum()))
+        say("fp64 diag", dist, list(got), want)
+        assert [int(v) for v in got] == want, (dist, got, want)
+        assert sum(want) > 0
