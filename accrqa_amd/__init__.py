@@ -97,6 +97,8 @@ def load():
         f = getattr(L, f"accrqa_amd_rqa_batch_{sfx}")
         f.argtypes = [P, P, I, P, I, P, I, P, I, P, I, I, P, P, P, P]
         f.restype = None
+    L.accrqa_amd_rqa_device_static_f32.argtypes = [P, SZ, I, I, P, I, P, I, I, P, P, P, P]
+    L.accrqa_amd_rqa_device_static_f32.restype = I
     L.accrqa_amd_fp64_mismatch_f32.argtypes = [P, SZ, I, I, P, I, I, P, P]
     L.accrqa_amd_fp64_mismatch_f32.restype = I
     L.accrqa_amd_profile.argtypes = [I]
@@ -178,6 +180,22 @@ def mfma16_probe(a, b, c):
     if e:
         raise AccrqaError(e, "mfma16_probe")
     return d
+
+
+def rqa_device_static(d_input, n, tau, emb, threshold_values, lmin_values, distance_type, d_rr=0, d_det=0, d_lam=0,
+                      stream=0):
+    """rqa_device for fp32 series without any device allocation (the
+    library's fixed work memory; at most 32,768 samples, 32 thresholds, 16
+    l_min values).  Calls that overlap on one GPU race."""
+    L = load()
+    thr = np.ascontiguousarray(np.asarray(threshold_values, dtype=np.float32).reshape(-1))
+    lm = _ints(lmin_values)
+    e = L.accrqa_amd_rqa_device_static_f32(C.c_void_p(d_input), n, int(tau), int(emb), _p(thr), thr.size, _p(lm),
+                                           lm.size, _dist(distance_type), C.c_void_p(d_rr or None),
+                                           C.c_void_p(d_det or None), C.c_void_p(d_lam or None),
+                                           C.c_void_p(stream or None))
+    if e != 0:
+        raise AccrqaError(e, L.accrqa_amd_last_error().decode())
 
 
 def fp64_mismatches(d_input, n, tau, emb, threshold_values, distance_type="euclidean", stream=0):

@@ -18,7 +18,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libaccrqa_amd.so")
-SOURCES = [os.path.join(HERE, "csrc", n) for n in ("rqa_engine.hip", "rqa_batch.hip", "accrqa_api.cpp")]
+SOURCES = [os.path.join(HERE, "csrc", n) for n in ("rqa_engine.hip", "rqa_batch.hip", "rqa_static.hip", "accrqa_api.cpp")]
 OBJDIR = os.path.join(HERE, "_build")
 REF_TREE = "/root/reference"
 

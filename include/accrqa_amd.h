@@ -114,6 +114,20 @@ int accrqa_amd_rqa_device_f64(const double *d_input, size_t data_size, int tau, 
                               const double *threshold_values, int nThresholds, const int *lmin_values, int nL,
                               int distance_type, double *d_rr, double *d_det, double *d_lam, void *stream);
 
+/* No counterpart in the reference.  accrqa_amd_rqa_device_f32 for callers
+ * that cannot allocate device memory (code under HIP stream capture, tight
+ * serving loops, harnesses that forbid hipMalloc): same arguments, outputs
+ * and results, but the call allocates nothing and copies nothing from the
+ * host.  Its work memory is a fixed array in device memory (8.2 MiB per GPU,
+ * held until the process ends); the series must fit it: data_size <= 32768
+ * samples, nThresholds <= 32, nL <= 16 (else 2, ERR_INVALID_ARGUMENT, with
+ * nothing launched).  Every call on one GPU uses that GPU's array, so calls
+ * that overlap on one GPU (several streams, several host threads) race; the
+ * caller orders them. */
+int accrqa_amd_rqa_device_static_f32(const float *d_input, size_t data_size, int tau, int emb,
+                                     const float *threshold_values, int nThresholds, const int *lmin_values, int nL,
+                                     int distance_type, float *d_rr, float *d_det, float *d_lam, void *stream);
+
 /* Many series in one call (no counterpart in the reference, whose API takes
  * one series per call: the batched configuration's form).  Series k is
  * series[offsets[k] .. offsets[k+1]) (offsets has nSeries + 1 entries,
