@@ -99,6 +99,9 @@ def load():
         f.restype = None
     L.accrqa_amd_rqa_device_static_f32.argtypes = [P, SZ, I, I, P, I, P, I, I, P, P, P, P]
     L.accrqa_amd_rqa_device_static_f32.restype = I
+    L.accrqa_amd_batch_options.argtypes = [I, I]
+    L.accrqa_amd_batch_options.restype = I
+    L.accrqa_amd_batch_report.restype = C.c_char_p
     L.accrqa_amd_fp64_mismatch_f32.argtypes = [P, SZ, I, I, P, I, I, P, P]
     L.accrqa_amd_fp64_mismatch_f32.restype = I
     L.accrqa_amd_profile.argtypes = [I]
@@ -296,6 +299,23 @@ def rqa_batch(series, tau_values, emb_values, lmin_values, threshold_values, dis
                                               _p(out["LAM"]) if "LAM" in out else None, C.byref(err))
     _raise(err)
     return out
+
+
+def batch_options(autotune=None, graphs=None):
+    """rqa_batch settings: autotune (time the persistent kernel's
+    instantiations on the first large call and keep the fastest) and graphs
+    (capture the batched launches in a hipGraph and replay it for repeated
+    calls of the same shape).  None leaves a setting unchanged."""
+    a = -1 if autotune is None else int(bool(autotune))
+    g = -1 if graphs is None else int(bool(graphs))
+    e = load().accrqa_amd_batch_options(a, g)
+    if e:
+        raise AccrqaError(e, "batch_options")
+
+
+def batch_report():
+    """The batched call's tuning choices and graph builds/replays (text)."""
+    return load().accrqa_amd_batch_report().decode()
 
 
 def RP(input_data, tau, emb, threshold, distance_type="euclidean", out_dtype=np.int8):

@@ -51,14 +51,23 @@ def _run(cmd, cwd=None):
     return r.stdout
 
 
-def build_library(out=LIB):
-    """Each translation unit to an object (in parallel), then one link."""
+def build_library(out=LIB, incremental=False):
+    """Each translation unit to an object (in parallel), then one link.
+    incremental=True keeps an object that is newer than its source and every
+    header of csrc/ and include/ (a development shortcut; build() always
+    compiles everything)."""
     from concurrent.futures import ThreadPoolExecutor
     os.makedirs(OBJDIR, exist_ok=True)
     inc = ["-I" + os.path.join(ROOT, "include")]
+    headers = [os.path.join(d, f) for d in (os.path.join(HERE, "csrc"), os.path.join(ROOT, "include"))
+               for f in os.listdir(d) if f.endswith((".h", ".hpp"))]
+    newest_header = max(os.path.getmtime(h) for h in headers)
 
     def compile_one(src):
         obj = os.path.join(OBJDIR, os.path.basename(src) + ".o")
+        if (incremental and os.path.exists(obj)
+                and os.path.getmtime(obj) > max(os.path.getmtime(src), newest_header)):
+            return obj
         _run([hipcc()] + HIPCC_FLAGS + inc + ["-c", src, "-o", obj])
         return obj
     with ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:

@@ -14,8 +14,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
+#include <map>
 #include <numeric>
+#include <string>
 #include <vector>
 
 #include "accrqa_amd.h"
@@ -43,13 +46,24 @@ struct SBlock {
 };
 
 size_t align16(size_t v) { return (v + 15) & ~(size_t)15; }
+size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
-// Queues one chunk of blocks [s0, s1) of `blocks` (all small, all with the
-// same TG) on st: descriptors up, then fill, k_batch, k_batch_fin.
-template <typename F, int NORM, int TG>
-int run_chunk(const std::vector<SBlock<F>> &blocks, size_t s0, size_t s1, int T, const int *lm, int nl, int kinds,
-              F *d_rr, F *d_det, F *d_lam, hipStream_t st) {
-    Ctx &c = g_ctx;
+// One chunk of the call, planned on the host: its descriptor blob (queue
+// counter, item prefix, problems, block outputs, rows, l_min list) and what
+// its launches need.
+struct ChunkPlan {
+    std::vector<char> blob;
+    size_t o_ipre = 16, o_probs = 0, o_outs = 0, o_rows = 0, o_lm = 0;
+    int np = 0;
+    int64_t nitems = 0, bins = 0, nfin = 0;
+    double pairs = 0;     // pair decisions of its blocks (N_c^2 per block and threshold group)
+    size_t dev_off = 0;   // the blob's place in the call's descriptor buffer
+};
+
+template <typename F, int TG>
+ChunkPlan plan_chunk(const std::vector<SBlock<F>> &blocks, size_t s0, size_t s1, int T, const int *lm, int nl,
+                     int kinds) {
+    ChunkPlan C;
     std::vector<rqa::BatchProb<F, TG>> probs;
     std::vector<int64_t> ipre(1, 0);
     std::vector<rqa::BatchBlockOut> outs;
@@ -60,6 +74,7 @@ int run_chunk(const std::vector<SBlock<F>> &blocks, size_t s0, size_t s1, int T,
         const int live = (int)B.b.size();
         const int64_t hs = B.nc + 1;
         rqa::BatchBlockOut o;
+        std::memset(&o, 0, sizeof o);   // padding too: the blob's bytes key the graph cache
         o.nc = (int)B.nc;
         o.rr = B.out * T;
         o.det = B.out * (int64_t)T * nl;
@@ -70,6 +85,7 @@ int run_chunk(const std::vector<SBlock<F>> &blocks, size_t s0, size_t s1, int T,
         for (int g0 = 0; g0 < live; g0 += TG) {
             const int nt = std::min(TG, live - g0);
             rqa::BatchProb<F, TG> P;
+            std::memset(&P, 0, sizeof P);
             P.xoff = B.xoff;
             P.hoff = bins;
             P.nc = (int)B.nc;
@@ -80,60 +96,182 @@ int run_chunk(const std::vector<SBlock<F>> &blocks, size_t s0, size_t s1, int T,
             for (int t = 0; t < nt; t++)
                 rows[r0 + B.order[g0 + t]] = longlong2{bins + (int64_t)t * hs, bins + (int64_t)(nt + t) * hs};
             bins += 2 * (int64_t)nt * hs;
+            C.pairs += (double)B.nc * (double)B.nc;
             probs.push_back(P);
             const int64_t ni = ((kinds & 1) ? rqa::batch_lam_items(B.nc) : 0) +
                                ((kinds & 2) ? rqa::batch_det_items(B.nc) : 0);
             ipre.push_back(ipre.back() + ni);
         }
     }
-    const int np = (int)probs.size();
-    const int64_t nitems = ipre.back();
-    // one descriptor blob: queue counter, ipre, problems, block outputs, rows, l_min list
-    const size_t o_ipre = 16, o_probs = align16(o_ipre + ipre.size() * sizeof(int64_t));
-    const size_t o_outs = align16(o_probs + probs.size() * sizeof(probs[0]));
-    const size_t o_rows = align16(o_outs + outs.size() * sizeof(outs[0]));
-    const size_t o_lm = align16(o_rows + rows.size() * sizeof(longlong2));
-    const size_t blob = align16(o_lm + (size_t)std::max(nl, 1) * sizeof(int));
-    std::vector<char> h(blob, 0);
-    std::memcpy(h.data() + o_ipre, ipre.data(), ipre.size() * sizeof(int64_t));
-    if (np) std::memcpy(h.data() + o_probs, probs.data(), probs.size() * sizeof(probs[0]));
-    std::memcpy(h.data() + o_outs, outs.data(), outs.size() * sizeof(outs[0]));
-    std::memcpy(h.data() + o_rows, rows.data(), rows.size() * sizeof(longlong2));
-    if (nl > 0) std::memcpy(h.data() + o_lm, lm, (size_t)nl * sizeof(int));
-    int e;
-    if ((e = ensure(c.bdesc, blob))) return e;
-    if ((e = ensure(c.bhist, (size_t)std::max<int64_t>(bins, 1) * sizeof(uint32_t)))) return e;
-    char *d = (char *)c.bdesc.p;
-    if ((e = stage_upload(d + 16, h.data() + 16, blob - 16, st))) return e;
-    uint32_t *hist = (uint32_t *)c.bhist.p;
-    unsigned long long *queue = (unsigned long long *)d;
-    const unsigned fb = (unsigned)std::min<int64_t>((bins + 255) / 256 + 1, 8192);
-    hipLaunchKernelGGL(rqa::k_fill_zero<uint32_t>, dim3(fb), dim3(256), 0, st, hist, (size_t)bins, queue);
-    HIP_TRY(hipGetLastError());
-    if (nitems > 0) {
-        const unsigned g = grid_for(rqa::k_batch<F, NORM, TG>, nitems, c.ncu);
-        hipLaunchKernelGGL((rqa::k_batch<F, NORM, TG>), dim3(g), dim3(rqa::WG), 0, st, (const F *)c.x.p,
-                           (const rqa::BatchProb<F, TG> *)(d + o_probs), (const int64_t *)(d + o_ipre), np, kinds,
-                           queue, hist);
-        HIP_TRY(hipGetLastError());
+    C.np = (int)probs.size();
+    C.nitems = ipre.back();
+    C.bins = bins;
+    C.nfin = (int64_t)outs.size() * T;
+    C.o_probs = align16(C.o_ipre + ipre.size() * sizeof(int64_t));
+    C.o_outs = align16(C.o_probs + probs.size() * sizeof(rqa::BatchProb<F, TG>));
+    C.o_rows = align16(C.o_outs + outs.size() * sizeof(rqa::BatchBlockOut));
+    C.o_lm = align16(C.o_rows + rows.size() * sizeof(longlong2));
+    C.blob.assign(align16(C.o_lm + (size_t)std::max(nl, 1) * sizeof(int)), 0);
+    char *h = C.blob.data();
+    std::memcpy(h + C.o_ipre, ipre.data(), ipre.size() * sizeof(int64_t));
+    if (C.np) std::memcpy(h + C.o_probs, probs.data(), probs.size() * sizeof(rqa::BatchProb<F, TG>));
+    std::memcpy(h + C.o_outs, outs.data(), outs.size() * sizeof(rqa::BatchBlockOut));
+    std::memcpy(h + C.o_rows, rows.data(), rows.size() * sizeof(longlong2));
+    if (nl > 0) std::memcpy(h + C.o_lm, lm, (size_t)nl * sizeof(int));
+    return C;
+}
+
+// ---------------------------------------------------------------- autotuner
+// k_batch comes in instantiations that differ in the waves per SIMD their
+// register allocation must allow (the compiler's choice, 4, 5).  The first
+// call of a (precision, norm, threshold group) with at least TUNE_MIN_PAIRS
+// pair decisions in its first chunk runs that chunk once with each
+// instantiation, timed with HIP events, and keeps the fastest for the rest of
+// the process; every run recomputes the chunk from zeroed histograms, so the
+// results are those of any one run.  accrqa_amd_batch_tuning() reports the
+// choices.
+constexpr int NVAR = 3;
+constexpr int VAR_WPE[NVAR] = {1, 4, 5};
+constexpr double TUNE_MIN_PAIRS = 2e9;
+
+struct TuneEntry {
+    int variant = -1;
+    float ms[NVAR] = {0.f, 0.f, 0.f};
+};
+std::map<int, TuneEntry> g_tune;
+bool g_autotune = true;
+std::string g_tune_report;
+
+int tune_key(int fsize, int norm, int tg) { return fsize * 10000 + norm * 100 + tg; }
+
+template <typename F, int NORM, int TG>
+int launch_batch_variant(int var, unsigned g, const char *d, const ChunkPlan &C, int kinds, uint32_t *hist,
+                         hipStream_t st) {
+    const F *x = (const F *)g_ctx.x.p;
+    const auto *probs = (const rqa::BatchProb<F, TG> *)(d + C.o_probs);
+    const auto *ipre = (const int64_t *)(d + C.o_ipre);
+    auto *queue = (unsigned long long *)d;
+    switch (var) {
+        case 1:
+            hipLaunchKernelGGL((rqa::k_batch<F, NORM, TG, 4>), dim3(g), dim3(rqa::WG), 0, st, x, probs, ipre, C.np,
+                               kinds, queue, hist);
+            break;
+        case 2:
+            hipLaunchKernelGGL((rqa::k_batch<F, NORM, TG, 5>), dim3(g), dim3(rqa::WG), 0, st, x, probs, ipre, C.np,
+                               kinds, queue, hist);
+            break;
+        default:
+            hipLaunchKernelGGL((rqa::k_batch<F, NORM, TG, 1>), dim3(g), dim3(rqa::WG), 0, st, x, probs, ipre, C.np,
+                               kinds, queue, hist);
     }
-    const int64_t nfin = (int64_t)outs.size() * T;
-    hipLaunchKernelGGL((rqa::k_batch_fin<F>), dim3((unsigned)nfin), dim3(rqa::BFIN), 0, st, (const uint32_t *)hist,
-                       (const rqa::BatchBlockOut *)(d + o_outs), (const longlong2 *)(d + o_rows), T,
-                       (const int *)(d + o_lm), nl, d_rr, d_det, d_lam);
     HIP_TRY(hipGetLastError());
     return E_OK;
 }
 
-template <typename F, int NORM>
-int run_small(const std::vector<SBlock<F>> &blocks, int T, const int *lm, int nl, int kinds, F *d_rr, F *d_det,
-              F *d_lam, hipStream_t st) {
-    int maxlive = 0;
-    for (const SBlock<F> &B : blocks) maxlive = std::max(maxlive, (int)B.b.size());
-    const int tg = maxlive <= 8 ? 8 : 16;
+template <typename F, int NORM, int TG>
+unsigned batch_grid(int var, int64_t nitems) {
+    switch (var) {
+        case 1: return grid_for(rqa::k_batch<F, NORM, TG, 4>, nitems, g_ctx.ncu);
+        case 2: return grid_for(rqa::k_batch<F, NORM, TG, 5>, nitems, g_ctx.ncu);
+        default: return grid_for(rqa::k_batch<F, NORM, TG, 1>, nitems, g_ctx.ncu);
+    }
+}
+
+void fill_launch(uint32_t *hist, int64_t bins, unsigned long long *queue, hipStream_t st) {
+    const unsigned fb = (unsigned)std::min<int64_t>((bins + 255) / 256 + 1, 8192);
+    hipLaunchKernelGGL(rqa::k_fill_zero<uint32_t>, dim3(fb), dim3(256), 0, st, hist, (size_t)bins, queue);
+}
+
+// The launches of one chunk whose descriptors are at d (device).
+template <typename F, int NORM, int TG>
+int chunk_launches(int var, const char *d, const ChunkPlan &C, int T, int nl, int kinds, F *d_rr, F *d_det, F *d_lam,
+                   hipStream_t st) {
+    uint32_t *hist = (uint32_t *)g_ctx.bhist.p;
+    fill_launch(hist, C.bins, (unsigned long long *)d, st);
+    HIP_TRY(hipGetLastError());
+    int e;
+    if (C.nitems > 0 &&
+        (e = launch_batch_variant<F, NORM, TG>(var, batch_grid<F, NORM, TG>(var, C.nitems), d, C, kinds, hist, st)))
+        return e;
+    hipLaunchKernelGGL((rqa::k_batch_fin<F>), dim3((unsigned)C.nfin), dim3(rqa::BFIN), 0, st, (const uint32_t *)hist,
+                       (const rqa::BatchBlockOut *)(d + C.o_outs), (const longlong2 *)(d + C.o_rows), T,
+                       (const int *)(d + C.o_lm), nl, d_rr, d_det, d_lam);
+    HIP_TRY(hipGetLastError());
+    return E_OK;
+}
+
+// Times every instantiation on chunk C (descriptors at d) and records the
+// fastest; the histograms are left as the last run wrote them.
+template <typename F, int NORM, int TG>
+int autotune_chunk(TuneEntry &E, const char *d, const ChunkPlan &C, int kinds, hipStream_t st) {
+    uint32_t *hist = (uint32_t *)g_ctx.bhist.p;
+    hipEvent_t ev[2 * NVAR];
+    for (int i = 0; i < 2 * NVAR; i++) HIP_TRY(hipEventCreate(&ev[i]));
+    for (int v = 0; v < NVAR; v++) {
+        fill_launch(hist, C.bins, (unsigned long long *)d, st);
+        HIP_TRY(hipEventRecord(ev[2 * v], st));
+        int e = launch_batch_variant<F, NORM, TG>(v, batch_grid<F, NORM, TG>(v, C.nitems), d, C, kinds, hist, st);
+        if (e) return e;
+        HIP_TRY(hipEventRecord(ev[2 * v + 1], st));
+    }
+    HIP_TRY(hipStreamSynchronize(st));
+    int best = 0;
+    for (int v = 0; v < NVAR; v++) {
+        HIP_TRY(hipEventElapsedTime(&E.ms[v], ev[2 * v], ev[2 * v + 1]));
+        if (E.ms[v] < E.ms[best]) best = v;
+    }
+    for (int i = 0; i < 2 * NVAR; i++) (void)hipEventDestroy(ev[i]);
+    E.variant = best;
+    char line[256];
+    std::snprintf(line, sizeof line, "k_batch<%s,%s,%d>: waves/SIMD auto %.2f ms, 4 %.2f ms, 5 %.2f ms -> %s\n",
+                  sizeof(F) == 4 ? "f32" : "f64", NORM == 1 ? "euclidean" : "maximal", TG, E.ms[0], E.ms[1], E.ms[2],
+                  best == 0 ? "auto" : (best == 1 ? "4" : "5"));
+    g_tune_report += line;
+    return E_OK;
+}
+
+// ---------------------------------------------------------------- graph
+// The launches of a call (all chunks) are captured in a hipGraph and kept
+// with a key of everything they depend on: the descriptor blobs (problems,
+// bounds, offsets), the instantiation and the device buffers' addresses.  A
+// later call with the same key (the same shapes and thresholds, any series
+// values) replays the graph: no descriptor upload and one launch.  The graph
+// holds only kernels (the histograms and the queue counter are zeroed by a
+// kernel).
+struct GraphCache {
+    bool valid = false;
+    uint64_t key = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint64_t replays = 0, builds = 0;
+} g_graph;
+bool g_use_graphs = true;
+
+uint64_t fnv(uint64_t h, const void *p, size_t n) {
+    const unsigned char *b = (const unsigned char *)p;
+    for (size_t i = 0; i < n; i++) {
+        h ^= b[i];
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+void graph_drop() {
+    if (g_graph.exec) (void)hipGraphExecDestroy(g_graph.exec);
+    if (g_graph.graph) (void)hipGraphDestroy(g_graph.graph);
+    g_graph.exec = nullptr;
+    g_graph.graph = nullptr;
+    g_graph.valid = false;
+}
+
+template <typename F, int NORM, int TG>
+int run_small_tg(const std::vector<SBlock<F>> &blocks, int T, const int *lm, int nl, int kinds, F *d_rr, F *d_det,
+                 F *d_lam, hipStream_t st) {
+    Ctx &c = g_ctx;
+    // chunks: blocks up to the histogram budget (at least one)
+    std::vector<ChunkPlan> plans;
     size_t s0 = 0;
     while (s0 < blocks.size()) {
-        // blocks up to the chunk's histogram budget (at least one)
         int64_t bins = 0;
         size_t s1 = s0;
         while (s1 < blocks.size()) {
@@ -142,12 +280,109 @@ int run_small(const std::vector<SBlock<F>> &blocks, int T, const int *lm, int nl
             bins += bb;
             s1++;
         }
-        const int e = tg == 8 ? run_chunk<F, NORM, 8>(blocks, s0, s1, T, lm, nl, kinds, d_rr, d_det, d_lam, st)
-                              : run_chunk<F, NORM, 16>(blocks, s0, s1, T, lm, nl, kinds, d_rr, d_det, d_lam, st);
-        if (e) return e;
+        plans.push_back(plan_chunk<F, TG>(blocks, s0, s1, T, lm, nl, kinds));
         s0 = s1;
     }
+    size_t dbytes = 0;
+    int64_t maxbins = 1;
+    for (ChunkPlan &C : plans) {
+        C.dev_off = dbytes;
+        dbytes += align256(C.blob.size());
+        maxbins = std::max(maxbins, C.bins);
+    }
+    int e;
+    if ((e = ensure(c.bdesc, dbytes)) || (e = ensure(c.bhist, (size_t)maxbins * sizeof(uint32_t)))) return e;
+    char *dbase = (char *)c.bdesc.p;
+
+    TuneEntry &TE = g_tune[tune_key((int)sizeof(F), NORM, TG)];
+    int var = TE.variant >= 0 ? TE.variant : 0;
+    // the first chunk with enough work tunes the instantiation (eagerly)
+    bool tuned_now = false;
+    if (g_autotune && TE.variant < 0) {
+        const ChunkPlan &C = plans[0];
+        if (C.pairs >= TUNE_MIN_PAIRS && C.nitems > 0) {
+            if ((e = stage_upload(dbase + C.dev_off + 16, C.blob.data() + 16, C.blob.size() - 16, st))) return e;
+            if ((e = autotune_chunk<F, NORM, TG>(TE, dbase + C.dev_off, C, kinds, st))) return e;
+            var = TE.variant;
+            tuned_now = true;
+        }
+    }
+    if (tuned_now) {
+        // finish the tuned chunk, then the rest eagerly
+        const ChunkPlan &C0 = plans[0];
+        const char *d0 = dbase + C0.dev_off;
+        hipLaunchKernelGGL((rqa::k_batch_fin<F>), dim3((unsigned)C0.nfin), dim3(rqa::BFIN), 0, st,
+                           (const uint32_t *)c.bhist.p, (const rqa::BatchBlockOut *)(d0 + C0.o_outs),
+                           (const longlong2 *)(d0 + C0.o_rows), T, (const int *)(d0 + C0.o_lm), nl, d_rr, d_det, d_lam);
+        HIP_TRY(hipGetLastError());
+        for (size_t i = 1; i < plans.size(); i++) {
+            const ChunkPlan &C = plans[i];
+            if ((e = stage_upload(dbase + C.dev_off + 16, C.blob.data() + 16, C.blob.size() - 16, st))) return e;
+            if ((e = chunk_launches<F, NORM, TG>(var, dbase + C.dev_off, C, T, nl, kinds, d_rr, d_det, d_lam, st)))
+                return e;
+        }
+        return E_OK;
+    }
+    // key of the call's launches
+    uint64_t key = 1469598103934665603ull;
+    const int params[6] = {(int)sizeof(F), NORM, TG, var, T, nl};
+    key = fnv(key, params, sizeof params);
+    key = fnv(key, &kinds, sizeof kinds);
+    const void *ptrs[6] = {c.x.p, c.bhist.p, c.bdesc.p, d_rr, d_det, d_lam};
+    key = fnv(key, ptrs, sizeof ptrs);
+    for (const ChunkPlan &C : plans) key = fnv(key, C.blob.data(), C.blob.size());
+    if (g_use_graphs && g_graph.valid && g_graph.key == key) {
+        HIP_TRY(hipGraphLaunch(g_graph.exec, st));
+        g_graph.replays++;
+        return E_OK;
+    }
+    for (const ChunkPlan &C : plans)
+        if ((e = stage_upload(dbase + C.dev_off + 16, C.blob.data() + 16, C.blob.size() - 16, st))) return e;
+    if (!g_use_graphs) {
+        for (const ChunkPlan &C : plans)
+            if ((e = chunk_launches<F, NORM, TG>(var, dbase + C.dev_off, C, T, nl, kinds, d_rr, d_det, d_lam, st)))
+                return e;
+        return E_OK;
+    }
+    // a replaced graph may still be running; the grid sizes are looked up
+    // (and cached) before the capture starts
+    if (g_graph.valid) HIP_TRY(hipStreamSynchronize(st));
+    graph_drop();
+    for (const ChunkPlan &C : plans)
+        if (C.nitems > 0) (void)batch_grid<F, NORM, TG>(var, C.nitems);
+    HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+    int ce = E_OK;
+    for (const ChunkPlan &C : plans)
+        if (!ce) ce = chunk_launches<F, NORM, TG>(var, dbase + C.dev_off, C, T, nl, kinds, d_rr, d_det, d_lam, st);
+    hipGraph_t graph = nullptr;
+    const hipError_t he = hipStreamEndCapture(st, &graph);
+    if (ce) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return ce;
+    }
+    HIP_TRY(he);
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (ie != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        return map_err(ie);
+    }
+    g_graph.graph = graph;
+    g_graph.exec = exec;
+    g_graph.key = key;
+    g_graph.valid = true;
+    g_graph.builds++;
+    HIP_TRY(hipGraphLaunch(exec, st));
     return E_OK;
+}
+
+template <typename F, int NORM>
+int run_small(const std::vector<SBlock<F>> &blocks, int T, const int *lm, int nl, int kinds, F *d_rr, F *d_det,
+              F *d_lam, hipStream_t st) {
+    int maxlive = 0;
+    for (const SBlock<F> &B : blocks) maxlive = std::max(maxlive, (int)B.b.size());
+    return maxlive <= 8 ? run_small_tg<F, NORM, 8>(blocks, T, lm, nl, kinds, d_rr, d_det, d_lam, st)
+                        : run_small_tg<F, NORM, 16>(blocks, T, lm, nl, kinds, d_rr, d_det, d_lam, st);
 }
 
 // Many series in one call: every series uploaded once (each followed by its
@@ -307,6 +542,30 @@ void accrqa_amd_rqa_batch_f64(const double *series, const size_t *offsets, int n
                               double *det_out, double *lam_out, int *error) {
     eng::host_batch<double>(series, offsets, nSeries, tau_values, nTaus, emb_values, nEmbs, lmin_values, nL,
                             threshold_values, nThresholds, distance_type, rr_out, det_out, lam_out, error);
+}
+
+int accrqa_amd_batch_options(int autotune, int graphs) {
+    if (autotune < -1 || autotune > 1 || graphs < -1 || graphs > 1) return eng::E_INVALID;
+    std::lock_guard<std::mutex> lk(eng::g_mu);
+    if (autotune >= 0) eng::g_autotune = autotune != 0;
+    if (graphs >= 0) {
+        eng::g_use_graphs = graphs != 0;
+        if (!eng::g_use_graphs && eng::g_graph.valid) {
+            if (eng::g_ctx.stream) (void)hipStreamSynchronize(eng::g_ctx.stream);
+            eng::graph_drop();
+        }
+    }
+    return eng::E_OK;
+}
+
+const char *accrqa_amd_batch_report(void) {
+    std::lock_guard<std::mutex> lk(eng::g_mu);
+    static std::string out;
+    char line[160];
+    std::snprintf(line, sizeof line, "graph: %llu built, %llu replays\n", (unsigned long long)eng::g_graph.builds,
+                  (unsigned long long)eng::g_graph.replays);
+    out = eng::g_tune_report + line;
+    return out.c_str();
 }
 
 }  // extern "C"

@@ -170,9 +170,11 @@ __device__ __forceinline__ void batch_flush(uint32_t *lh, int nt, uint32_t *hrow
 // The persistent kernel.  queue: the item counter (zero before the launch);
 // ipre[p] = the first item of problem p (np + 1 entries); kinds: bit 0 = the
 // LAM bands are items, bit 1 = the DET strips are; hist: the u32 histograms
-// of every problem.
-template <typename F, int NORM, int TG>
-__global__ __launch_bounds__(WG) void k_batch(const F *__restrict__ x, const BatchProb<F, TG> *__restrict__ probs,
+// of every problem.  WPE: the minimum waves per SIMD the register allocation
+// must allow (1 = the compiler's choice); the host picks the fastest
+// instantiation by measurement.
+template <typename F, int NORM, int TG, int WPE = 1>
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(WPE))) void k_batch(const F *__restrict__ x, const BatchProb<F, TG> *__restrict__ probs,
                                               const int64_t *__restrict__ ipre, int np, int kinds,
                                               unsigned long long *__restrict__ queue, uint32_t *__restrict__ hist) {
     __shared__ BatchWave<F, TG> wv[WAVES];

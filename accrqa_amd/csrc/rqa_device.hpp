@@ -1403,10 +1403,14 @@ __global__ __launch_bounds__(WG) RQA_WPE_MX(SCAN) void k_lam_mx(const F *__restr
             pre1 = more && lane + WAVE < rspan ? x[i0 + BLK + WAVE + lane] : 0.f;
             float ni[16];
             tile_row_norms(nrm, i0, h, ni);
-            // real rows of this lane's elements (rows past the segment are padding)
-            uint32_t rowm = 0;
+            // real rows of this lane's elements (rows past the segment are
+            // padding); all of them but in a segment's last block
+            uint32_t rowm = 0xFFFFu;
+            if (nb < BLK) {
+                rowm = 0u;
 #pragma unroll
-            for (int e = 0; e < 16; e++) rowm |= ((8 * (e >> 2) + 4 * h + (e & 3)) < nb ? 1u : 0u) << e;
+                for (int e = 0; e < 16; e++) rowm |= ((8 * (e >> 2) + 4 * h + (e & 3)) < nb ? 1u : 0u) << e;
+            }
             uint32_t Rr[8], fm = 0u;
             uint32_t flw0, flw1;
             {

@@ -152,6 +152,21 @@ void accrqa_amd_rqa_batch_f64(const double *series, const size_t *offsets, int n
                               const double *threshold_values, int nThresholds, int distance_type, double *rr_out,
                               double *det_out, double *lam_out, int *error);
 
+/* Options of the batched call (no counterpart in the reference).
+ * autotune: 1 = the first call of each (precision, norm, threshold group)
+ * whose first chunk has at least 2e9 pair decisions times the persistent
+ * kernel's instantiations (different register budgets, so different waves
+ * per SIMD) on that chunk and keeps the fastest for the process (default);
+ * 0 = use the compiler's register budget.  graphs: 1 = the batched launches
+ * of a call are captured in a hipGraph, replayed by later calls with the same
+ * shapes, thresholds and device buffers (default); 0 = launched directly.
+ * -1 leaves a setting unchanged.  Results are identical either way.  Returns
+ * 0, or 2 for an out-of-range value. */
+int accrqa_amd_batch_options(int autotune, int graphs);
+/* Text report of the batched call's tuning choices (measured times per
+ * instantiation) and graph builds and replays. */
+const char *accrqa_amd_batch_report(void);
+
 /* Recurrence plot.  Replaces the three overloads of accrqa_RP
  * (src/AccRQA_functions.cpp:996-1060, CPU only there:
  * rqa_CPU_R_matrix_parallel, src/AccRQA_CPU_function.cpp:226-244):
