@@ -128,7 +128,7 @@ ChunkPlan plan_chunk(const std::vector<SBlock<F>> &blocks, size_t s0, size_t s1,
 // pair decisions in its first chunk runs that chunk once with each
 // instantiation, timed with HIP events, and keeps the fastest for the rest of
 // the process; every run recomputes the chunk from zeroed histograms, so the
-// results are those of any one run.  accrqa_amd_batch_tuning() reports the
+// results are those of any one run.  accrqa_amd_batch_report() gives the
 // choices.
 constexpr int NVAR = 3;
 constexpr int VAR_WPE[NVAR] = {1, 4, 5};
@@ -235,17 +235,33 @@ int autotune_chunk(TuneEntry &E, const char *d, const ChunkPlan &C, int kinds, h
 // with a key of everything they depend on: the descriptor blobs (problems,
 // bounds, offsets), the instantiation and the device buffers' addresses.  A
 // later call with the same key (the same shapes and thresholds, any series
-// values) replays the graph: no descriptor upload and one launch.  The graph
-// holds only kernels (the histograms and the queue counter are zeroed by a
-// kernel).
-struct GraphCache {
-    bool valid = false;
+// values) replays the graph: one launch.  The graph holds only kernels (the
+// histograms and the queue counter are zeroed by a kernel).  The graph reads
+// its descriptors from the shared descriptor buffer, so a replay uploads them
+// again first unless the buffer still holds that key's blobs (g_desc_key;
+// every upload of other descriptors clears it).  Up to GRAPH_SLOTS graphs are
+// kept, the least recently used replaced first.
+constexpr int GRAPH_SLOTS = 4;
+
+struct GraphSlot {
     uint64_t key = 0;
+    uint64_t used = 0;
+    uint64_t kernels = 0;   // kernel nodes
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+};
+struct GraphCache {
+    GraphSlot slot[GRAPH_SLOTS];
+    uint64_t tick = 0;
     uint64_t replays = 0, builds = 0;
 } g_graph;
 bool g_use_graphs = true;
+uint64_t g_desc_key = 0;   // key whose blobs the descriptor buffer holds (0: none known)
+
+// Counters of the batched calls (accrqa_amd_batch_report).
+struct BatchCounts {
+    uint64_t calls = 0, chunks = 0, kernels = 0, graph_launches = 0, desc_uploads = 0;
+} g_counts;
 
 uint64_t fnv(uint64_t h, const void *p, size_t n) {
     const unsigned char *b = (const unsigned char *)p;
@@ -256,12 +272,21 @@ uint64_t fnv(uint64_t h, const void *p, size_t n) {
     return h;
 }
 
+void slot_drop(GraphSlot &S) {
+    if (S.exec) (void)hipGraphExecDestroy(S.exec);
+    if (S.graph) (void)hipGraphDestroy(S.graph);
+    S = GraphSlot{};
+}
+
+// Drops every kept graph; the caller has synchronised the library's stream.
 void graph_drop() {
-    if (g_graph.exec) (void)hipGraphExecDestroy(g_graph.exec);
-    if (g_graph.graph) (void)hipGraphDestroy(g_graph.graph);
-    g_graph.exec = nullptr;
-    g_graph.graph = nullptr;
-    g_graph.valid = false;
+    for (GraphSlot &S : g_graph.slot) slot_drop(S);
+}
+
+GraphSlot *graph_find(uint64_t key) {
+    for (GraphSlot &S : g_graph.slot)
+        if (S.exec && S.key == key) return &S;
+    return nullptr;
 }
 
 template <typename F, int NORM, int TG>
@@ -293,35 +318,53 @@ int run_small_tg(const std::vector<SBlock<F>> &blocks, int T, const int *lm, int
     int e;
     if ((e = ensure(c.bdesc, dbytes)) || (e = ensure(c.bhist, (size_t)maxbins * sizeof(uint32_t)))) return e;
     char *dbase = (char *)c.bdesc.p;
+    static const void *desc_buf = nullptr;   // the buffer g_desc_key describes
+    if (desc_buf != c.bdesc.p) {
+        g_desc_key = 0;
+        desc_buf = c.bdesc.p;
+    }
 
     TuneEntry &TE = g_tune[tune_key((int)sizeof(F), NORM, TG)];
     int var = TE.variant >= 0 ? TE.variant : 0;
-    // the first chunk with enough work tunes the instantiation (eagerly)
-    bool tuned_now = false;
-    if (g_autotune && TE.variant < 0) {
-        const ChunkPlan &C = plans[0];
-        if (C.pairs >= TUNE_MIN_PAIRS && C.nitems > 0) {
-            if ((e = stage_upload(dbase + C.dev_off + 16, C.blob.data() + 16, C.blob.size() - 16, st))) return e;
-            if ((e = autotune_chunk<F, NORM, TG>(TE, dbase + C.dev_off, C, kinds, st))) return e;
-            var = TE.variant;
-            tuned_now = true;
-        }
-    }
-    if (tuned_now) {
-        // finish the tuned chunk, then the rest eagerly
-        const ChunkPlan &C0 = plans[0];
-        const char *d0 = dbase + C0.dev_off;
-        hipLaunchKernelGGL((rqa::k_batch_fin<F>), dim3((unsigned)C0.nfin), dim3(rqa::BFIN), 0, st,
-                           (const uint32_t *)c.bhist.p, (const rqa::BatchBlockOut *)(d0 + C0.o_outs),
-                           (const longlong2 *)(d0 + C0.o_rows), T, (const int *)(d0 + C0.o_lm), nl, d_rr, d_det, d_lam);
-        HIP_TRY(hipGetLastError());
-        for (size_t i = 1; i < plans.size(); i++) {
+    g_counts.calls++;
+    g_counts.chunks += plans.size();
+    auto upload_all = [&](size_t from) -> int {
+        int ue = E_OK;
+        for (size_t i = from; i < plans.size() && !ue; i++) {
             const ChunkPlan &C = plans[i];
-            if ((e = stage_upload(dbase + C.dev_off + 16, C.blob.data() + 16, C.blob.size() - 16, st))) return e;
-            if ((e = chunk_launches<F, NORM, TG>(var, dbase + C.dev_off, C, T, nl, kinds, d_rr, d_det, d_lam, st)))
-                return e;
+            ue = stage_upload(dbase + C.dev_off + 16, C.blob.data() + 16, C.blob.size() - 16, st);
         }
-        return E_OK;
+        g_counts.desc_uploads++;
+        return ue;
+    };
+    auto eager = [&](size_t from) -> int {
+        int le = E_OK;
+        for (size_t i = from; i < plans.size() && !le; i++) {
+            const ChunkPlan &C = plans[i];
+            le = chunk_launches<F, NORM, TG>(var, dbase + C.dev_off, C, T, nl, kinds, d_rr, d_det, d_lam, st);
+            g_counts.kernels += C.nitems > 0 ? 3 : 2;
+        }
+        return le;
+    };
+    // the first chunk with enough work tunes the instantiation (eagerly)
+    if (g_autotune && TE.variant < 0) {
+        const ChunkPlan &C0 = plans[0];
+        if (C0.pairs >= TUNE_MIN_PAIRS && C0.nitems > 0) {
+            g_desc_key = 0;
+            if ((e = upload_all(0))) return e;
+            if ((e = autotune_chunk<F, NORM, TG>(TE, dbase + C0.dev_off, C0, kinds, st))) return e;
+            var = TE.variant;
+            g_counts.kernels += 2 * NVAR;
+            // finish the tuned chunk, then the rest eagerly
+            const char *d0 = dbase + C0.dev_off;
+            hipLaunchKernelGGL((rqa::k_batch_fin<F>), dim3((unsigned)C0.nfin), dim3(rqa::BFIN), 0, st,
+                               (const uint32_t *)c.bhist.p, (const rqa::BatchBlockOut *)(d0 + C0.o_outs),
+                               (const longlong2 *)(d0 + C0.o_rows), T, (const int *)(d0 + C0.o_lm), nl, d_rr, d_det,
+                               d_lam);
+            HIP_TRY(hipGetLastError());
+            g_counts.kernels++;
+            return eager(1);
+        }
     }
     // key of the call's launches
     uint64_t key = 1469598103934665603ull;
@@ -331,29 +374,37 @@ int run_small_tg(const std::vector<SBlock<F>> &blocks, int T, const int *lm, int
     const void *ptrs[6] = {c.x.p, c.bhist.p, c.bdesc.p, d_rr, d_det, d_lam};
     key = fnv(key, ptrs, sizeof ptrs);
     for (const ChunkPlan &C : plans) key = fnv(key, C.blob.data(), C.blob.size());
-    if (g_use_graphs && g_graph.valid && g_graph.key == key) {
-        HIP_TRY(hipGraphLaunch(g_graph.exec, st));
+    if (key == 0) key = 1;
+    if (g_desc_key != key) {
+        g_desc_key = 0;
+        if ((e = upload_all(0))) return e;
+        g_desc_key = key;
+    }
+    if (!g_use_graphs) return eager(0);
+    if (GraphSlot *S = graph_find(key)) {
+        HIP_TRY(hipGraphLaunch(S->exec, st));
+        S->used = ++g_graph.tick;
         g_graph.replays++;
+        g_counts.graph_launches++;
+        g_counts.kernels += S->kernels;
         return E_OK;
     }
-    for (const ChunkPlan &C : plans)
-        if ((e = stage_upload(dbase + C.dev_off + 16, C.blob.data() + 16, C.blob.size() - 16, st))) return e;
-    if (!g_use_graphs) {
-        for (const ChunkPlan &C : plans)
-            if ((e = chunk_launches<F, NORM, TG>(var, dbase + C.dev_off, C, T, nl, kinds, d_rr, d_det, d_lam, st)))
-                return e;
-        return E_OK;
-    }
-    // a replaced graph may still be running; the grid sizes are looked up
-    // (and cached) before the capture starts
-    if (g_graph.valid) HIP_TRY(hipStreamSynchronize(st));
-    graph_drop();
+    // the least recently used slot; its graph may still be running.  The
+    // grid sizes are looked up (and cached) before the capture starts.
+    GraphSlot *S = &g_graph.slot[0];
+    for (GraphSlot &Q : g_graph.slot)
+        if (Q.used < S->used) S = &Q;
+    if (S->exec) HIP_TRY(hipStreamSynchronize(st));
+    slot_drop(*S);
     for (const ChunkPlan &C : plans)
         if (C.nitems > 0) (void)batch_grid<F, NORM, TG>(var, C.nitems);
     HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
     int ce = E_OK;
-    for (const ChunkPlan &C : plans)
+    uint64_t kern = 0;
+    for (const ChunkPlan &C : plans) {
         if (!ce) ce = chunk_launches<F, NORM, TG>(var, dbase + C.dev_off, C, T, nl, kinds, d_rr, d_det, d_lam, st);
+        kern += C.nitems > 0 ? 3 : 2;
+    }
     hipGraph_t graph = nullptr;
     const hipError_t he = hipStreamEndCapture(st, &graph);
     if (ce) {
@@ -367,12 +418,15 @@ int run_small_tg(const std::vector<SBlock<F>> &blocks, int T, const int *lm, int
         (void)hipGraphDestroy(graph);
         return map_err(ie);
     }
-    g_graph.graph = graph;
-    g_graph.exec = exec;
-    g_graph.key = key;
-    g_graph.valid = true;
+    S->graph = graph;
+    S->exec = exec;
+    S->key = key;
+    S->used = ++g_graph.tick;
+    S->kernels = kern;
     g_graph.builds++;
     HIP_TRY(hipGraphLaunch(exec, st));
+    g_counts.graph_launches++;
+    g_counts.kernels += kern;
     return E_OK;
 }
 
@@ -550,7 +604,7 @@ int accrqa_amd_batch_options(int autotune, int graphs) {
     if (autotune >= 0) eng::g_autotune = autotune != 0;
     if (graphs >= 0) {
         eng::g_use_graphs = graphs != 0;
-        if (!eng::g_use_graphs && eng::g_graph.valid) {
+        if (!eng::g_use_graphs) {
             if (eng::g_ctx.stream) (void)hipStreamSynchronize(eng::g_ctx.stream);
             eng::graph_drop();
         }
@@ -561,9 +615,14 @@ int accrqa_amd_batch_options(int autotune, int graphs) {
 const char *accrqa_amd_batch_report(void) {
     std::lock_guard<std::mutex> lk(eng::g_mu);
     static std::string out;
-    char line[160];
-    std::snprintf(line, sizeof line, "graph: %llu built, %llu replays\n", (unsigned long long)eng::g_graph.builds,
-                  (unsigned long long)eng::g_graph.replays);
+    char line[320];
+    const eng::BatchCounts &K = eng::g_counts;
+    std::snprintf(line, sizeof line,
+                  "batch: %llu calls, %llu chunks, %llu kernels run, %llu graph launches, %llu descriptor uploads\n"
+                  "graph: %llu built, %llu replays\n",
+                  (unsigned long long)K.calls, (unsigned long long)K.chunks, (unsigned long long)K.kernels,
+                  (unsigned long long)K.graph_launches, (unsigned long long)K.desc_uploads,
+                  (unsigned long long)eng::g_graph.builds, (unsigned long long)eng::g_graph.replays);
     out = eng::g_tune_report + line;
     return out.c_str();
 }

@@ -1,11 +1,13 @@
 """The batched call's hipGraph replay and kernel autotuning.
 
-Repeated calls with the same shapes and thresholds replay one captured graph
+Repeated calls with the same shapes and thresholds replay a captured graph
 (k_batch and k_batch_fin per chunk, the histograms zeroed by a kernel): every
-replay, with other library work and synchronisations between replays and new
-series values in some of them, must equal the eager launches and the
-single-series calls (RR bit for bit, DET/LAM to the parity bar).  The first
-large call times the persistent kernel's instantiations and reports them.
+replay, with other library work and synchronisations between replays, new
+series values in some of them, and calls of another graph (the other norm)
+that overwrite the shared descriptor buffer in between, must equal the eager
+launches and the single-series calls (RR bit for bit, DET/LAM to the parity
+bar).  The first large call times the persistent kernel's instantiations and
+reports them.
 """
 import numpy as np
 import pytest
@@ -25,6 +27,13 @@ def _single(amd, xs, c, thr, d, k):
     det = amd.DET(xs[k], c["tau"], c["emb"], c["lmin"], thr[k], d)
     lam = amd.LAM(xs[k], c["tau"], c["emb"], c["lmin"], thr[k], d)
     return rr, det, lam
+
+
+def _graph_counts(rep):
+    """(graphs built, graph replays) from the report's last line."""
+    last = rep.strip().split("\n")[-1]
+    assert last.startswith("graph:"), rep
+    return [int(v) for v in last.replace("graph:", "").replace("built,", "").replace("replays", "").split()]
 
 
 def _same(a, b):
@@ -60,9 +69,26 @@ def test_batch_graph_replay_and_tuning():
             assert _same(o, ref)
         rep = amd.batch_report()
         say(rep)
-        built, replays = [int(v) for v in rep.strip().split("\n")[-1].replace("graph:", "").replace(
-            "built,", "").replace("replays", "").split()]
+        built, replays = _graph_counts(rep)
         assert built >= 1 and replays >= 3, rep
+        # another graph (the maximal norm) overwrites the shared descriptors;
+        # both graphs stay kept, and each replay must still see its own
+        ref2 = None
+        for rep_i in range(3):
+            o2 = amd.rqa_batch(xs, c["tau"], c["emb"], c["lmin"], thr, 2)
+            if ref2 is None:
+                ref2 = o2
+            assert _same(o2, ref2)
+            assert _same(amd.rqa_batch(xs, c["tau"], c["emb"], c["lmin"], thr, 1), ref)
+        for k in (2, 9):
+            rr, det, lam = _single(amd, xs, c, thr, 2, k)
+            assert same_bits(ref2["RR"][k], rr)
+            assert_measures(ref2["DET"][k], det, np.float32, f"graph max-norm DET {k}")
+            assert_measures(ref2["LAM"][k], lam, np.float32, f"graph max-norm LAM {k}")
+        rep = amd.batch_report()
+        say(rep)
+        built2, replays2 = _graph_counts(rep)
+        assert built2 - built <= 1 and replays2 - replays >= 4, rep
         # new series values, same shapes and thresholds: a replay on new data
         xs2 = [np.ascontiguousarray(s[::-1]) for s in xs]
         got = amd.rqa_batch(xs2, c["tau"], c["emb"], c["lmin"], thr, 1)

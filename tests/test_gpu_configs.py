@@ -6,7 +6,8 @@ size-independent properties beyond that).
 * the batched configuration: a sample of its series (all three generators,
   lengths 540 .. 8,364) through the batched call on its full grid
   (tau {1, 2, 4} x m {2, 3, 5} x 8 thresholds x both norms) against the
-  oracle, series by series;
+  oracle, series by series; and all 4,096 series in one batched call per
+  norm against the per-series public API on every 16th series;
 * the AR(1)+sine configuration at m = 32 and m = 64, N = 262,144: the MFMA
   form, the exact form and the direct (no tile store) form agree bit for bit,
   every histogram satisfies sum l*h_V = sum l*h_D = the separate recurrence
@@ -69,6 +70,30 @@ def test_config4_sample_vs_oracle(amd, oracle):
             lam, e = oracle.LAM(s, c["tau"], c["emb"], c["lmin"], thr[k], d)
             assert e == 0
             assert_measures(got["LAM"][k], lam, np.float32, f"config4 {CONFIG4_SAMPLE[k]} LAM {d}")
+
+
+def test_config4_full_batch_vs_single_series(amd):
+    """The whole batched configuration (4,096 series, both norms) in one
+    batched call per norm, against the per-series public API on every 16th
+    series (every (tau, m) block, RR bit for bit, DET/LAM to the parity bar),
+    and RR of every series against a second batched call that asks for RR
+    only (LAM bands alone)."""
+    c = W.config4()
+    xs = c["series"]
+    thr = np.array([[r * s.std() for r in c["thr_rel"]] for s in xs], dtype=np.float32)
+    for d in c["norms"]:
+        say("config4 full batch, norm", d)
+        got = amd.rqa_batch(xs, c["tau"], c["emb"], c["lmin"], thr, d)
+        assert got["RR"].shape == (len(xs), 3, 3, 8)
+        for k in range(0, len(xs), 16):
+            rr = amd.RR(xs[k], c["tau"], c["emb"], thr[k], d)
+            det = amd.DET(xs[k], c["tau"], c["emb"], c["lmin"], thr[k], d)
+            lam = amd.LAM(xs[k], c["tau"], c["emb"], c["lmin"], thr[k], d)
+            assert same_bits(got["RR"][k], rr), (k, d)
+            assert_measures(got["DET"][k], det, np.float32, f"config4 full {k} DET {d}")
+            assert_measures(got["LAM"][k], lam, np.float32, f"config4 full {k} LAM {d}")
+        rr_only = amd.rqa_batch(xs, c["tau"], c["emb"], c["lmin"], thr, d, want=("RR",))
+        assert same_bits(rr_only["RR"], got["RR"]), d
 
 
 def _hists(amd, x, tau, emb, thr, dist, which=7):
