@@ -120,6 +120,10 @@ def load():
     L.accrqa_amd_mx_stats_read.restype = I
     L.accrqa_amd_mfma16_probe.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     L.accrqa_amd_mfma16_probe.restype = I
+    L.accrqa_amd_set_gram.argtypes = [I]
+    L.accrqa_amd_set_gram.restype = I
+    L.accrqa_amd_gram_probe.argtypes = [I, P, P, P, P, P, P]
+    L.accrqa_amd_gram_probe.restype = I
     L.accrqa_amd_last_error.restype = C.c_char_p
     L.accrqa_amd_version.restype = C.c_char_p
     _lib = L
@@ -183,6 +187,36 @@ def mfma16_probe(a, b, c):
     if e:
         raise AccrqaError(e, "mfma16_probe")
     return d
+
+
+GRAM_HILO, GRAM_F16, GRAM_BF16, GRAM_FP8 = 0, 1, 2, 3
+
+
+def set_gram(gram):
+    """Gram operand precision of the split MFMA form's LAM rank pass (the
+    measured comparison of precisions): GRAM_HILO (default), GRAM_F16,
+    GRAM_BF16 or GRAM_FP8; decisions stay exact in every one."""
+    e = load().accrqa_amd_set_gram(int(gram))
+    if e:
+        raise AccrqaError(e, "set_gram")
+
+
+def gram_probe(gram, a, b, c):
+    """One matrix-core instruction of precision gram on fp32 operands a [32,
+    16], b [16, 32] and c [32, 32], rounded as the Gram's windows are: returns
+    (d = c + qa @ qb from the instruction, qa, qb the rounded operands)."""
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    b = np.ascontiguousarray(b, dtype=np.float32)
+    c = np.ascontiguousarray(c, dtype=np.float32)
+    if a.shape != (32, 16) or b.shape != (16, 32) or c.shape != (32, 32):
+        raise ValueError("gram_probe: shapes (32,16), (16,32), (32,32)")
+    d = np.empty((32, 32), dtype=np.float32)
+    qa = np.empty((32, 16), dtype=np.float32)
+    qb = np.empty((16, 32), dtype=np.float32)
+    e = load().accrqa_amd_gram_probe(int(gram), _p(a), _p(b), _p(c), _p(d), _p(qa), _p(qb))
+    if e:
+        raise AccrqaError(e, "gram_probe")
+    return d, qa, qb
 
 
 def rqa_device_static(d_input, n, tau, emb, threshold_values, lmin_values, distance_type, d_rr=0, d_det=0, d_lam=0,

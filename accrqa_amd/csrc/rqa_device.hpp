@@ -928,7 +928,12 @@ __global__ __launch_bounds__(WG) void k_fp64_diag(const float *__restrict__ x, i
 //      <= 14.04u (A + B) + m 2^-25.9, using sqrt(mA) <= 2A + m/8;
 //  * accumulation, allowing each MFMA up to 16 additions of relative error
 //    2u (a faithful adder, any order; h.h of every chunk after all small
-//    products): 2 * 32.2C u (A + B) / 2.
+//    products): 2 * 32.2C u (A + B) / 2.  Measured on the instruction: its
+//    error stays within 4.4u (|c| + sum |p~|), where p~ counts an fp16
+//    subnormal operand at 2^-14, the smallest normal (a product with one is
+//    aligned by that exponent, so against the true products the error can
+//    reach 10^3 u).  With every operand raised to 2^-14 the bound grows by
+//    under 0.03u (A + B) and m 2^-32 in scaled units, inside the spare and dA.
 // For the reference's exact-order fp32 sum s:
 //   |s - s~| <= (3m + 31 + 33C) u (n_i + n_j) + m 2^-25 2^-2e + 1e-30 =: E_ij
 // (exact-order sum: gamma_{m+1} * 2(n_i+n_j); rounded norms 2u; the Gram as
@@ -952,10 +957,27 @@ struct MxParams {
     int qlo;         // base >> sh: bucket 0's number on the key >> sh scale
     float cE;        // (3m + 31 + 33 ceil(m/16)) * 2^-24 (+ 4 * 2^-24 for fp64 input)
     float dA;        // m * 2^-25: the additive part of E_ij in scaled units (x 2^-2e)
+    int amax;        // scaled samples lie below 2^amax (14; 8 for the FP8 Gram)
 };
 
+// Precision of the Gram's operands.  GRAM_HILO is the form every path uses;
+// the others exist for the measured comparison of Gram precisions (split
+// form, fp32 input, up to 16 thresholds) and differ only in the window word,
+// the matrix instruction and the error bound (cE, dA, amax; engine
+// mx_params), so the table, the second stage and the exact re-decision keep
+// every decision exact whatever the precision:
+//   GRAM_F16   one FP16 half per sample (h.h only): |a - h| <= 2^-11 |a| or
+//              2^-25 (subnormal h);
+//   GRAM_BF16  one BF16 half per sample: |a - bf16(a)| <= 2^-8 |a| or
+//              2^-126 (a flushed fp32 subnormal);
+//   GRAM_FP8   one FP8 (E4M3) byte per sample, |a| < 2^8: the bound allows
+//              2^-3 |a| (round-to-nearest gives 2^-4) or 2^-6 for the
+//              subnormal range (a flushed byte included), and an
+//              accumulation error of 2^-9 (|c| + sum |p|) per instruction.
+constexpr int GRAM_HILO = 0, GRAM_F16 = 1, GRAM_BF16 = 2, GRAM_FP8 = 3;
+
 // Scale of the FP16 hi/lo Gram, from the largest norm: samples enter the
-// windows as a = x 2^e with |a| < 2^14 (every sample is a coordinate of some
+// windows as a = x 2^e with |a| < 2^amax = 2^14 (every sample is a coordinate of some
 // row, so |x| <= sqrt(n_max)); the Gram comes back through c2 = -2^(1-2e) in
 // the fma that forms s~ (exact: a power of two); dE = m 2^-25 2^-2e + 1e-30.
 // Norms that are zero, not finite or above 1e30 keep e = 0 (the last two
@@ -970,7 +992,7 @@ __device__ __forceinline__ MxScale mx_scale(const unsigned int *__restrict__ nma
     int e = 0;
     if (nmax > 0.0f && nmax <= 1e30f) {
         const int q = (int)(nb >> 23) - 126;   // n_max < 2^q
-        e = 14 - ((q + 1) >> 1);               // sqrt(n_max) 2^e < 2^14
+        e = P.amax - ((q + 1) >> 1);           // sqrt(n_max) 2^e < 2^amax
         e = e < -60 ? -60 : (e > 60 ? 60 : e);
     }
     MxScale Z;
@@ -1117,6 +1139,19 @@ __device__ __forceinline__ float hl_word(float a) {
                            ((uint32_t)__builtin_bit_cast(uint16_t, lo) << 16));
 }
 
+// The window word of the other Gram precisions (GRAM_*): bf16(a) in bits
+// 0..15 (round to nearest even), or the FP8 (E4M3) byte of a in bits 0..7.
+template <int G>
+__device__ __forceinline__ float win_word(float a) {
+    if constexpr (G == GRAM_BF16) {
+        return __uint_as_float((uint32_t)__builtin_bit_cast(uint16_t, (__bf16)a));
+    } else if constexpr (G == GRAM_FP8) {
+        return __uint_as_float((uint32_t)__builtin_amdgcn_cvt_pk_fp8_f32(a, 0.0f, 0, false) & 0xFFu);
+    } else {
+        return hl_word(a);
+    }
+}
+
 // Gram tile G[a][b] = x_(i0+a) . x_(c0+b), a, b < 32, on MFMA.  Lane l
 // supplies A[row l%32][k] and B[k][col l%32] for the eight dimensions
 // k = 8(l/32) + t, t < 8, of each chunk of 16 (slot t; A and B take the same
@@ -1170,6 +1205,36 @@ __device__ __forceinline__ floatx16 mfma16(halfx8 a, halfx8 b, floatx16 c) {
     return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
 }
 
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+// The eight low bytes of eight window words as the 64-bit FP8 operand
+// (slot t = byte t).
+__device__ __forceinline__ long fp8_bytes(const uint32_t (&w)[8]) {
+    uint32_t v[2];
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+        const uint32_t lo = __builtin_amdgcn_perm(w[4 * i + 1], w[4 * i], 0x0C0C0400u);
+        const uint32_t hi = __builtin_amdgcn_perm(w[4 * i + 3], w[4 * i + 2], 0x04000C0Cu);
+        v[i] = lo | hi;
+    }
+    return (long)(((unsigned long long)v[1] << 32) | v[0]);
+}
+
+// One MFMA of the Gram over one chunk of 16 dimensions in precision G (not
+// GRAM_HILO, whose three products gram_tile issues itself).
+template <int G>
+__device__ __forceinline__ floatx16 mfma_plain(const uint32_t (&a)[8], const uint32_t (&b)[8], floatx16 c) {
+    constexpr uint32_t SH = 0x05040100u;
+    if constexpr (G == GRAM_BF16) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, hl_halves<SH>(a)),
+                                                       __builtin_bit_cast(bf16x8, hl_halves<SH>(b)), c, 0, 0, 0);
+    } else if constexpr (G == GRAM_FP8) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(fp8_bytes(a), fp8_bytes(b), c, 0, 0, 0);
+    } else {
+        return mfma16(hl_halves<SH>(a), hl_halves<SH>(b), c);
+    }
+}
+
 // One MFMA of the Gram's instruction on given operands (diagnostic, one
 // wave): a = A[32][16], b = B[16][32] as fp16 bits, row-major; c = the
 // accumulator C[32][32] in, d = C + A B out.  Lanes supply the operands as
@@ -1193,10 +1258,40 @@ __global__ void k_mfma16_probe(const uint16_t *__restrict__ a, const uint16_t *_
 }
 #endif
 
+// One MFMA of the other Gram precisions on given operands (diagnostic, one
+// wave): a = A[32][16], b = B[16][32], c = C[32][32], row-major fp32; each
+// operand is rounded as the window words of precision G are (win_word), qa
+// and qb get the rounded values as the hardware decodes them, and d = C +
+// qa qb from one mfma_plain<G> in the kernels' lane layout.
+template <int G>
+__global__ void k_gram_probe(const float *__restrict__ a, const float *__restrict__ b, const float *__restrict__ c,
+                             float *__restrict__ d, float *__restrict__ qa, float *__restrict__ qb) {
+    const int lane = threadIdx.x & (WAVE - 1), col = lane & 31, h = lane >> 5;
+    auto dec = [](uint32_t w) -> float {
+        if constexpr (G == GRAM_BF16) return __uint_as_float(w << 16);
+        else if constexpr (G == GRAM_FP8) return __builtin_amdgcn_cvt_f32_fp8((int)w, 0);
+        else return (float)__builtin_bit_cast(_Float16, (uint16_t)(w & 0xFFFFu));
+    };
+    uint32_t wa[8], wb[8];
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+        wa[t] = __float_as_uint(win_word<G>(a[col * 16 + 8 * h + t]));
+        wb[t] = __float_as_uint(win_word<G>(b[(8 * h + t) * 32 + col]));
+        qa[col * 16 + 8 * h + t] = dec(wa[t]);
+        qb[(8 * h + t) * 32 + col] = dec(wb[t]);
+    }
+    floatx16 acc;
+#pragma unroll
+    for (int e = 0; e < 16; e++) acc[e] = c[(8 * (e >> 2) + 4 * h + (e & 3)) * 32 + col];
+    acc = mfma_plain<G>(wa, wb, acc);
+#pragma unroll
+    for (int e = 0; e < 16; e++) d[(8 * (e >> 2) + 4 * h + (e & 3)) * 32 + col] = acc[e];
+}
+
 // One 32x32 Gram tile (column tile p) in scaled units: G[row][32p + col]
 // over the m dimensions.  Order of accumulation: l.h and h.l of every chunk,
 // then h.h of every chunk (the bound on the adds is taken in that order).
-template <int NT, int p>
+template <int NT, int p, int G = GRAM_HILO>
 __device__ __forceinline__ floatx16 gram_tile(const MxWin<NT> &W, const float *const (&pa)[8], int col, int h, int m,
                                               int tau) {
     constexpr int CW = MxWin<NT>::CWO + 32 * p;
@@ -1212,6 +1307,14 @@ __device__ __forceinline__ floatx16 gram_tile(const MxWin<NT> &W, const float *c
             b[t] = __float_as_uint(q[CW]);
         }
     };
+    if constexpr (G != GRAM_HILO) {
+        // one product per chunk, in chunk order
+        for (int k0 = 0; k0 < m; k0 += 16) {
+            load(k0);
+            acc = mfma_plain<G>(a, b, acc);
+        }
+        return acc;
+    }
     if (m <= 16) {
         load(0);
         const halfx8 ah = hl_halves<SH>(a), bh = hl_halves<SH>(b);
@@ -1298,7 +1401,8 @@ __device__ __forceinline__ void tile_row_norms(const float *__restrict__ nrm, in
 // j0 + l, all 32 rows, and the scan is k_lam's.  SCAN = false (split form):
 // ranks and rank tiles only, the column runs left to k_lam_vscan.  STATS
 // (diagnostic instantiation): stats[0..2] += elements ranked, elements the
-// table flagged, elements re-decided in exact order.
+// table flagged, elements re-decided in exact order.  G: the Gram's operand
+// precision (GRAM_HILO everywhere but the precision comparison).
 //
 // F = double (fp64 input): the windows hold the series rounded to fp32 (then
 // scaled and split as for fp32 input, the Gram's bound taken on them), the
@@ -1310,7 +1414,7 @@ __device__ __forceinline__ void tile_row_norms(const float *__restrict__ nrm, in
 // the table's and the second stage's bounds are the fp64 bounds rounded to
 // fp32, off by at most 2u (n_i + n_j) near any s~ that reaches them.  The
 // host adds 4u to the fp32 coefficient cE, which covers all of it.
-template <typename F, int TG, bool SCAN, bool STATS = false>
+template <typename F, int TG, bool SCAN, bool STATS = false, int G = GRAM_HILO>
 __global__ __launch_bounds__(WG) RQA_WPE_MX(SCAN) void k_lam_mx(const F *__restrict__ x, const float *__restrict__ nrm,
                                                const unsigned int *__restrict__ nmax_bits, int64_t nc,
                                                int tau, int m, Bounds<float, TG> S, Bounds<F, TG> SX, int nt,
@@ -1386,15 +1490,15 @@ __global__ __launch_bounds__(WG) RQA_WPE_MX(SCAN) void k_lam_mx(const F *__restr
         for (int t = 0; t < TG; t++) run[t] = jv ? PRE : 0u;
 
         // column window once per item; row window of the first block
-        for (int e = lane; e < cspan; e += WAVE) cw[e] = hl_word((float)x[j0 + e] * Z.s);
+        for (int e = lane; e < cspan; e += WAVE) cw[e] = win_word<G>((float)x[j0 + e] * Z.s);
         float pre0 = lane < rspan ? x[ib + lane] : 0.f;
         float pre1 = lane + WAVE < rspan ? x[ib + WAVE + lane] : 0.f;
 
         for (int64_t i0 = ib; i0 < ie; i0 += BLK) {
             const int nb = ie - i0 < BLK ? (int)(ie - i0) : BLK;
             wave_sync();
-            if (lane < rspan) rw[lane] = hl_word(pre0 * Z.s);
-            if (lane + WAVE < rspan) rw[WAVE + lane] = hl_word(pre1 * Z.s);
+            if (lane < rspan) rw[lane] = win_word<G>(pre0 * Z.s);
+            if (lane + WAVE < rspan) rw[WAVE + lane] = win_word<G>(pre1 * Z.s);
             wave_sync();
             // next block's row window, in flight while this block computes
             // (only if there is a next block: its window ends by x[N + 30])
@@ -1414,7 +1518,7 @@ __global__ __launch_bounds__(WG) RQA_WPE_MX(SCAN) void k_lam_mx(const F *__restr
             uint32_t Rr[8], fm = 0u;
             uint32_t flw0, flw1;
             {
-                const floatx16 acc = gram_tile<2, 0>(Wn, pa, col, h, m, tau);
+                const floatx16 acc = gram_tile<2, 0, G>(Wn, pa, col, h, m, tau);
                 uint32_t Rq[4], f;
                 tile_ranks(acc, ni, nrm[j0 + col], lut_s, P, Z, bnd_s, Rq, f, flw0);
 #pragma unroll
@@ -1422,7 +1526,7 @@ __global__ __launch_bounds__(WG) RQA_WPE_MX(SCAN) void k_lam_mx(const F *__restr
                 if (j0 + col < nc) fm |= f & rowm;
             }
             {
-                const floatx16 acc = gram_tile<2, 1>(Wn, pa, col, h, m, tau);
+                const floatx16 acc = gram_tile<2, 1, G>(Wn, pa, col, h, m, tau);
                 uint32_t Rq[4], f;
                 tile_ranks(acc, ni, nrm[j0 + 32 + col], lut_s, P, Z, bnd_s, Rq, f, flw1);
 #pragma unroll

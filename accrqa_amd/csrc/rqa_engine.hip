@@ -86,6 +86,14 @@ int g_form = 0;
 // LAM/DET split through kept rank tiles: 0 = when the tiles fit, 1 = never.
 int g_split = 0;
 
+// Gram operand precision of the split form's LAM rank pass (rqa::GRAM_*):
+// GRAM_HILO always, but for the measured comparison of precisions.  Another
+// precision applies to the passes that run the split MFMA form (the tile
+// store), which then need fp32 input and at most 16 thresholds per block
+// (ERR_INVALID_ARGUMENT otherwise; groups of up to 16 run at TG = 16); every
+// other pass keeps its usual form.
+int g_gram = 0;
+
 // MFMA-form decision statistics (diagnostic): when on, the split form's rank
 // kernel counts elements ranked, flagged by the table and re-decided in exact
 // order into the context's mxstats buffer (3 u64).
@@ -533,8 +541,22 @@ int build_items(int64_t nc, hipStream_t st) {
 // Parameters of the MFMA form's byte table for sorted bounds b[0..nt).
 // fp64 input: the Gram's inputs and the table's bounds are rounded to fp32;
 // extra_u units of 2^-24 (n_i + n_j) cover that (see the fp64 note at
-// rqa::k_lam_mx).
-rqa::MxParams mx_params(const float *b, int nt, int m, int nbmax, int extra_u = 0) {
+// rqa::k_lam_mx).  gram: the Gram's operand precision (rqa::GRAM_*).  With
+// operands rounded as |a - q(a)| <= eps |a| + eta, the dot product moves by at
+// most sum (2 eps + eps^2)|a||b| + eta (1 + eps)(|a| + |b|) + eta^2, so s~ by
+// R (n_i + n_j) + D with R = 2 eps + eps^2 + 4 eta (1 + eps) and
+// D = eta (1 + eps) m / 2 + 2 m eta^2 (scaled units; sum |a| <= 2A + m/8):
+//   GRAM_F16  eps = 2^-11, eta = 2^-25: R < 16392 u, D < m 2^-25;
+//   GRAM_BF16 eps = 2^-8, eta = 2^-126 (a flushed fp32 subnormal):
+//             R < 131336 u, D < m 2^-25;
+//   GRAM_FP8  eps = 2^-3, eta = 2^-6 (|a| < 2^8): R <= 5636096 u, D < m 2^-6;
+// added to the hi/lo form's coefficient, whose split term they replace (the
+// rest of it, over 2m + 30 u, also covers the accumulation's growth with the
+// rounded operands, at most (1 + eps)^2 - 1 of its 33C u).  The FP8 matrix
+// instruction accumulates more coarsely than the 16-bit ones: measured within
+// 5400 u (|c| + sum |p|) per instruction, taken as 2^-9 (|c| + sum |p|), i.e.
+// at most 2^-9 C (1 + eps)^2 (A + B) < 90000 C u (A + B) more in s~.
+rqa::MxParams mx_params(const float *b, int nt, int m, int nbmax, int extra_u = 0, int gram = rqa::GRAM_HILO) {
     rqa::MxParams P;
     const float lo = b[0] * 0.5f;
     float hi = b[nt - 1] * 2.0f;
@@ -551,9 +573,53 @@ rqa::MxParams mx_params(const float *b, int nt, int m, int nbmax, int extra_u = 
     P.nb = nb;
     P.qlo = (int)(base_al >> sh);
     const int C = (m + 15) / 16;   // chunks of 16 dimensions in the FP16 hi/lo Gram
-    P.cE = (float)((3.0 * m + 31.0 + 33.0 * C + extra_u) * std::ldexp(1.0, -24));
-    P.dA = (float)(m * std::ldexp(1.0, -25));
+    const double gram_u = gram == rqa::GRAM_F16 ? 16392.0
+                          : gram == rqa::GRAM_BF16 ? 131336.0
+                          : gram == rqa::GRAM_FP8  ? 5636100.0 + 90000.0 * C
+                                                   : 0.0;
+    P.cE = (float)((3.0 * m + 31.0 + 33.0 * C + extra_u + gram_u) * std::ldexp(1.0, -24));
+    P.dA = (float)(m * std::ldexp(1.0, gram == rqa::GRAM_FP8 ? -6 : -25));
+    P.amax = gram == rqa::GRAM_FP8 ? 8 : 14;
     return P;
+}
+
+// The split form's LAM rank pass with another Gram precision (the measured
+// comparison of precisions; fp32 input, 16 thresholds), with the decision
+// counters when they are on.
+template <int TG, int G, bool STATS>
+void launch_gram_g(const float *dx, int64_t nc, int tau, int m, const rqa::Bounds<float, TG> &Bf, int nt,
+                   const rqa::MxParams &MP, int L, const rqa::LamWork &LW, unsigned long long *hist, uint2 *carry,
+                   uint8_t *tiles, hipStream_t st) {
+    Ctx &c = g_ctx;
+    const unsigned g = grid_for(rqa::k_lam_mx<float, TG, false, STATS, G>, LW.nitems, c.ncu);
+    hipLaunchKernelGGL((rqa::k_lam_mx<float, TG, false, STATS, G>), dim3(g), dim3(rqa::WG), 0, st, dx,
+                       (const float *)c.nrm.p, (const unsigned int *)c.nmax.p, nc, tau, m, Bf, Bf, nt,
+                       (const uint8_t *)c.lut8.p, MP, L, LW, hist, carry, (uint32_t *)c.priv.p, tiles,
+                       STATS ? (unsigned long long *)c.mxstats.p : (unsigned long long *)nullptr);
+}
+
+template <int TG>
+int launch_gram_variant(int gram, const float *dx, int64_t nc, int tau, int m, const rqa::Bounds<float, TG> &Bf,
+                        int nt, const rqa::MxParams &MP, int L, const rqa::LamWork &LW, unsigned long long *hist,
+                        uint2 *carry, uint8_t *tiles, hipStream_t st) {
+    int e;
+    if (g_mxstats && (e = ensure(g_ctx.mxstats, 3 * sizeof(unsigned long long)))) return e;
+    auto go = [&](auto gk, auto sk) {
+        launch_gram_g<TG, decltype(gk)::value, decltype(sk)::value>(dx, nc, tau, m, Bf, nt, MP, L, LW, hist, carry,
+                                                                   tiles, st);
+    };
+    using F16 = std::integral_constant<int, rqa::GRAM_F16>;
+    using BF16 = std::integral_constant<int, rqa::GRAM_BF16>;
+    using FP8 = std::integral_constant<int, rqa::GRAM_FP8>;
+    using On = std::true_type;
+    using Off = std::false_type;
+    switch (gram) {
+        case rqa::GRAM_F16: g_mxstats ? go(F16{}, On{}) : go(F16{}, Off{}); break;
+        case rqa::GRAM_BF16: g_mxstats ? go(BF16{}, On{}) : go(BF16{}, Off{}); break;
+        default: g_mxstats ? go(FP8{}, On{}) : go(FP8{}, Off{}); break;
+    }
+    HIP_TRY(hipGetLastError());
+    return E_OK;
 }
 
 // One group of up to 32 possibly-recurrent thresholds (sorted bounds).
@@ -608,7 +674,8 @@ int launch_group(const F *dx, int64_t nc, int tau, int m, const F *bounds, int n
             if (use_mfma<F>(1, m, tau)) {
                 rqa::Bounds<float, TG> Bf;
                 for (int t = 0; t < TG; t++) Bf.s[t] = (float)B.s[t];
-                const rqa::MxParams MP = mx_params(Bf.s, nt, m, rqa::NBX_LAM, sizeof(F) == 8 ? 4 : 0);
+                const int gram = tiles ? g_gram : rqa::GRAM_HILO;
+                const rqa::MxParams MP = mx_params(Bf.s, nt, m, rqa::NBX_LAM, sizeof(F) == 8 ? 4 : 0, gram);
                 if ((e = ensure(c.lut8, rqa::NBX_LAM))) return e;
                 hipLaunchKernelGGL((rqa::k_mx_lut<TG>), dim3((MP.nb + 255) / 256), dim3(256), 0, st, Bf, nt, MP,
                                    (const unsigned int *)c.nmax.p, (uint8_t *)c.lut8.p);
@@ -616,7 +683,17 @@ int launch_group(const F *dx, int64_t nc, int tau, int m, const F *bounds, int n
                 if (tiles) {
                     // split form: ranks and tiles, then the column runs from the tiles
                     prof_begin("k_lam_mx", st);
-                    if (g_mxstats) {
+                    if (gram != rqa::GRAM_HILO) {
+                        if constexpr (sizeof(F) == 4 && TG == 16) {
+                            if ((e = launch_gram_variant<TG>(gram, dx, nc, tau, m, Bf, nt, MP, L, LW, hist, carry,
+                                                             tiles, st)))
+                                return e;
+                        } else {
+                            g_last_error = "the Gram precision comparison needs fp32 input and at most 16 "
+                                           "thresholds per block";
+                            return E_INVALID;
+                        }
+                    } else if (g_mxstats) {
                         if ((e = ensure(c.mxstats, 3 * sizeof(unsigned long long)))) return e;
                         const unsigned g = grid_for(rqa::k_lam_mx<F, TG, false, true>, LW.nitems, c.ncu);
                         hipLaunchKernelGGL((rqa::k_lam_mx<F, TG, false, true>), dim3(g), dim3(rqa::WG), 0, st,
@@ -868,7 +945,7 @@ int run_block(const F *dx, size_t n, int tau, int emb, const F *thr, int T, cons
         const int *gperm = (const int *)c.perm.p + g0;
         const rqa::LutEntry<F> *glut = (const rqa::LutEntry<F> *)c.lut.p + (size_t)g * rqa::NB_MAX;
         if (dist == 1)
-            e = launch_tg<F, 1>(tg_for(nt), dx, nc, tau, emb, sorted.data() + g0, nt, glut, lps[g], d_rr != nullptr,
+            e = launch_tg<F, 1>(g_gram != rqa::GRAM_HILO && nt <= 16 ? 16 : tg_for(nt), dx, nc, tau, emb, sorted.data() + g0, nt, glut, lps[g], d_rr != nullptr,
                                 d_det != nullptr, d_lam != nullptr, (const int *)c.lmins.p, nl, gperm, T, d_rr, d_det,
                                 d_lam, st);
         else
@@ -1352,6 +1429,45 @@ int accrqa_amd_rp_device_f32(const float *d_input, size_t data_size, int tau, in
 int accrqa_amd_rp_device_f64(const double *d_input, size_t data_size, int tau, int emb, double threshold,
                              int distance_type, char *d_output, void *stream) {
     return device_rp<double, char>(d_input, data_size, tau, emb, threshold, distance_type, d_output, stream);
+}
+
+int accrqa_amd_set_gram(int gram) {
+    if (gram < rqa::GRAM_HILO || gram > rqa::GRAM_FP8) return E_INVALID;
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_gram = gram;
+    return E_OK;
+}
+
+int accrqa_amd_gram_probe(int gram, const float *a, const float *b, const float *c, float *d, float *qa,
+                          float *qb) {
+    if (!a || !b || !c || !d || !qa || !qb) return E_NULL;
+    if (gram < rqa::GRAM_F16 || gram > rqa::GRAM_FP8) return E_INVALID;
+    std::lock_guard<std::mutex> lk(g_mu);
+    int e = init_ctx();
+    if (e) return e;
+    // a[32x16], b[16x32] at 0 and 2 KiB, c at 4 KiB, d at 8 KiB, qa at 12 KiB, qb at 14 KiB
+    if ((e = ensure(g_ctx.probe, 16384))) return e;
+    char *p = (char *)g_ctx.probe.p;
+    hipStream_t st = g_ctx.stream;
+    CallScope call(st);
+    if ((e = call.err)) return e;
+    HIP_TRY(hipMemcpyAsync(p, a, 2048, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(p + 2048, b, 2048, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(p + 4096, c, 4096, hipMemcpyHostToDevice, st));
+    auto args = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(1), dim3(rqa::WAVE), 0, st, (const float *)p, (const float *)(p + 2048),
+                           (const float *)(p + 4096), (float *)(p + 8192), (float *)(p + 12288),
+                           (float *)(p + 14336));
+    };
+    if (gram == rqa::GRAM_F16) args(rqa::k_gram_probe<rqa::GRAM_F16>);
+    else if (gram == rqa::GRAM_BF16) args(rqa::k_gram_probe<rqa::GRAM_BF16>);
+    else args(rqa::k_gram_probe<rqa::GRAM_FP8>);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(d, p + 8192, 4096, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(qa, p + 12288, 2048, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(qb, p + 14336, 2048, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return E_OK;
 }
 
 int accrqa_amd_mfma16_probe(const uint16_t *a, const uint16_t *b, const float *c, float *d) {

@@ -94,14 +94,14 @@ def tile_store_bytes(nc):
 # (FETCH_SIZE reads 0.5 GiB for it: x2.0; WRITE_SIZE 1 GiB: x1.0).
 _TRAFFIC_SRC = "profiles/ar1_m16_traffic.md"
 MEASURED_TRAFFIC = {
-    ("ar1_m16", "k_lam_mx"): (2.0 * 0.9557e9 + 34.373e9, _TRAFFIC_SRC),
-    ("ar1_m16", "k_lam_vscan"): (2.0 * 17.306e9 + 5.311e9, _TRAFFIC_SRC),
+    ("ar1_m16", "k_lam_mx"): (2.0 * 0.9967e9 + 34.373e9, _TRAFFIC_SRC),
+    ("ar1_m16", "k_lam_vscan"): (2.0 * 17.305e9 + 5.311e9, _TRAFFIC_SRC),
     ("ar1_m16", "k_lam_hscan"): (2.0 * 17.287e9 + 5.291e9, _TRAFFIC_SRC),
-    ("ar1_m16", "k_det_scan"): (2.0 * 25.418e9 + 4.015e9, _TRAFFIC_SRC),
+    ("ar1_m16", "k_det_scan"): (2.0 * 25.343e9 + 4.015e9, _TRAFFIC_SRC),
 }
 
 
-def kernel_rooflines(kern, nc, emb, dist, workload, tile_store):
+def kernel_rooflines(kern, nc, emb, T, dist, workload, tile_store):
     """Roofline figures of the main kernels from their average launch times.
     Algorithmic work per launch (the distinct pairs are the N_c(N_c+1)/2 of
     the upper triangle and the diagonal; each is decided once per launch):
@@ -109,8 +109,8 @@ def kernel_rooflines(kern, nc, emb, dist, workload, tile_store):
                    against the dense FP16 peak; beside it the hi/lo form's
                    3 x 2m issued FLOP and the bytes of rank tiles it writes;
       k_lam        3m (Euclidean) or 2m (maximal) exact-order FP32 VALU
-                   operations per distinct pair against the VALU issue rate
-                   (VALU_PEAK_TOPS);
+                   operations plus the T threshold compares per distinct pair
+                   against the VALU issue rate (VALU_PEAK_TOPS);
       k_lam_vscan, one read of the rank-tile store (the form's intermediate,
       k_lam_hscan, one byte per distinct pair) against 8 TB/s.
       k_det_scan"""
@@ -131,9 +131,14 @@ def kernel_rooflines(kern, nc, emb, dist, workload, tile_store):
                 w = float(tile_store_bytes(nc))
                 out[k]["tile_store_write"] = {"bytes": w, "GB_s": w / t / 1e9, "frac_of_8TB_s": w / t / 8e12}
         elif k == "k_lam":
-            w = pairs * (3 * emb if dist == 1 else 2 * emb)
+            w = pairs * ((3 * emb if dist == 1 else 2 * emb) + T)
+            w_sum = pairs * (3 * emb if dist == 1 else 2 * emb)
             out[k] = {"bound": "valu", "achieved": w / t / 1e12, "peak": VALU_PEAK_TOPS, "unit": "TFLOP/s",
-                      "work_per_launch": w, "work_unit": "FP32 VALU operations (exact-order sum per distinct pair)",
+                      "work_per_launch": w,
+                      "work_unit": ("FP32 VALU operations per distinct pair: the exact-order sum (%s) and "
+                                    "the T threshold compares" % ("3m" if dist == 1 else "2m")),
+                      "sum_only": {"work_per_launch": w_sum, "achieved": w_sum / t / 1e12,
+                                   "frac": w_sum / t / 1e12 / VALU_PEAK_TOPS},
                       "peak_note": "FP32 VALU non-FMA issue rate, VGPR operands (sub/mul/add in order, no FMA)"}
         elif k in ("k_lam_vscan", "k_lam_hscan", "k_det_scan") and tile_store:
             w = float(tile_store_bytes(nc))
@@ -151,6 +156,15 @@ def kernel_rooflines(kern, nc, emb, dist, workload, tile_store):
             r["traffic_note"] = ("HBM bytes per launch: rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by the "
                                  "counters' reading for a device copy of known size; " + tr[1])
     return out
+
+
+_T0 = time.perf_counter()
+
+
+def progress(what):
+    """One line on stderr as each part of the run ends (the JSON line is the
+    only stdout)."""
+    print(f"bench: {what} done at {time.perf_counter() - _T0:.1f} s", file=sys.stderr, flush=True)
 
 
 def cpu_model():
@@ -307,6 +321,7 @@ def main():
         todo = ["exact"]
     res = {f: measure(forms[f]) for f in todo}
     accrqa_amd.set_form(accrqa_amd.FORM_AUTO)
+    progress("headline (%s)" % ", ".join(todo))
     head = "mfma" if "mfma" in res else "exact"
     ms_step, kern, outs, _ = res[head]
     mismatches = None
@@ -321,7 +336,7 @@ def main():
         form_report[f] = {"value": float(nc) * nc * T / (ms_f / 1e3), "ms_per_step": ms_f,
                           "ms_per_step_with_kernel_events": ms_ev,
                           "kernels_ms_per_step": {k: v["ms_total"] / args.steps for k, v in kern_f.items()}}
-    rooflines = {f: kernel_rooflines(res[f][1], nc, emb, dist, args.workload, tiles) for f in res}
+    rooflines = {f: kernel_rooflines(res[f][1], nc, emb, T, dist, args.workload, tiles) for f in res}
     for f in res:
         form_report[f]["rooflines"] = rooflines[f]
     # the dominant kernel of the reported form: the largest share of the step
@@ -359,8 +374,14 @@ def main():
     }
     if not args.no_extras and "mfma" in res:
         out.update(extras(accrqa_amd, L, torch, args, c, x, step))
+        if "exact" in res:
+            out["gram_precision"] = gram_precision(
+                accrqa_amd, L, torch, step,
+                lambda: (d_rr.cpu().numpy(), d_det.cpu().numpy(), d_lam.cpu().numpy()), res["exact"][2])
+            progress("Gram precision comparison")
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(x, tau, emb, thr, lmin, dist, args.cpu_budget)
+        progress("cpu baseline")
     print(json.dumps(out), flush=True)
 
 
@@ -404,7 +425,8 @@ def config4_loop(accrqa_amd, stride=64):
     """The batched configuration (4,096 series, tau {1, 2, 4} x m {2, 3, 5} x 8
     thresholds x both norms) through the public host API, one series at a time
     (RR, DET and LAM calls per series and norm), on the systematic sample of
-    every stride-th series; there is no batched kernel in this build."""
+    every stride-th series: the caller's loop that the batched call
+    (config4_batched) replaces."""
     from accrqa_amd import workloads as W
     c = W.config4(sample=range(0, 4096, stride))
     pairs = 0
@@ -426,24 +448,43 @@ def config4_loop(accrqa_amd, stride=64):
             "path": "per-series loop through accrqa_amd.RR/DET/LAM (host arrays), every %dth series" % stride}
 
 
+def _batch_counts(rep):
+    """The batched engine's counters from accrqa_amd.batch_report()."""
+    out = {}
+    for line in rep.strip().split("\n"):
+        if line.startswith("batch:"):
+            for part in line[len("batch:"):].split(","):
+                v, name = part.strip().split(" ", 1)
+                out[name] = int(v)
+    return out
+
+
 def config4_batched(accrqa_amd):
     """The full batched configuration (4,096 series, tau {1, 2, 4} x m {2, 3,
     5} x 8 thresholds at {0.05 .. 0.40} of each series' std, l_min 2) through
     rqa_batch: one call per norm for all series; every block with
-    N_c <= 65,535 (here all) through the persistent batched kernels.  Timed:
-    the two calls after a warm-up call on two series; the series are made
-    (and held in host memory) beforehand.  RR is checked against the
-    per-series API for the first series."""
+    N_c <= 65,535 (here all) through the persistent batched kernels.  A first
+    call per norm tunes the kernel's register budget and captures the call's
+    launches as a hipGraph (timed and reported as `first_call_seconds`); the
+    timed calls (one per norm) replay the graphs.  The series are made (and
+    held in host memory) beforehand; every call uploads them.  RR is checked
+    against the per-series API for the first series."""
     from accrqa_amd import workloads as W
     t0 = time.perf_counter()
     c = W.config4()
     gen_s = time.perf_counter() - t0
     xs = c["series"]
     thr = np.array([[r * s.std() for r in c["thr_rel"]] for s in xs], dtype=np.float32)
-    accrqa_amd.rqa_batch(xs[:2], c["tau"], c["emb"], c["lmin"], thr[:2], 1)
+    t0 = time.perf_counter()
+    for d in c["norms"]:
+        accrqa_amd.rqa_batch(xs, c["tau"], c["emb"], c["lmin"], thr, d)
+    first_s = time.perf_counter() - t0
+    before = _batch_counts(accrqa_amd.batch_report())
     t0 = time.perf_counter()
     outs = [accrqa_amd.rqa_batch(xs, c["tau"], c["emb"], c["lmin"], thr, d) for d in c["norms"]]
     dt = time.perf_counter() - t0
+    after = _batch_counts(accrqa_amd.batch_report())
+    per_call = {k: (after.get(k, 0) - before.get(k, 0)) / len(c["norms"]) for k in after if k != "calls"}
     same = bool(np.array_equal(outs[0]["RR"][0], accrqa_amd.RR(xs[0], c["tau"], c["emb"], thr[0], 1)))
     pairs = 0
     for s in xs:
@@ -453,8 +494,11 @@ def config4_batched(accrqa_amd):
                 pairs += nc * nc * len(c["thr_rel"]) * len(c["norms"])
     k = len(xs)
     return {"series": k, "seconds": dt, "series_per_s": k / dt, "pair_evals_per_s": pairs / dt,
+            "first_call_seconds": first_s, "per_call": per_call,
+            "tuning": accrqa_amd.batch_report().strip().split("\n")[:-2],
             "generator_seconds": gen_s, "rr_equals_single_series_api": same,
-            "path": "accrqa_amd.rqa_batch, one call per norm over all 4,096 series (persistent batched kernels)"}
+            "path": "accrqa_amd.rqa_batch, one call per norm over all 4,096 series (persistent batched kernels, "
+                    "hipGraph replay)"}
 
 
 def rp_bench(accrqa_amd, torch, x, thr, reps=3):
@@ -517,6 +561,51 @@ def rr_target_bench(accrqa_amd, x):
                 "speedup": out[1][0] / out[5][0], "same_result": out[5][1:] == out[1][1:],
                 "threshold": out[5][1], "rr": out[5][2]})
     return res
+
+
+def gram_precision(accrqa_amd, L, torch, step, outputs, exact_outs, steps=2):
+    """The measured comparison of Gram precisions on the headline workload:
+    the split MFMA form with the rank pass's Gram in FP16 hi/lo (the
+    production form), one FP16 half, one BF16 half and one FP8 (E4M3) byte per
+    sample, each with its own rigorous error bound.  Per precision: ms per
+    step, the rank kernel's time, the elements the table flagged and the
+    fraction re-decided in exact order, and the outputs against the exact
+    form's bit for bit (every precision must give 0 mismatches)."""
+    import ctypes as C
+    names = {0: "fp16_hilo", 1: "fp16", 2: "bf16", 3: "fp8_e4m3"}
+    out = {}
+    accrqa_amd.set_form(accrqa_amd.FORM_MFMA)
+    try:
+        for g, name in names.items():
+            accrqa_amd.set_gram(g)
+            step()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                step()
+            torch.cuda.synchronize()
+            ms = (time.perf_counter() - t0) * 1e3 / steps
+            got = outputs()
+            mis = int(sum(int((u.view(np.uint32) != v.view(np.uint32)).sum()) for u, v in zip(exact_outs, got)))
+            L.accrqa_amd_profile_reset()
+            L.accrqa_amd_profile(1)
+            step()
+            torch.cuda.synchronize()
+            L.accrqa_amd_profile(0)
+            t, n = C.c_double(0), C.c_longlong(0)
+            L.accrqa_amd_profile_read(b"k_lam_mx", C.byref(t), C.byref(n))
+            accrqa_amd.mx_stats(True)
+            step()
+            torch.cuda.synchronize()
+            ranked, flagged, exact = accrqa_amd.mx_stats_read()
+            accrqa_amd.mx_stats(False)
+            out[name] = {"ms_per_step": ms, "k_lam_mx_ms": t.value / max(n.value, 1),
+                         "flagged_frac": flagged / max(ranked, 1), "recheck_frac": exact / max(ranked, 1),
+                         "mismatches_vs_exact": mis}
+    finally:
+        accrqa_amd.set_gram(0)
+        accrqa_amd.set_form(accrqa_amd.FORM_AUTO)
+    return out
 
 
 def extras(accrqa_amd, L, torch, args, c, x, step):
@@ -602,11 +691,16 @@ def extras(accrqa_amd, L, torch, args, c, x, step):
                  "(k_fp64_diag); the MFMA+recheck form reproduces the fp32 decisions exactly "
                  "(mismatches_mfma_vs_exact), and for fp64 input it matches the exact fp64 form "
                  "(fp64_input.mismatches_mfma_vs_exact)")}
+    progress("config3 m=32/64, fp64 input, fp64 diagnostic")
     out["config5"] = config5_step(accrqa_amd, torch)
+    progress("config5")
     out["config4_per_series"] = config4_loop(accrqa_amd)
+    progress("config4 per-series sample")
     out["config4_batched"] = config4_batched(accrqa_amd)
+    progress("config4 batched")
     out["rp"] = rp_bench(accrqa_amd, torch, x, float(c["thr"][5]))
     out["rr_target"] = rr_target_bench(accrqa_amd, x)
+    progress("RP and RR_target")
     out["fp64_input"] = {"exact_ms_per_step": me, "exact_value": ve, "mfma_ms_per_step": mm, "mfma_value": vm,
                          "mfma_vs_exact_speedup": me / mm, "mismatches_mfma_vs_exact": mis,
                          "forms": {"exact": "exact-order FP64 sums on VALU",

@@ -233,6 +233,26 @@ int accrqa_amd_mx_stats_read(unsigned long long *counts);
  * Returns 0 or an Accrqa_Error code. */
 int accrqa_amd_mfma16_probe(const uint16_t *a, const uint16_t *b, const float *c, float *d);
 
+/* Gram operand precision of the split MFMA form's LAM rank pass, for the
+ * measured comparison of precisions (experiment, no counterpart in the
+ * reference): 0 = FP16 hi/lo (the default and the only production form),
+ * 1 = one FP16 half per sample, 2 = one BF16 half, 3 = one FP8 (E4M3) byte.
+ * Every precision keeps every decision exact (its own error bound, then the
+ * exact re-decision); the lower ones flag more pairs for it.  While it is not
+ * 0, a pass that runs the split MFMA form needs fp32 input and at most 16
+ * thresholds per block and returns ERR_INVALID_ARGUMENT otherwise.  Returns 0
+ * or ERR_INVALID_ARGUMENT for an unknown value. */
+int accrqa_amd_set_gram(int gram);
+
+/* One matrix-core instruction of precision gram (1, 2 or 3 as above;
+ * diagnostic): a = A[32][16], b = B[16][32] and c = C[32][32] fp32 row-major
+ * host arrays; each operand is rounded as the Gram's window words are, qa and
+ * qb receive the rounded values as the hardware decodes them, and
+ * d = C + qa qb from one v_mfma_f32_32x32x16_{f16,bf16,fp8_fp8} in the Gram
+ * kernels' lane layout.  Returns 0 or an Accrqa_Error code. */
+int accrqa_amd_gram_probe(int gram, const float *a, const float *b, const float *c, float *d, float *qa,
+                          float *qb);
+
 /* Decisions in fp32 against true fp64 (diagnostic, no counterpart in the
  * reference): for fp32 series d_input (device memory) and each threshold,
  * mismatches[t] = the number of pairs (i, j) of the full N_c x N_c matrix

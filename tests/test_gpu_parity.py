@@ -342,8 +342,12 @@ def test_mfma16_instruction_properties(amd):
     """What the MFMA form's error bound takes of v_mfma_f32_32x32x16_f16, in
     the Gram kernels' lane layout: products exact, fp16 subnormal operands
     kept, and the result within 16 additions of relative error 2^-23 each of
-    the exact c + a b, i.e. |d - (c + ab)| <= 32u (|c| + |a||b|), u = 2^-24.
-    A layout fault (a and b in different dimension slots) fails every case."""
+    the exact c + a b against the operands' nominal sizes, i.e.
+    |d - (c + ab)| <= 32u (|c| + |a~||b~|), u = 2^-24, where a~ counts an fp16
+    subnormal at 2^-14, the smallest normal (the instruction aligns such a
+    product by that exponent: against |a||b| alone the error reaches 10^3 u
+    when subnormal and normal operands meet in one sum).  A layout fault (a
+    and b in different dimension slots) fails every case."""
     rng = np.random.default_rng(3)
     u = 2.0 ** -24
     g16 = 32 * u / (1 - 32 * u)
@@ -357,7 +361,9 @@ def test_mfma16_instruction_properties(amd):
         want = Cc + A @ B   # fp16 products are exact in fp64; 17-term sums off by < 2^-48 relative
         if exact_result:
             assert np.array_equal(d, want), (what, np.abs(d - want).max())
-        bound = g16 * (np.abs(Cc) + np.abs(A) @ np.abs(B)) * (1 + 2.0 ** -40)
+        nA = np.where(A == 0, 0.0, np.maximum(np.abs(A), 2.0 ** -14))
+        nB = np.where(B == 0, 0.0, np.maximum(np.abs(B), 2.0 ** -14))
+        bound = g16 * (np.abs(Cc) + nA @ nB) * (1 + 2.0 ** -40)
         assert np.all(np.abs(d - want) <= bound), (what, (np.abs(d - want) / np.maximum(bound, 1e-300)).max())
 
     z = np.zeros((32, 32), np.float32)
@@ -385,6 +391,14 @@ def test_mfma16_instruction_properties(amd):
           "normal x subnormal")
     assert np.all(amd.mfma16_probe(np.full((32, 16), 2.0 ** -24, np.float16), np.ones((16, 32), np.float16), z)
                   == np.float32(16 * 2.0 ** -24))
+    # subnormal and normal operands in one sum, with and without an accumulator
+    # (the hi/lo Gram's l halves of small samples are such operands)
+    for s in range(4):
+        ea, eb = rng.integers(-30, 13, (32, 16)), rng.integers(-30, 13, (16, 32))
+        a = rng.choice([-1.0, 1.0], (32, 16)) * (1 + rng.random((32, 16))) * 2.0 ** ea
+        b = rng.choice([-1.0, 1.0], (16, 32)) * (1 + rng.random((16, 32))) * 2.0 ** eb
+        c = z if s < 2 else (rng.standard_normal((32, 32)) * 2.0 ** rng.integers(-10, 10, (32, 32))).astype(np.float32)
+        check(a, b, c, f"mixed subnormal {s}")
 
 
 @pytest.mark.gpu
