@@ -123,26 +123,37 @@ ChunkPlan plan_chunk(const std::vector<SBlock<F>> &blocks, size_t s0, size_t s1,
 
 // ---------------------------------------------------------------- autotuner
 // k_batch comes in instantiations that differ in the waves per SIMD their
-// register allocation must allow (the compiler's choice, 4, 5).  The first
+// register allocation must allow (the compiler's choice, 4, 5, 6).  The first
 // call of a (precision, norm, threshold group) with at least TUNE_MIN_PAIRS
 // pair decisions in its first chunk runs that chunk once with each
 // instantiation, timed with HIP events, and keeps the fastest for the rest of
 // the process; every run recomputes the chunk from zeroed histograms, so the
 // results are those of any one run.  accrqa_amd_batch_report() gives the
 // choices.
-constexpr int NVAR = 3;
-constexpr int VAR_WPE[NVAR] = {1, 4, 5};
+constexpr int NVAR = 4;
+constexpr int VAR_WPE[NVAR] = {1, 4, 5, 6};
 constexpr double TUNE_MIN_PAIRS = 2e9;
 
 struct TuneEntry {
     int variant = -1;
-    float ms[NVAR] = {0.f, 0.f, 0.f};
+    float ms[NVAR] = {};
 };
 std::map<int, TuneEntry> g_tune;
 bool g_autotune = true;
 std::string g_tune_report;
 
 int tune_key(int fsize, int norm, int tg) { return fsize * 10000 + norm * 100 + tg; }
+
+// Calls f with the k_batch instantiation of variant var.
+template <typename F, int NORM, int TG, typename Fn>
+auto with_variant(int var, Fn &&f) {
+    switch (var) {
+        case 1: return f(rqa::k_batch<F, NORM, TG, 4>);
+        case 2: return f(rqa::k_batch<F, NORM, TG, 5>);
+        case 3: return f(rqa::k_batch<F, NORM, TG, 6>);
+        default: return f(rqa::k_batch<F, NORM, TG, 1>);
+    }
+}
 
 template <typename F, int NORM, int TG>
 int launch_batch_variant(int var, unsigned g, const char *d, const ChunkPlan &C, int kinds, uint32_t *hist,
@@ -151,30 +162,17 @@ int launch_batch_variant(int var, unsigned g, const char *d, const ChunkPlan &C,
     const auto *probs = (const rqa::BatchProb<F, TG> *)(d + C.o_probs);
     const auto *ipre = (const int64_t *)(d + C.o_ipre);
     auto *queue = (unsigned long long *)d;
-    switch (var) {
-        case 1:
-            hipLaunchKernelGGL((rqa::k_batch<F, NORM, TG, 4>), dim3(g), dim3(rqa::WG), 0, st, x, probs, ipre, C.np,
-                               kinds, queue, hist);
-            break;
-        case 2:
-            hipLaunchKernelGGL((rqa::k_batch<F, NORM, TG, 5>), dim3(g), dim3(rqa::WG), 0, st, x, probs, ipre, C.np,
-                               kinds, queue, hist);
-            break;
-        default:
-            hipLaunchKernelGGL((rqa::k_batch<F, NORM, TG, 1>), dim3(g), dim3(rqa::WG), 0, st, x, probs, ipre, C.np,
-                               kinds, queue, hist);
-    }
+    with_variant<F, NORM, TG>(var, [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(g), dim3(rqa::WG), 0, st, x, probs, ipre, C.np, kinds, queue, hist);
+        return 0;
+    });
     HIP_TRY(hipGetLastError());
     return E_OK;
 }
 
 template <typename F, int NORM, int TG>
 unsigned batch_grid(int var, int64_t nitems) {
-    switch (var) {
-        case 1: return grid_for(rqa::k_batch<F, NORM, TG, 4>, nitems, g_ctx.ncu);
-        case 2: return grid_for(rqa::k_batch<F, NORM, TG, 5>, nitems, g_ctx.ncu);
-        default: return grid_for(rqa::k_batch<F, NORM, TG, 1>, nitems, g_ctx.ncu);
-    }
+    return with_variant<F, NORM, TG>(var, [&](auto kern) { return grid_for(kern, nitems, g_ctx.ncu); });
 }
 
 void fill_launch(uint32_t *hist, int64_t bins, unsigned long long *queue, hipStream_t st) {
@@ -222,10 +220,15 @@ int autotune_chunk(TuneEntry &E, const char *d, const ChunkPlan &C, int kinds, h
     }
     for (int i = 0; i < 2 * NVAR; i++) (void)hipEventDestroy(ev[i]);
     E.variant = best;
-    char line[256];
-    std::snprintf(line, sizeof line, "k_batch<%s,%s,%d>: waves/SIMD auto %.2f ms, 4 %.2f ms, 5 %.2f ms -> %s\n",
-                  sizeof(F) == 4 ? "f32" : "f64", NORM == 1 ? "euclidean" : "maximal", TG, E.ms[0], E.ms[1], E.ms[2],
-                  best == 0 ? "auto" : (best == 1 ? "4" : "5"));
+    std::string line = std::string("k_batch<") + (sizeof(F) == 4 ? "f32" : "f64") + "," +
+                       (NORM == 1 ? "euclidean" : "maximal") + "," + std::to_string(TG) + ">: waves/SIMD";
+    char part[64];
+    for (int v = 0; v < NVAR; v++) {
+        if (v == 0) std::snprintf(part, sizeof part, " auto %.2f ms", E.ms[v]);
+        else std::snprintf(part, sizeof part, ", %d %.2f ms", VAR_WPE[v], E.ms[v]);
+        line += part;
+    }
+    line += " -> " + (best == 0 ? std::string("auto") : std::to_string(VAR_WPE[best])) + "\n";
     g_tune_report += line;
     return E_OK;
 }

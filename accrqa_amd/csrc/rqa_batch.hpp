@@ -14,7 +14,7 @@
 // kernels (dim_step, pack_ranks, rank_planes, GeTree, consume; exact-order
 // sums, rqa_device.hpp), so every histogram is the reference's bit for bit.
 // Each wave keeps the run-length histogram of its current (problem, kind) in
-// its own LDS (lengths < LCAP) and adds it to the problem's u32 histogram in
+// its own LDS (lengths < BLCAP) and adds it to the problem's u32 histogram in
 // HBM when it moves on; longer runs go straight to HBM.  The bucket table of
 // the ranks is built per wave from the problem's sorted bounds.
 #pragma once
@@ -23,8 +23,13 @@
 namespace rqa {
 
 constexpr int BMAX_NC = 65535;   // u32 bins cannot overflow: every count is below N_c^2 < 2^32
-constexpr int BNB = 256;         // bucket table entries per wave
+constexpr int BNB = 128;         // bucket table entries per wave
 constexpr int BWIN = 256;        // DET column window per wave (samples)
+// Run lengths below BLCAP go to the wave's LDS histogram.  Smaller than the
+// large-N kernels' LCAP: a wave's LDS (histogram, table, window) is 4.3 KB for
+// 8 thresholds, so LDS no longer caps the waves per SIMD below what the
+// registers allow; the short series of a batch rarely have runs this long.
+constexpr int BLCAP = 64;
 
 // One problem: a series at x + xoff (n = N_c + (m-1) tau samples, followed
 // by at least PAD zeros in the batch buffer), one (tau, emb), nt <= TG live
@@ -96,7 +101,7 @@ __device__ __forceinline__ LutParams batch_lut(const F *b, int nt, LutEntry<F> *
 // Per-wave LDS of the batched kernel.
 template <typename F, int TG>
 struct BatchWave {
-    uint32_t lh[TG * LCAP];   // run lengths < LCAP of the current (problem, kind), row t at t * LCAP
+    uint32_t lh[TG * BLCAP];   // run lengths < BLCAP of the current (problem, kind), row t at t * BLCAP
     uint32_t dummy[WAVE];
     LutEntry<F> lut[BNB];
     F bnd[TG];
@@ -105,7 +110,7 @@ struct BatchWave {
 
 // The word scan of one 32-step block for every threshold of the group, with
 // runs that start at the sequence's first step counted like any other (the
-// sequence is never cut) and runs of LCAP or more added to row t of `hrow`
+// sequence is never cut) and runs of BLCAP or more added to row t of `hrow`
 // (u32) straight away.
 template <int TG>
 __device__ __forceinline__ void batch_scan(const uint32_t (&R)[8], uint32_t keep, uint32_t force,
@@ -124,8 +129,8 @@ __device__ __forceinline__ void batch_scan(const uint32_t (&R)[8], uint32_t keep
                 if (__builtin_amdgcn_ballot_w64((~nw | run[t]) != 0u) == 0ull) continue;
                 lead_in = false;
             }
-            uint32_t *lht = lh + t * LCAP;
-            consume(
+            uint32_t *lht = lh + t * BLCAP;
+            consume<BLCAP>(
                 nw, run[t], [&](uint32_t len, bool ok) { lds_inc(ok ? &lht[ok ? len : 0u] : dummy); },
                 [&](uint32_t first) {
                     const uint32_t tt = opaque(t);
@@ -143,8 +148,8 @@ __device__ __forceinline__ void batch_close(const uint32_t (&run)[TG], int nt, u
     for (int t = 0; t < TG; t++) {
         const uint32_t len = run[t];
         if (t < nt && len) {
-            if (len < (uint32_t)LCAP)
-                lds_inc(&lh[t * LCAP + len]);
+            if (len < (uint32_t)BLCAP)
+                lds_inc(&lh[t * BLCAP + len]);
             else if ((int64_t)len < hs)
                 atomicAdd(&hrow[(size_t)opaque(t) * hs + len], 1u);
         }
@@ -156,10 +161,10 @@ template <int TG>
 __device__ __forceinline__ void batch_flush(uint32_t *lh, int nt, uint32_t *hrow, int64_t hs) {
     const int lane = threadIdx.x & (WAVE - 1);
     wave_sync();
-    for (int e = lane; e < TG * LCAP; e += WAVE) {
+    for (int e = lane; e < TG * BLCAP; e += WAVE) {
         const uint32_t v = lh[e];
         if (v) {
-            const int t = e / LCAP, len = e - t * LCAP;
+            const int t = e / BLCAP, len = e - t * BLCAP;
             if (t < nt && len < hs) atomicAdd(&hrow[(size_t)t * hs + len], v);
             lh[e] = 0u;
         }
@@ -181,7 +186,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     const int lane = threadIdx.x & (WAVE - 1);
     const int wave = wave_id();
     BatchWave<F, TG> &Wv = wv[wave];
-    for (int e = lane; e < TG * LCAP; e += WAVE) Wv.lh[e] = 0u;
+    for (int e = lane; e < TG * BLCAP; e += WAVE) Wv.lh[e] = 0u;
     const int64_t nitems = ipre[np];
     constexpr int MJ = sizeof(F) == 4 ? 16 : 8;   // column samples kept in registers (LAM)
 

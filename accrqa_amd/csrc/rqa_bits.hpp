@@ -124,19 +124,22 @@ struct GeTree {
 // One 32-step word of one threshold.  nw: bit u set = step u not recurrent
 // (bit 0 = oldest).  run: recurrent steps carried in from earlier words, with
 // PRE added while the segment's first run is open.  emit(len, ok) counts a
-// closed run of length 1 <= len < LCAP when ok, and does nothing countable
+// closed run of length 1 <= len < CAP when ok, and does nothing countable
 // otherwise (the device form always issues its add, to a dummy bin when !ok,
 // so the common case needs no branch); slow(first) takes the run that closes
-// at the word's first non-recurrent step when it is >= LCAP long or is the
+// at the word's first non-recurrent step when it is >= CAP long or is the
 // segment's first run (first >= PRE: its length is first - PRE, 0 included).
-template <class Emit, class Slow>
+// CAP (at least 32) is the caller's LDS histogram size: LCAP for the large-N
+// kernels.
+template <uint32_t CAP = LCAP, class Emit, class Slow>
 RQA_HD void consume(uint32_t nw, uint32_t &run, Emit &&emit, Slow &&slow) {
+    static_assert(CAP >= 32, "inner runs (up to 30 steps) always take the histogram");
     const uint32_t lead = ffbl(nw);     // recurrent steps before the first non-recurrent one
     const uint32_t tail = ffbh(nw);     // recurrent steps after the last non-recurrent one
     const uint32_t first = run + lead;  // the carried run, closed by that first step
     const bool closes = nw != 0u;
-    emit(first, closes && first - 1u < (uint32_t)(LCAP - 1));
-    if (closes && first >= (uint32_t)LCAP) slow(first);
+    emit(first, closes && first - 1u < CAP - 1u);
+    if (closes && first >= CAP) slow(first);
     // runs strictly inside the word: recurrent steps between the first and
     // last non-recurrent ones (for nw == 0 both shifts are by 31 and mid is 0)
     uint32_t mid = ~nw & (0xFFFFFFFFu << (lead & 31u)) & ones_shr(tail);

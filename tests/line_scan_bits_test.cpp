@@ -81,6 +81,7 @@ static Result direct(const std::vector<int> &b) {
     return r;
 }
 
+template <uint32_t CAP>
 static Result scanned(const std::vector<int> &b, uint32_t start_run) {
     const long L = (long)b.size();
     Result r{std::vector<long>(L + 2, 0), -1, -1};
@@ -92,10 +93,11 @@ static Result scanned(const std::vector<int> &b, uint32_t start_run) {
         for (int u = 0; u < nb; u++)
             if (!b[w0 + u]) nw |= 1u << u;
         // steps past the end are filled as recurrent and taken back afterwards
-        consume(
+        consume<CAP>(
             nw, run,
             [&](uint32_t len, bool ok) {
                 if (ok) r.hist[len]++;
+                CHECK(!ok || len < CAP, "emitted %u >= CAP\n", len);
             },
             [&](uint32_t first) {
                 if (first >= PRE) {
@@ -118,6 +120,9 @@ static Result scanned(const std::vector<int> &b, uint32_t start_run) {
     return r;
 }
 
+// CAP: the LDS histogram size of the caller (128 for the large-N kernels, 64
+// for the batched kernel).
+template <uint32_t CAP>
 static void test_consume(std::mt19937_64 &g) {
     for (int it = 0; it < 40000; it++) {
         const long L = 1 + (long)(g() % (it % 10 == 0 ? 2000 : 300));
@@ -132,7 +137,7 @@ static void test_consume(std::mt19937_64 &g) {
             else if (mode == 4) b[k] = (g() % 50) != 0;          // long runs
             else b[k] = (g() % 50) == 0;                         // isolated ones
         }
-        const Result d = direct(b), s = scanned(b, PRE);
+        const Result d = direct(b), s = scanned<CAP>(b, PRE);
         CHECK(d.pre == s.pre, "L=%ld mode=%d pre %ld vs %ld\n", L, mode, d.pre, s.pre);
         CHECK(d.suf == s.suf, "L=%ld mode=%d suf %ld vs %ld\n", L, mode, d.suf, s.suf);
         for (long k = 0; k <= L; k++)
@@ -146,7 +151,8 @@ int main() {
     test_planes<8>(g);
     test_planes<16>(g);
     test_planes<32>(g);
-    test_consume(g);
+    test_consume<LCAP>(g);
+    test_consume<64>(g);
     if (fails) {
         std::printf("FAILED %d checks\n", fails);
         return 1;
