@@ -120,6 +120,8 @@ def load():
     L.accrqa_amd_mx_stats_read.restype = I
     L.accrqa_amd_mfma16_probe.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     L.accrqa_amd_mfma16_probe.restype = I
+    L.accrqa_amd_set_rp_wide.argtypes = [I]
+    L.accrqa_amd_set_rp_wide.restype = I
     L.accrqa_amd_set_gram.argtypes = [I]
     L.accrqa_amd_set_gram.restype = I
     L.accrqa_amd_gram_probe.argtypes = [I, P, P, P, P, P, P]
@@ -187,6 +189,14 @@ def mfma16_probe(a, b, c):
     if e:
         raise AccrqaError(e, "mfma16_probe")
     return d
+
+
+def set_rp_wide(on):
+    """Recurrence plot, exact form: four adjacent columns per lane where the
+    stores align (default on), or one."""
+    e = load().accrqa_amd_set_rp_wide(1 if on else 0)
+    if e:
+        raise AccrqaError(e, "set_rp_wide")
 
 
 GRAM_HILO, GRAM_F16, GRAM_BF16, GRAM_FP8 = 0, 1, 2, 3

@@ -779,6 +779,75 @@ __global__ __launch_bounds__(WG) void k_rp(const F *__restrict__ x, int64_t nc, 
     }
 }
 
+// ---------------------------------------------------------------- k_rp4
+// k_rp with four columns per lane: work item = RP4_ROWS rows x 256 columns,
+// lane l owns columns j0 + 4l .. j0 + 4l + 3, so each row of the block is one
+// store of 4 elements per lane (256 consecutive elements per instruction,
+// against 64 in k_rp).  The host picks it when N_c % 4 == 0 and the plot is
+// aligned for those stores (4 bytes for chars, 16 for ints).  Column samples
+// of columns past N_c are read at N_c - 1 (their elements are not stored);
+// row reads reach x[N + RP4_ROWS - 2] (the series buffer is padded with zeros).
+template <typename F>
+constexpr int rp4_rows() { return sizeof(F) == 4 ? 16 : 8; }
+
+template <typename F, int NORM, typename O>
+__global__ __launch_bounds__(WG) void k_rp4(const F *__restrict__ x, int64_t nc, int tau, int m, F bound,
+                                            O *__restrict__ out) {
+    constexpr int ROWS = rp4_rows<F>();
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wave = wave_id();
+    const int64_t ncb = (nc + 4 * WAVE - 1) / (4 * WAVE);
+    const int64_t nblk = ((nc + ROWS - 1) / ROWS) * ncb;
+    for (int64_t b = (int64_t)blockIdx.x * WAVES + wave; b < nblk; b += (int64_t)gridDim.x * WAVES) {
+        const int64_t I = b / ncb, Jb = b - I * ncb;
+        const int64_t i0 = I * ROWS;
+        const int64_t j = Jb * 4 * WAVE + 4 * lane;   // first of the lane's four columns
+        int64_t jc[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) jc[c] = j + c < nc ? j + c : nc - 1;
+        F s[ROWS][4];
+#pragma unroll
+        for (int u = 0; u < ROWS; u++)
+#pragma unroll
+            for (int c = 0; c < 4; c++) s[u][c] = F(0);
+        for (int k = 0; k < m; k++) {
+            F xj[4];
+#pragma unroll
+            for (int c = 0; c < 4; c++) xj[c] = x[jc[c] + (int64_t)k * tau];
+            const F *xr = x + i0 + (int64_t)k * tau;
+#pragma unroll
+            for (int u = 0; u < ROWS; u++)
+#pragma unroll
+                for (int c = 0; c < 4; c++) s[u][c] = dim_step<NORM>(s[u][c], xr[u], xj[c]);
+        }
+        if (j < nc) {
+            const int nr = nc - i0 < ROWS ? (int)(nc - i0) : ROWS;
+            const bool full = j + 3 < nc;
+#pragma unroll
+            for (int u = 0; u < ROWS; u++) {
+                if (u < nr) {
+                    O *o = out + (i0 + u) * nc + j;
+                    if (full) {
+                        if constexpr (sizeof(O) == 1) {
+                            uint32_t w = 0u;
+#pragma unroll
+                            for (int c = 0; c < 4; c++) w |= (s[u][c] <= bound ? 1u : 0u) << (8 * c);
+                            *(uint32_t *)o = w;
+                        } else {
+                            *(uint4 *)o = make_uint4(s[u][0] <= bound ? 1u : 0u, s[u][1] <= bound ? 1u : 0u,
+                                                     s[u][2] <= bound ? 1u : 0u, s[u][3] <= bound ? 1u : 0u);
+                        }
+                    } else {
+#pragma unroll
+                        for (int c = 0; c < 4; c++)
+                            if (j + c < nc) o[c] = (O)(s[u][c] <= bound ? 1 : 0);
+                    }
+                }
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------- k_rr
 // Work item = (band of 64 columns, row segment q) over rows i <= max column
 // of the band.  Pair weight: 2 if i < j, 1 if i == j, 0 otherwise.

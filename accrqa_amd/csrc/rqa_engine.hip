@@ -94,6 +94,10 @@ int g_split = 0;
 // other pass keeps its usual form.
 int g_gram = 0;
 
+// Recurrence plot, exact form: four columns per lane (k_rp4, where its
+// stores align) or one (k_rp; accrqa_amd_set_rp_wide(0)).
+bool g_rp_wide = true;
+
 // MFMA-form decision statistics (diagnostic): when on, the split form's rank
 // kernel counts elements ranked, flagged by the table and re-decided in exact
 // order into the context's mxstats buffer (3 u64).
@@ -1216,6 +1220,24 @@ int rp_run(const F *dx, size_t n, int tau, int emb, F thr, int dist, O *d_out, h
             return E_OK;
         }
     }
+    // four columns per lane when its 4-element stores align
+    const uintptr_t oa = (uintptr_t)d_out;
+    if (g_rp_wide && nc % 4 == 0 && oa % (sizeof(O) == 1 ? 4 : 16) == 0) {
+        constexpr int R4 = rqa::rp4_rows<F>();
+        const int64_t nb4 = ((nc + R4 - 1) / R4) * ((nc + 4 * rqa::WAVE - 1) / (4 * rqa::WAVE));
+        prof_begin("k_rp4", st);
+        if (dist == 1) {
+            const unsigned g = grid_for(rqa::k_rp4<F, 1, O>, nb4, c.ncu);
+            hipLaunchKernelGGL((rqa::k_rp4<F, 1, O>), dim3(g), dim3(rqa::WG), 0, st, dx, nc, tau, emb, b, d_out);
+        } else {
+            const unsigned g = grid_for(rqa::k_rp4<F, 2, O>, nb4, c.ncu);
+            hipLaunchKernelGGL((rqa::k_rp4<F, 2, O>), dim3(g), dim3(rqa::WG), 0, st, dx, nc, tau, emb, b, d_out);
+        }
+        prof_end(st);
+        HIP_TRY(hipGetLastError());
+        prof_collect(st);
+        return E_OK;
+    }
     prof_begin("k_rp", st);
     if (dist == 1) {
         const unsigned g = grid_for(rqa::k_rp<F, 1, O>, nblk, c.ncu);
@@ -1429,6 +1451,13 @@ int accrqa_amd_rp_device_f32(const float *d_input, size_t data_size, int tau, in
 int accrqa_amd_rp_device_f64(const double *d_input, size_t data_size, int tau, int emb, double threshold,
                              int distance_type, char *d_output, void *stream) {
     return device_rp<double, char>(d_input, data_size, tau, emb, threshold, distance_type, d_output, stream);
+}
+
+int accrqa_amd_set_rp_wide(int on) {
+    if (on < 0 || on > 1) return E_INVALID;
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_rp_wide = on != 0;
+    return E_OK;
 }
 
 int accrqa_amd_set_gram(int gram) {

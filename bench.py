@@ -504,10 +504,12 @@ def config4_batched(accrqa_amd):
 def rp_bench(accrqa_amd, torch, x, thr, reps=3):
     """accrqa_RP through the device entry point, series and plot in HBM, on a
     65,536-row plot (4.3 GB of bytes) of the default workload's series, at
-    m = 16 and m = 3 (tau = 1, Euclidean).  k_rp computes every element of
-    the plot and writes one byte per element; the write rate is priced
-    against 8 TB/s.  Beside it, the reference's own accrqa_RP (char output,
-    OpenMP over rows) timed on a 6,000-row prefix."""
+    m = 16 (k_rp_mx, the MFMA form) and m = 3 (the exact form: k_rp4, four
+    adjacent columns per lane), tau = 1, Euclidean; one byte per element
+    written, priced against 8 TB/s.  At m = 3 `one_column_form_ms` is k_rp
+    (one column per lane, accrqa_amd_set_rp_wide(0)), with the same plot.
+    Beside it, the reference's own accrqa_RP (char output, OpenMP over rows)
+    timed on a 6,000-row prefix."""
     dev = torch.device("cuda:0")
     st = torch.cuda.current_stream()
     res = {}
@@ -516,19 +518,28 @@ def rp_bench(accrqa_amd, torch, x, thr, reps=3):
         xs = np.ascontiguousarray(x[:nc + m - 1])
         dx = torch.from_numpy(xs).to(dev)
         dout = torch.empty(nc * nc, dtype=torch.int8, device=dev)
-        accrqa_amd.rp_device(dx.data_ptr(), xs.size, 1, m, thr, 1, dout.data_ptr(), st.cuda_stream)
-        torch.cuda.synchronize()
-        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        a.record(st)
-        for _ in range(reps):
-            accrqa_amd.rp_device(dx.data_ptr(), xs.size, 1, m, thr, 1, dout.data_ptr(), st.cuda_stream)
-        b.record(st)
-        torch.cuda.synchronize()
-        t = a.elapsed_time(b) / reps / 1e3
+        times, ones = {}, {}
+        try:
+            for wide in ((0, 1) if m == 3 else (1,)):
+                accrqa_amd.set_rp_wide(wide)
+                accrqa_amd.rp_device(dx.data_ptr(), xs.size, 1, m, thr, 1, dout.data_ptr(), st.cuda_stream)
+                torch.cuda.synchronize()
+                a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                a.record(st)
+                for _ in range(reps):
+                    accrqa_amd.rp_device(dx.data_ptr(), xs.size, 1, m, thr, 1, dout.data_ptr(), st.cuda_stream)
+                b.record(st)
+                torch.cuda.synchronize()
+                times[wide] = a.elapsed_time(b) / reps / 1e3
+                ones[wide] = int(dout.sum(dtype=torch.int64).item())
+        finally:
+            accrqa_amd.set_rp_wide(1)
+        t = times[1]
         el = float(nc) * nc
         res[f"m{m}"] = {"n_c": nc, "ms_per_plot": t * 1e3, "elements_per_s": el / t,
-                        "write_GB_s": el / t / 1e9, "write_frac_of_8TB_s": el / t / 8e12,
-                        "ones": int(dout.sum(dtype=torch.int64).item())}
+                        "write_GB_s": el / t / 1e9, "write_frac_of_8TB_s": el / t / 8e12, "ones": ones[1]}
+        if 0 in times:
+            res[f"m{m}"].update({"one_column_form_ms": times[0] * 1e3, "same_plot_ones": ones[0] == ones[1]})
         del dout, dx
         torch.cuda.empty_cache()
     sys.path.insert(0, os.path.join(ROOT, "oracle"))

@@ -233,6 +233,13 @@ int accrqa_amd_mx_stats_read(unsigned long long *counts);
  * Returns 0 or an Accrqa_Error code. */
 int accrqa_amd_mfma16_probe(const uint16_t *a, const uint16_t *b, const float *c, float *d);
 
+/* Recurrence plot, exact form (tuning aid, no counterpart in the reference):
+ * 1 (the default) gives each lane four adjacent columns, so every store
+ * writes 256 consecutive elements, where N_c % 4 == 0 and the plot's address
+ * aligns those stores; 0 keeps one column per lane.  The plots are identical.
+ * Returns 0 or ERR_INVALID_ARGUMENT. */
+int accrqa_amd_set_rp_wide(int on);
+
 /* Gram operand precision of the split MFMA form's LAM rank pass, for the
  * measured comparison of precisions (experiment, no counterpart in the
  * reference): 0 = FP16 hi/lo (the default and the only production form),

@@ -587,6 +587,51 @@ def test_rqa_batch_matches_single_series(amd, oracle):
         amd.rqa_batch([np.zeros(10, np.float32), np.zeros(3, np.float32)], [1], [5], [2], [0.1], 1)
 
 
+def test_rp_wide_form(amd, oracle):
+    """The exact-form recurrence plot with four adjacent columns per lane
+    (k_rp4: blocks of 16 or 8 rows x 256 columns, one 4-element store per lane
+    and row), taken when N_c % 4 == 0 and the plot is aligned: N_c below,
+    at and past the block sizes (partial last blocks), against the oracle and
+    against one column per lane (accrqa_amd_set_rp_wide(0)), both norms,
+    fp32 and fp64, char and int, tau = 0; a plot at an odd device address
+    takes the one-column form."""
+    import torch
+    rng = np.random.default_rng(83)
+    amd.set_form(amd.FORM_EXACT)
+    try:
+        for nc, tau, emb in ((4, 1, 2), (32, 1, 1), (100, 2, 2), (256, 1, 4), (260, 0, 3), (1028, 1, 5),
+                             (1040, 3, 9)):
+            n = nc + (emb - 1) * tau
+            w = np.cumsum(rng.standard_normal(n))
+            w = (w - w.mean()) / w.std()
+            for dt in (np.float32, np.float64):
+                x = w.astype(dt)
+                for dist in (1, 2):
+                    thr = dt(0.3 * np.sqrt(emb) if dist == 1 else 0.3)
+                    want, e = oracle.RP(x, tau, emb, thr, dist)
+                    assert e == 0
+                    for wide in (1, 0):
+                        amd.set_rp_wide(wide)
+                        say("rp wide", nc, tau, emb, dt.__name__, dist, wide)
+                        assert np.array_equal(amd.RP(x, tau, emb, thr, dist), want), (nc, dt, dist, wide)
+                    amd.set_rp_wide(1)
+                    if dt == np.float64:
+                        want32, _ = oracle.RP(x, tau, emb, thr, dist, np.int32)
+                        assert np.array_equal(amd.RP(x, tau, emb, thr, dist, np.int32), want32), (nc, dist)
+        # the device entry point into a plot at an odd address
+        x = np.cumsum(rng.standard_normal(96 + 2)).astype(np.float32)
+        want, _ = oracle.RP(x, 1, 3, np.float32(0.9), 1)
+        dx = torch.from_numpy(x).cuda()
+        buf = torch.full((96 * 96 + 1,), 7, dtype=torch.int8, device="cuda")
+        amd.rp_device(dx.data_ptr(), x.size, 1, 3, 0.9, 1, buf.data_ptr() + 1)
+        torch.cuda.synchronize()
+        got = buf.cpu().numpy()
+        assert got[0] == 7 and np.array_equal(got[1:].reshape(96, 96), want)
+    finally:
+        amd.set_rp_wide(1)
+        amd.set_form(amd.FORM_AUTO)
+
+
 def test_rp_mfma_form(amd, oracle):
     """The recurrence plot through the MFMA form (hi/lo Gram, one-bound
     table, exact recheck) against the oracle, bit for bit, also with the form
