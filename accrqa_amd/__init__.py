@@ -114,6 +114,8 @@ def load():
     L.accrqa_amd_set_form.restype = I
     L.accrqa_amd_set_split.argtypes = [I]
     L.accrqa_amd_set_split.restype = I
+    L.accrqa_amd_set_slab_rows.argtypes = [I]
+    L.accrqa_amd_set_slab_rows.restype = I
     L.accrqa_amd_mx_stats.argtypes = [I]
     L.accrqa_amd_mx_stats.restype = I
     L.accrqa_amd_mx_stats_read.argtypes = [C.c_void_p]
@@ -158,6 +160,15 @@ def set_split(mode):
     e = load().accrqa_amd_set_split(int(mode))
     if e:
         raise AccrqaError(e, f"split mode {mode}")
+
+
+def set_slab_rows(rows):
+    """Slab form of the split: 0 = automatic (slabs only when the tile store
+    of all rows does not fit), rows > 0 = always, slabs of that many rows
+    (a multiple of 64, at least 256)."""
+    e = load().accrqa_amd_set_slab_rows(int(rows))
+    if e:
+        raise AccrqaError(e, "set_slab_rows")
 
 
 def mx_stats(enable=True):

@@ -43,7 +43,15 @@ struct Ctx {
     int ncu = 0;
     hipStream_t stream = nullptr;
     DevBuf x, hist, carry, metric, scan, counts, perm, lmins, lut, nrm, nmax, lut8, out_rr, out_det, out_lam, priv,
-        tiles, dump, tops, mxstats, probe, out_rp, bhist, bdesc;
+        tiles, dump, tops, mxstats, probe, out_rp, bhist, bdesc, hist2, carry2;
+    // segment length of the current (tau, emb) block, and whether its split
+    // form runs slab by slab (the tile store of all rows does not fit)
+    int blk_L = 0;
+    bool blk_slab = false;
+    // per-slab item ranges of the current item tables (slab form): items
+    // [off[s], off[s+1]) of each table belong to slab s
+    const int64_t *slab_lamv = nullptr, *slab_lamh = nullptr, *slab_dets = nullptr;
+    int nslabs = 0;
     // work-item tables of the current (nc, seg_len): views of an ItemSet
     // entry (owned there, never freed through these)
     DevBuf items_rr, items_det, items_dets, items_lamv, items_lamh;

@@ -70,9 +70,11 @@ struct CallOrder {
 struct ItemSet {
     int64_t nc = -1;
     int L = -1;
+    bool slab = false;   // lamv, lamh and dets sorted slab by slab (slab form)
     uint64_t used = 0;
     DevBuf rr, det, dets, lamv, lamh;
     int64_t n_rr = 0, n_det = 0, n_dets = 0, n_lamv = 0, n_lamh = 0;
+    std::vector<int64_t> off_lamv, off_lamh, off_dets;   // slab form: first item of each slab (+ end)
 };
 constexpr int NITEMSETS = 8;
 ItemSet g_items[NITEMSETS];
@@ -85,6 +87,11 @@ int g_form = 0;
 
 // LAM/DET split through kept rank tiles: 0 = when the tiles fit, 1 = never.
 int g_split = 0;
+
+// Slab form of the split: 0 = when the tile store of all rows does not fit
+// (slabs as tall as fit), > 0 = always, with slabs of this many rows (a
+// multiple of 64, at least 256; for tests).
+int g_slab_rows = 0;
 
 // Gram operand precision of the split form's LAM rank pass (rqa::GRAM_*):
 // GRAM_HILO always, but for the measured comparison of precisions.  Another
@@ -219,7 +226,8 @@ int init_ctx() {
                              &g_ctx.lut,     &g_ctx.nrm,        &g_ctx.nmax,       &g_ctx.lut8,
                              &g_ctx.out_rr,  &g_ctx.out_det,    &g_ctx.out_lam,    &g_ctx.priv,
                              &g_ctx.tiles,   &g_ctx.dump,       &g_ctx.tops,
-                             &g_ctx.mxstats, &g_ctx.probe, &g_ctx.out_rp, &g_ctx.bhist, &g_ctx.bdesc};
+                             &g_ctx.mxstats, &g_ctx.probe, &g_ctx.out_rp, &g_ctx.bhist, &g_ctx.bdesc,
+                             &g_ctx.hist2,   &g_ctx.carry2};
             for (DevBuf *b : all) {
                 if (b->p) (void)hipFree(b->p);
                 *b = DevBuf();
@@ -464,6 +472,31 @@ bool tile_store_fits(int64_t nc) {
     return need + g_ctx.tiles.cap < fr / 2;
 }
 
+// First tile of row block I in the packed store (rqa::tile_index(I, I/2)).
+int64_t tile_start(int64_t I, int64_t nbands) { return I * nbands - (I / 2) * ((I - 1) / 2); }
+
+// Bytes of the largest slab of L rows (the first: row block I holds the
+// tiles of bands >= I/2, so earlier slabs hold more).
+size_t slab_bytes(int64_t nc, int64_t L) {
+    const int64_t nrb = (nc + rqa::BLK - 1) / rqa::BLK;
+    const int64_t nbands = (nc + rqa::WAVE - 1) / rqa::WAVE;
+    const int64_t ib = std::min<int64_t>(L / rqa::BLK, nrb);
+    return (size_t)(tile_start(ib, nbands) - tile_start(0, nbands)) * rqa::TILE_BYTES;
+}
+
+// Slab height of the slab form: the tallest multiple of 64 rows (at least
+// 4,096, at most the default segment length) whose largest slab fits in under
+// half of the free device memory; 0 if none does.
+int slab_len_for(int64_t nc) {
+    if (g_slab_rows > 0) return g_slab_rows;
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) return 0;
+    const size_t budget = fr / 2 + g_ctx.tiles.cap;
+    for (int64_t L = seg_len_for(nc) / rqa::WAVE * rqa::WAVE; L >= 4096; L = L / 2 / rqa::WAVE * rqa::WAVE)
+        if (slab_bytes(nc, L) < budget) return (int)L;
+    return 0;
+}
+
 unsigned stitch_blocks(int64_t nseq) {
     return (unsigned)std::max<int64_t>(1, std::min<int64_t>((nseq + rqa::STT - 1) / rqa::STT, 64));
 }
@@ -483,13 +516,21 @@ void select_items(const ItemSet &s) {
     c.n_items_dets = s.n_dets;
     c.n_items_lamv = s.n_lamv;
     c.n_items_lamh = s.n_lamh;
+    c.nslabs = s.slab ? (int)s.off_lamv.size() - 1 : 0;
+    c.slab_lamv = s.slab ? s.off_lamv.data() : nullptr;
+    c.slab_lamh = s.slab ? s.off_lamh.data() : nullptr;
+    c.slab_dets = s.slab ? s.off_dets.data() : nullptr;
 }
 
-int build_items(int64_t nc, hipStream_t st) {
-    const int L = seg_len_for(nc);
+// Work-item tables of segment length L.  Slab form: the split passes' tables
+// (lamv, lamh, dets) are ordered slab by slab, slab s being the rows
+// [sL, (s+1)L) -- the row segment q = s of a column band, the row band
+// 64b / L of a row part, the step segment q = s of a strip group -- and by
+// decreasing size within a slab.
+int build_items(int64_t nc, int L, bool slab, hipStream_t st) {
     ItemSet *slot = &g_items[0];
     for (ItemSet &s : g_items) {
-        if (s.nc == nc && s.L == L) {
+        if (s.nc == nc && s.L == L && s.slab == slab) {
             s.used = ++g_items_clock;
             select_items(s);
             return E_OK;
@@ -517,8 +558,24 @@ int build_items(int64_t nc, hipStream_t st) {
     const int64_t ngroups = (nc - 1 + rqa::WG - 1) / rqa::WG;
     for (int64_t g = 0; g < ngroups; g++) seg(dets, g, nc - (1 + g * rqa::WG));
     int e;
-    auto upload = [&](std::vector<std::pair<int64_t, int2>> &v, DevBuf &buf, int64_t &n) -> int {
-        std::stable_sort(v.begin(), v.end(), [](const auto &x, const auto &y) { return x.first > y.first; });
+    const int64_t nsl = (nc + L - 1) / L;
+    // slab of an item: -1 = not a slab-ordered table
+    auto upload = [&](std::vector<std::pair<int64_t, int2>> &v, DevBuf &buf, int64_t &n, int kind = -1,
+                      std::vector<int64_t> *off = nullptr) -> int {
+        auto slab_of = [&](const int2 &x) -> int64_t {
+            return kind == 1 ? ((int64_t)x.x * rqa::WAVE) / L : (int64_t)x.y;   // lamh: the row band's slab
+        };
+        if (off) {
+            std::stable_sort(v.begin(), v.end(), [&](const auto &x, const auto &y) {
+                const int64_t sx = slab_of(x.second), sy = slab_of(y.second);
+                return sx != sy ? sx < sy : x.first > y.first;
+            });
+            off->assign((size_t)nsl + 1, 0);
+            for (const auto &x : v) (*off)[(size_t)slab_of(x.second) + 1]++;
+            for (int64_t k = 0; k < nsl; k++) (*off)[(size_t)k + 1] += (*off)[(size_t)k];
+        } else {
+            std::stable_sort(v.begin(), v.end(), [](const auto &x, const auto &y) { return x.first > y.first; });
+        }
         std::vector<int2> it(v.size());
         for (size_t i = 0; i < v.size(); i++) it[i] = v[i].second;
         int e2;
@@ -532,11 +589,13 @@ int build_items(int64_t nc, hipStream_t st) {
     S.nc = -1;
     if ((e = upload(rr, S.rr, S.n_rr))) return e;
     if ((e = upload(det, S.det, S.n_det))) return e;
-    if ((e = upload(dets, S.dets, S.n_dets))) return e;
-    if ((e = upload(lamv, S.lamv, S.n_lamv))) return e;
-    if ((e = upload(lamh, S.lamh, S.n_lamh))) return e;
+    if ((e = upload(dets, S.dets, S.n_dets, 0, slab ? &S.off_dets : nullptr))) return e;
+    if ((e = upload(lamv, S.lamv, S.n_lamv, 0, slab ? &S.off_lamv : nullptr))) return e;
+    if ((e = upload(lamh, S.lamh, S.n_lamh, 1, slab ? &S.off_lamh : nullptr))) return e;
+    if (!slab) S.off_dets.clear(), S.off_lamv.clear(), S.off_lamh.clear();
     S.nc = nc;
     S.L = L;
+    S.slab = slab;
     S.used = ++g_items_clock;
     select_items(S);
     return E_OK;
@@ -626,6 +685,147 @@ int launch_gram_variant(int gram, const float *dx, int64_t nc, int tau, int m, c
     return E_OK;
 }
 
+// The split form slab by slab, when the tile store of all rows does not fit:
+// slab s is the rows [sL, (s+1)L), i.e. the row segment q = s of every
+// column band (k_lam_mx + k_lam_vscan, or k_lam), the row bands 64b / L = s
+// (k_lam_hscan) and the step segment q = s of every strip group
+// (k_det_scan).  Each of them reads and writes only the tiles of the slab's
+// row blocks, so one slab's tiles at a time suffice: the tile pointer handed
+// to the kernels is moved back by the slab's first tile index.  Segment
+// boundaries already fall on slab boundaries, so the carries and k_stitch
+// are those of the whole-store form; DET runs slab by slab beside LAM, in its
+// own histogram and carry buffers, since the tiles are gone once the slabs
+// are done.
+template <typename F, int NORM, int TG>
+int launch_group_slabs(const F *dx, int64_t nc, int tau, int m, const rqa::Bounds<F, TG> &B, int nt,
+                       const rqa::LutEntry<F> *d_lut, const rqa::LutParams &P, bool want_rr, bool want_det,
+                       const int *d_lmins, int nl, const int *d_perm, int T, F *d_rr, F *d_det, F *d_lam,
+                       hipStream_t st) {
+    Ctx &c = g_ctx;
+    if (g_gram != rqa::GRAM_HILO) {
+        g_last_error = "the Gram precision comparison needs the tile store of all rows";
+        return E_INVALID;
+    }
+    const int L = c.blk_L;
+    const int nseg = (int)((nc + L - 1) / L);
+    const size_t hs = (size_t)nc + 1;
+    const int64_t nbands = (nc + rqa::WAVE - 1) / rqa::WAVE;
+    int e;
+    unsigned long long *counts = (unsigned long long *)c.counts.p;
+    if ((e = ensure(c.tiles, slab_bytes(nc, L)))) return e;
+    if ((e = ensure(c.hist, (size_t)nt * hs * 8))) return e;
+    if ((e = ensure(c.metric, (size_t)nt * hs * 8))) return e;
+    if ((e = ensure(c.scan, (size_t)nt * hs * 8))) return e;
+    if ((e = ensure(c.carry, (size_t)nt * (size_t)nc * 2 * nseg * sizeof(uint2)))) return e;
+    if (want_det) {
+        if ((e = ensure(c.hist2, (size_t)nt * hs * 8))) return e;
+        if ((e = ensure(c.carry2, (size_t)nt * (size_t)(nc - 1) * nseg * sizeof(uint2)))) return e;
+    }
+    unsigned long long *hist = (unsigned long long *)c.hist.p, *hist2 = (unsigned long long *)c.hist2.p;
+    uint2 *carry = (uint2 *)c.carry.p, *carry2 = (uint2 *)c.carry2.p;
+    fill_zero(hist, (size_t)nt * hs, st);
+    if (want_det) fill_zero(hist2, (size_t)nt * hs, st);
+    const bool mx = NORM == 1 && use_mfma<F>(1, m, tau);
+    rqa::Bounds<float, TG> Bf;
+    for (int t = 0; t < TG; t++) Bf.s[t] = (float)B.s[t];
+    rqa::MxParams MP{};
+    if (mx) {
+        MP = mx_params(Bf.s, nt, m, rqa::NBX_LAM, sizeof(F) == 8 ? 4 : 0);
+        if ((e = ensure(c.lut8, rqa::NBX_LAM))) return e;
+        hipLaunchKernelGGL((rqa::k_mx_lut<TG>), dim3((MP.nb + 255) / 256), dim3(256), 0, st, Bf, nt, MP,
+                           (const unsigned int *)c.nmax.p, (uint8_t *)c.lut8.p);
+        HIP_TRY(hipGetLastError());
+    }
+    for (int s = 0; s < c.nslabs; s++) {
+        const int64_t ia = (int64_t)s * L / rqa::BLK;
+        // the kernels index tiles from the store's start; this slab's block ia is the buffer's first
+        uint8_t *tiles = (uint8_t *)((uintptr_t)c.tiles.p - (uintptr_t)(tile_start(ia, nbands) * rqa::TILE_BYTES));
+        const int64_t v0 = c.slab_lamv[s], v1 = c.slab_lamv[s + 1];
+        const rqa::LamWork LW{(const int2 *)c.items_lamv.p + v0, v1 - v0, nseg, 2 * nseg};
+        if (LW.nitems > 0) {
+            bool done = false;
+            if constexpr (NORM == 1) {
+                if (mx) {
+                    const unsigned g = grid_for(rqa::k_lam_mx<F, TG, false>, LW.nitems, c.ncu);
+                    prof_begin("k_lam_mx", st);
+                    hipLaunchKernelGGL((rqa::k_lam_mx<F, TG, false>), dim3(g), dim3(rqa::WG), 0, st, dx,
+                                       (const float *)c.nrm.p, (const unsigned int *)c.nmax.p, nc, tau, m, Bf, B, nt,
+                                       (const uint8_t *)c.lut8.p, MP, L, LW, hist, carry, (uint32_t *)c.priv.p, tiles,
+                                       (unsigned long long *)nullptr);
+                    prof_end(st);
+                    HIP_TRY(hipGetLastError());
+                    const unsigned gv = grid_for(rqa::k_lam_vscan<TG>, LW.nitems, c.ncu);
+                    if ((e = ensure_priv<TG>(gv))) return e;
+                    prof_begin("k_lam_vscan", st);
+                    hipLaunchKernelGGL((rqa::k_lam_vscan<TG>), dim3(gv), dim3(rqa::WG), 0, st, (const uint8_t *)tiles,
+                                       nbands, nc, nt, L, LW, hist, carry, (uint32_t *)c.priv.p);
+                    prof_end(st);
+                    HIP_TRY(hipGetLastError());
+                    done = true;
+                }
+            }
+            if (!done) {
+                const unsigned g = grid_for(rqa::k_lam<F, NORM, TG>, LW.nitems, c.ncu);
+                if ((e = ensure_priv<TG>(g))) return e;
+                prof_begin("k_lam", st);
+                hipLaunchKernelGGL((rqa::k_lam<F, NORM, TG>), dim3(g), dim3(rqa::WG), 0, st, dx, nc, tau, m, B, nt,
+                                   d_lut, P, L, LW, hist, carry, (uint32_t *)c.priv.p, tiles);
+                prof_end(st);
+                HIP_TRY(hipGetLastError());
+            }
+        }
+        const int64_t h0 = c.slab_lamh[s], h1 = c.slab_lamh[s + 1];
+        if (h1 > h0) {
+            const unsigned g = grid_for(rqa::k_lam_hscan<TG>, h1 - h0, c.ncu);
+            if ((e = ensure_priv<TG>(g))) return e;
+            prof_begin("k_lam_hscan", st);
+            hipLaunchKernelGGL((rqa::k_lam_hscan<TG>), dim3(g), dim3(rqa::WG), 0, st, (const uint8_t *)tiles, nbands,
+                               nc, nt, L, nseg, (const int2 *)c.items_lamh.p + h0, h1 - h0, hist, carry,
+                               (uint32_t *)c.priv.p);
+            prof_end(st);
+            HIP_TRY(hipGetLastError());
+        }
+        const int64_t d0 = c.slab_dets[s], d1 = c.slab_dets[s + 1];
+        if (want_det && d1 > d0) {
+            const unsigned g = grid_for_wg(rqa::k_det_scan<TG>, d1 - d0, c.ncu);
+            if ((e = ensure_priv<TG>(g))) return e;
+            prof_begin("k_det_scan", st);
+            hipLaunchKernelGGL((rqa::k_det_scan<TG>), dim3(g), dim3(rqa::WG), 0, st, (const uint8_t *)tiles, nbands,
+                               nc, nt, L, nseg, (const int2 *)c.items_dets.p + d0, d1 - d0, hist2, carry2,
+                               (uint32_t *)c.priv.p);
+            prof_end(st);
+            HIP_TRY(hipGetLastError());
+        }
+    }
+    prof_begin("k_stitch", st);
+    hipLaunchKernelGGL(rqa::k_stitch, dim3(stitch_blocks(nc), nt), dim3(rqa::STT), 0, st, (const uint2 *)carry, nt,
+                       nc, nc, 0, 1, L, nseg, 2 * nseg, hist);
+    prof_end(st);
+    HIP_TRY(hipGetLastError());
+    if ((e = dump_rows(hist, nt, hs, d_perm, c.dump_lam, st))) return e;
+    prof_begin("k_finalize", st);
+    if ((e = finalize<F>(hist, nc, nt, 0, d_lmins, nl, d_lam, d_perm, T, counts, st))) return e;
+    prof_end(st);
+    if (want_rr) {
+        if ((e = dump_rows(counts, nt, 1, d_perm, c.dump_rr, st))) return e;
+        hipLaunchKernelGGL((rqa::k_rr_out<F>), dim3(1), dim3(64), 0, st, (const unsigned long long *)counts, nt, nc,
+                           d_perm, d_rr);
+        HIP_TRY(hipGetLastError());
+    }
+    if (want_det) {
+        prof_begin("k_stitch", st);
+        hipLaunchKernelGGL(rqa::k_stitch, dim3(stitch_blocks(nc - 1), nt), dim3(rqa::STT), 0, st,
+                           (const uint2 *)carry2, nt, nc - 1, nc, 1, 0, L, nseg, nseg, hist2);
+        prof_end(st);
+        HIP_TRY(hipGetLastError());
+        prof_begin("k_finalize", st);
+        if ((e = finalize<F>(hist2, nc, nt, 1, d_lmins, nl, d_det, d_perm, T, nullptr, st))) return e;
+        prof_end(st);
+        if ((e = dump_rows(hist2, nt, hs, d_perm, c.dump_det, st))) return e;
+    }
+    return E_OK;
+}
+
 // One group of up to 32 possibly-recurrent thresholds (sorted bounds).
 template <typename F, int NORM, int TG>
 int launch_group(const F *dx, int64_t nc, int tau, int m, const F *bounds, int nt, const rqa::LutEntry<F> *d_lut,
@@ -634,11 +834,14 @@ int launch_group(const F *dx, int64_t nc, int tau, int m, const F *bounds, int n
     Ctx &c = g_ctx;
     rqa::Bounds<F, TG> B;
     for (int t = 0; t < TG; t++) B.s[t] = t < nt ? bounds[t] : bounds[nt - 1];
-    const int L = seg_len_for(nc);
+    const int L = c.blk_L;
     const int nseg = (int)((nc + L - 1) / L);
     const size_t hs = (size_t)nc + 1;
     int e;
     unsigned long long *counts = (unsigned long long *)c.counts.p;
+    if (want_lam && nc > 1 && c.blk_slab)
+        return launch_group_slabs<F, NORM, TG>(dx, nc, tau, m, B, nt, d_lut, P, want_rr, want_det, d_lmins, nl,
+                                               d_perm, T, d_rr, d_det, d_lam, st);
     // Split form (rank tiles fit in memory): the LAM column kernel ranks the
     // pairs of the upper triangle and the diagonal tiles once, keeps them, and
     // k_lam_hscan / k_det_scan read the rest of LAM and all of DET from them.
@@ -933,7 +1136,18 @@ int run_block(const F *dx, size_t n, int tau, int emb, const F *thr, int T, cons
     if (ngroups > 0 && (e = upload_once(c.lut, g_up_lut, luts.data(),
                                         (size_t)ngroups * rqa::NB_MAX * sizeof(rqa::LutEntry<F>), st)))
         return e;
-    if ((e = build_items(nc, st))) return e;
+    // the split form's shape: the whole tile store, slab by slab, or none
+    // (launch_group decides per group whether to take it)
+    c.blk_L = seg_len_for(nc);
+    c.blk_slab = false;
+    if (d_lam && nc > 1 && g_split == 0 && (g_slab_rows > 0 || !tile_store_fits(nc))) {
+        const int Ls = slab_len_for(nc);
+        if (Ls > 0) {
+            c.blk_L = Ls;
+            c.blk_slab = true;
+        }
+    }
+    if ((e = build_items(nc, c.blk_L, c.blk_slab, st))) return e;
     if (use_mfma<F>(dist, emb, tau) && ngroups > 0) {
         if ((e = ensure(c.nrm, ((size_t)nc + 2 * rqa::WAVE) * sizeof(float)))) return e;
         if ((e = ensure(c.nmax, 256))) return e;
@@ -1147,7 +1361,7 @@ int device_fp64_diag(const float *d_in, size_t n, int tau, int emb, const float 
     if ((e = call.err) || (e = upload_series(d_in, n, true, st))) return e;
     Ctx &c = g_ctx;
     const int64_t nc = (int64_t)n - (int64_t)(emb - 1) * tau;
-    if ((e = build_items(nc, st))) return e;
+    if ((e = build_items(nc, seg_len_for(nc), false, st))) return e;
     if ((e = ensure(c.counts, std::max<size_t>(64, (size_t)T) * sizeof(unsigned long long)))) return e;
     unsigned long long *d_counts = (unsigned long long *)c.counts.p;
     fill_zero(d_counts, (size_t)T, st);
@@ -1399,6 +1613,13 @@ int accrqa_amd_set_form(int form) {
     if (form < 0 || form > 2) return E_INVALID;
     std::lock_guard<std::mutex> lk(g_mu);
     g_form = form;
+    return E_OK;
+}
+
+int accrqa_amd_set_slab_rows(int rows) {
+    if (rows < 0 || (rows > 0 && (rows < 256 || rows % rqa::WAVE != 0))) return E_INVALID;
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_slab_rows = rows;
     return E_OK;
 }
 

@@ -389,8 +389,8 @@ def config5_step(accrqa_amd, torch, steps=2):
     """The million-sample configuration (Roessler x, N = 1,048,576, tau 1,
     m 10, 32 thresholds, Euclidean, l_min 2): one RR+DET+LAM call through the
     device entry point in the default form (the rank-tile store, 550 GB, does
-    not fit, so DET recomputes its pairs); one warm-up, then the mean of
-    `steps` calls."""
+    not fit whole, so the split form runs slab by slab); one warm-up, then the
+    mean of `steps` calls."""
     from accrqa_amd import workloads as W
     t0 = time.perf_counter()
     c = W.config5()
@@ -418,7 +418,8 @@ def config5_step(accrqa_amd, torch, steps=2):
     torch.cuda.empty_cache()
     return {"seconds_per_step": dt, "pair_evals_per_s": float(nc) * nc * T / dt, "n_c": nc, "thresholds": T,
             "generator_seconds": gen_s,
-            "path": "accrqa_amd_rqa_device_f32, RR+DET+LAM, default form (direct: tile store does not fit)"}
+            "path": "accrqa_amd_rqa_device_f32, RR+DET+LAM, default form (split form slab by slab: the "
+                    "whole tile store does not fit)"}
 
 
 def config4_loop(accrqa_amd, stride=64):

@@ -120,18 +120,27 @@ def test_config2_both_norms(forms):
 
 def test_config5_million(forms):
     """Rossler x, N = 1,048,576, m = 10, 32 thresholds: the rank tiles
-    (550 GB) do not fit, so this runs the direct form in both distance forms."""
+    (550 GB) do not fit whole, so the split form runs slab by slab; both
+    distance forms, and the direct form (DET recomputing its pairs), agree
+    bit for bit."""
     amd = forms
     c = W.config5()
     x, tau, emb, thr = c["x"], c["tau"][0], c["emb"][0], c["thr"]
     amd.set_form(amd.FORM_MFMA)
-    say("config5 mfma")
+    say("config5 mfma (slab form)")
     mx = hists(amd, x, tau, emb, thr, 1)
     amd.set_form(amd.FORM_EXACT)
-    say("config5 exact")
+    say("config5 exact (slab form)")
     ex = hists(amd, x, tau, emb, thr, 1)
     amd.set_form(amd.FORM_AUTO)
     assert_same(ex, mx, "config5 exact vs mfma")
+    try:
+        amd.set_split(amd.SPLIT_NEVER)
+        say("config5 direct form")
+        dr = hists(amd, x, tau, emb, thr, 1)
+    finally:
+        amd.set_split(amd.SPLIT_AUTO)
+    assert_same(dr, mx, "config5 direct vs slab")
     say("config5 counts")
     check_counts(amd, x, tau, emb, thr, 1, mx, "config5")
 

@@ -434,6 +434,50 @@ def test_split_and_direct_forms(amd, oracle):
         amd.set_split(amd.SPLIT_AUTO)
 
 
+def test_slab_form(amd, oracle):
+    """The split form slab by slab (the tiles of one band of rows at a time,
+    LAM and DET together, as for series whose tile store does not fit),
+    forced with slabs of 256 and 512 rows: the histograms and every measure
+    equal the whole-store split form's and the oracle's, both norms, fp32 and
+    fp64, both distance forms, several threshold groups, N_c not a multiple of
+    the slab height, and DET or LAM asked alone."""
+    rng = np.random.default_rng(2025)
+    cases = []
+    for n, m, tau in ((300, 3, 1), (1000, 16, 1), (2113, 5, 2), (3001, 10, 1)):
+        w = np.cumsum(rng.standard_normal(n))
+        cases.append((f"walk{n}", (w - w.mean()) / w.std(), m, tau))
+    try:
+        for name, x64, m, tau in cases:
+            for dt in (np.float32, np.float64):
+                x = x64.astype(dt)
+                thr = np.concatenate([np.linspace(0.05, 2.5, 37), [-1.0]]).astype(dt)
+                for dist in (1, 2):
+                    for form in (amd.FORM_AUTO, amd.FORM_EXACT):
+                        amd.set_form(form)
+                        amd.set_slab_rows(0)
+                        a = amd.histograms(x, tau, m, thr, dist, which=7)
+                        ref = (amd.RR(x, [tau], [m], thr, dist), amd.DET(x, [tau], [m], [2, 3], thr, dist),
+                               amd.LAM(x, [tau], [m], [2, 3], thr, dist))
+                        for rows in (256, 512):
+                            say("slab", name, dt.__name__, dist, form, rows)
+                            amd.set_slab_rows(rows)
+                            b = amd.histograms(x, tau, m, thr, dist, which=7)
+                            for key in ("rr", "det", "lam"):
+                                assert np.array_equal(a[key], b[key]), (name, dt, dist, form, rows, key)
+                            got = (amd.RR(x, [tau], [m], thr, dist), amd.DET(x, [tau], [m], [2, 3], thr, dist),
+                                   amd.LAM(x, [tau], [m], [2, 3], thr, dist))
+                            assert same_bits(got[0], ref[0])
+                            assert_measures(got[1], ref[1], dt, f"slab DET {name}")
+                            assert_measures(got[2], ref[2], dt, f"slab LAM {name}")
+                    for k in (3, 20):
+                        for kind in ("lam", "det"):
+                            want = oracle.hist(kind, x, dt(thr[k]), tau, m, dist)
+                            assert np.array_equal(a[kind][k], want), (name, dt, dist, kind, thr[k])
+    finally:
+        amd.set_slab_rows(0)
+        amd.set_form(amd.FORM_AUTO)
+
+
 def test_mfma_decision_counters(mfma_form, oracle):
     """The MFMA form's diagnostic counters: elements ranked equals the split
     form's column-pass pairs (each band's columns times V_J rows), flagged and
