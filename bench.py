@@ -101,6 +101,22 @@ MEASURED_TRAFFIC = {
 }
 
 
+# VALU issue of the kernels on the default workload (rocprofv3 --pmc
+# SQ_INSTS_VALU and GRBM_GUI_ACTIVE over one step): wave64 VALU instructions
+# per launch.  An issue-rate probe puts the ceiling of their instruction mix at
+# 0.99-1.01 per CU per cycle for shifts, shift-ors and find-first-bit, 1.72 for
+# v_bitop3 and 1.87 for FP32 adds.
+_ISSUE_SRC = "profiles/ar1_m16_instruction_mix.md"
+MEASURED_VALU = {
+    ("ar1_m16", "k_lam_vscan"): 7.469e9,
+    ("ar1_m16", "k_lam_hscan"): 7.818e9,
+    ("ar1_m16", "k_det_scan"): 7.774e9,
+    ("ar1_m16", "k_lam_mx"): 7.193e9,
+    ("ar1_m16", "k_lam"): 3.986e10,
+}
+CU_COUNT, CLOCK_GHZ = 256, 2.4
+
+
 def kernel_rooflines(kern, nc, emb, T, dist, workload, tile_store):
     """Roofline figures of the main kernels from their average launch times.
     Algorithmic work per launch (the distinct pairs are the N_c(N_c+1)/2 of
@@ -143,12 +159,19 @@ def kernel_rooflines(kern, nc, emb, T, dist, workload, tile_store):
         elif k in ("k_lam_vscan", "k_lam_hscan", "k_det_scan") and tile_store:
             w = float(tile_store_bytes(nc))
             out[k] = {"bound": "hbm", "achieved": w / t / 1e9, "peak": 8000.0, "unit": "GB/s",
-                      "work_per_launch": w, "work_unit": "bytes (one read of the rank-tile store)"}
+                      "work_per_launch": w, "work_unit": "bytes (one read of the rank-tile store)",
+                      "note": ("the rank-tile store is this design's intermediate (the path's own traffic is "
+                               "under 0.1 GB per step); the scans are bound by VALU issue (valu_issue)")}
         else:
             continue
         r = out[k]
         r["frac"] = r["achieved"] / r["peak"]
         r["avg_ms"] = t * 1e3
+        vi = MEASURED_VALU.get((workload, k))
+        if vi:
+            r["valu_issue"] = {"wave_instructions_per_launch": vi,
+                               "per_cu_per_cycle": vi / (CU_COUNT * t * CLOCK_GHZ * 1e9),
+                               "source": _ISSUE_SRC}
         r["launches"] = v["launches"]
         tr = MEASURED_TRAFFIC.get((workload, k))
         r["traffic"] = tr[0] if tr else None
