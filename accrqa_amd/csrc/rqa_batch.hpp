@@ -358,7 +358,6 @@ __global__ __launch_bounds__(BFIN) void k_batch_fin(const uint32_t *__restrict__
     const int tid = threadIdx.x;
     __shared__ unsigned long long ru[BFIN];
     __shared__ double rd[BFIN];
-    __shared__ unsigned long long sh_top;
 
     auto red_u = [&](unsigned long long v, bool is_max) -> unsigned long long {
         __syncthreads();

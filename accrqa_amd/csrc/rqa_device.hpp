@@ -1968,7 +1968,8 @@ __global__ __launch_bounds__(STT) void k_stitch(const uint2 *__restrict__ carry,
                                                 int is_det, int split, int seg_len, int nseg, int cseg,
                                                 unsigned long long *__restrict__ hist) {
     __shared__ uint32_t sh[SCAP];
-    const int t = blockIdx.y;
+    const int t = blockIdx.y;   // threshold: the grid has nt rows
+    if (t >= nt) return;
     for (int e = threadIdx.x; e < SCAP; e += STT) sh[e] = 0u;
     __syncthreads();
     unsigned long long *h = hist + (size_t)t * (nc + 1);

@@ -408,6 +408,68 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+def device_call_seconds(accrqa_amd, torch, x, tau, emb, thr, lmin, dist, steps=3):
+    """Seconds per RR+DET+LAM call through the device entry point (series in
+    HBM, default form): one warm-up, then the mean of `steps` calls."""
+    T, nl = len(thr), len(lmin)
+    dev = torch.device("cuda:0")
+    st = torch.cuda.current_stream(dev)
+    dx = torch.from_numpy(np.ascontiguousarray(x, dtype=np.float32)).to(dev)
+    o = [torch.zeros(n, dtype=torch.float32, device=dev) for n in (T, T * nl * 5, nl * T * 5)]
+
+    def one():
+        accrqa_amd.rqa_device(dx.data_ptr(), x.size, tau, emb, [float(v) for v in thr], lmin, dist, o[0].data_ptr(),
+                              o[1].data_ptr(), o[2].data_ptr(), stream=st.cuda_stream, dtype=np.float32)
+    one()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        one()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def config1_2(accrqa_amd, torch):
+    """The two smaller configurations at their full sizes, through the device
+    entry point: config 1 (logistic map, N = 10,000, tau 1, m 3, 5 thresholds;
+    the reference's own CPU path timed on the whole of it beside the GPU:
+    one warm-up, the median of three runs) and config 2 (Lorenz x, N =
+    131,072, tau 8, m 3, 20 thresholds, each norm)."""
+    from accrqa_amd import workloads as W
+    out = {}
+    c = W.config1()
+    x = np.ascontiguousarray(c["x"], dtype=np.float32)
+    tau, emb, thr, lmin = c["tau"][0], c["emb"][0], [float(v) for v in c["thr"]], c["lmin"]
+    nc = x.size - (emb - 1) * tau
+    t = device_call_seconds(accrqa_amd, torch, x, tau, emb, thr, lmin, 1, steps=10)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from pyoracle import Oracle, Reference
+    impl, kind = (Reference(), "reference") if Reference.available() else (Oracle(), "port")
+
+    def cpu_run():
+        t0 = time.perf_counter()
+        impl.RR(x, [tau], [emb], thr, 1)
+        impl.DET(x, [tau], [emb], lmin, thr, 1)
+        impl.LAM(x, [tau], [emb], lmin, thr, 1)
+        return time.perf_counter() - t0
+    cpu_run()
+    tc = statistics.median([cpu_run() for _ in range(3)])
+    out["config1"] = {"n_c": nc, "seconds_per_call": t, "pair_evals_per_s": float(nc) * nc * len(thr) / t,
+                      "cpu": {"kind": kind, "cores": Oracle().num_threads(), "seconds_per_call": tc,
+                              "pair_evals_per_s": float(nc) * nc * len(thr) / tc,
+                              "sample": "the whole configuration, median of 3 runs after a warm-up"}}
+    c = W.config2()
+    x = np.ascontiguousarray(c["x"], dtype=np.float32)
+    tau, emb, thr, lmin = c["tau"][0], c["emb"][0], [float(v) for v in c["thr"]], c["lmin"]
+    nc = x.size - (emb - 1) * tau
+    out["config2"] = {"n_c": nc}
+    for d in c["norms"]:
+        t = device_call_seconds(accrqa_amd, torch, x, tau, emb, thr, lmin, d)
+        out["config2"]["euclidean" if d == 1 else "maximal"] = {
+            "seconds_per_call": t, "pair_evals_per_s": float(nc) * nc * len(thr) / t}
+    return out
+
+
 def config5_step(accrqa_amd, torch, steps=2):
     """The million-sample configuration (Roessler x, N = 1,048,576, tau 1,
     m 10, 32 thresholds, Euclidean, l_min 2): one RR+DET+LAM call through the
@@ -660,7 +722,8 @@ def extras(accrqa_amd, L, torch, args, c, x, step):
     * rp: the recurrence plot (rp_bench);
     * rr_target: the threshold search (rr_target_bench);
     * fp32_vs_true_fp64: the fp32 decisions against true fp64 ones;
-    * config5: the million-sample configuration (config5_step)."""
+    * config5: the million-sample configuration (config5_step);
+    * config1, config2: the two smaller configurations (config1_2)."""
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream(dev)
     out = {}
@@ -729,6 +792,8 @@ def extras(accrqa_amd, L, torch, args, c, x, step):
     progress("config3 m=32/64, fp64 input, fp64 diagnostic")
     out["config5"] = config5_step(accrqa_amd, torch)
     progress("config5")
+    out.update(config1_2(accrqa_amd, torch))
+    progress("config1 and config2")
     out["config4_per_series"] = config4_loop(accrqa_amd)
     progress("config4 per-series sample")
     out["config4_batched"] = config4_batched(accrqa_amd)
