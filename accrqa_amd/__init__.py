@@ -124,6 +124,8 @@ def load():
     L.accrqa_amd_mfma16_probe.restype = I
     L.accrqa_amd_set_rp_wide.argtypes = [I]
     L.accrqa_amd_set_rp_wide.restype = I
+    L.accrqa_amd_set_scan_waves.argtypes = [I]
+    L.accrqa_amd_set_scan_waves.restype = I
     L.accrqa_amd_set_gram.argtypes = [I]
     L.accrqa_amd_set_gram.restype = I
     L.accrqa_amd_gram_probe.argtypes = [I, P, P, P, P, P, P]
@@ -208,6 +210,14 @@ def set_rp_wide(on):
     e = load().accrqa_amd_set_rp_wide(1 if on else 0)
     if e:
         raise AccrqaError(e, "set_rp_wide")
+
+
+def set_scan_waves(waves):
+    """Waves per SIMD of the three tile scans (default 7; 0 = as many as fit).
+    A tuning aid: results are identical for every value."""
+    e = load().accrqa_amd_set_scan_waves(int(waves))
+    if e:
+        raise AccrqaError(e, "set_scan_waves")
 
 
 GRAM_HILO, GRAM_F16, GRAM_BF16, GRAM_FP8 = 0, 1, 2, 3

@@ -9,6 +9,7 @@
 #include <cstdint>
 #include <limits>
 #include <map>
+#include <utility>
 #include <mutex>
 #include <string>
 
@@ -149,16 +150,17 @@ bool never_recurs(F b) {
 }
 
 // Persistent grid: as many workgroups as can be resident at once (from the
-// kernel's register and LDS use), or fewer if there are fewer items.
+// kernel's register and LDS use, with `dyn` bytes of dynamic LDS per
+// workgroup), or fewer if there are fewer items.
 template <typename Kern>
-unsigned grid_for(Kern kern, int64_t nitems, int ncu) {
-    static std::map<const void *, int> per_cu;
+unsigned grid_for(Kern kern, int64_t nitems, int ncu, size_t dyn = 0) {
+    static std::map<std::pair<const void *, size_t>, int> per_cu;
     const void *key = reinterpret_cast<const void *>(kern);
-    auto it = per_cu.find(key);
+    auto it = per_cu.find({key, dyn});
     int nb = 0;
     if (it == per_cu.end()) {
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, key, rqa::ENG_WG, 0) != hipSuccess || nb < 1) nb = 1;
-        per_cu[key] = nb;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, key, rqa::ENG_WG, dyn) != hipSuccess || nb < 1) nb = 1;
+        per_cu[{key, dyn}] = nb;
     } else {
         nb = it->second;
     }
@@ -169,8 +171,31 @@ unsigned grid_for(Kern kern, int64_t nitems, int ncu) {
 
 // Persistent grid for a kernel whose work items belong to whole workgroups.
 template <typename Kern>
-unsigned grid_for_wg(Kern kern, int64_t nitems, int ncu) {
-    return grid_for(kern, nitems * (rqa::ENG_WG / 64), ncu);
+unsigned grid_for_wg(Kern kern, int64_t nitems, int ncu, size_t dyn = 0) {
+    return grid_for(kern, nitems * (rqa::ENG_WG / 64), ncu, dyn);
+}
+
+// Dynamic LDS that holds a kernel to at most `per_cu` resident workgroups per
+// CU (of four waves, so per_cu waves per SIMD): the workgroup's static LDS
+// padded past a (per_cu + 1)-th share of the CU's 160 KB.  0 when the kernel's
+// own LDS already keeps it there, or per_cu <= 0 (no cap).
+constexpr size_t LDS_PER_CU = 160 * 1024;
+template <typename Kern>
+size_t lds_cap_pad(Kern kern, int per_cu) {
+    if (per_cu <= 0) return 0;
+    static std::map<const void *, size_t> stat;
+    const void *key = reinterpret_cast<const void *>(kern);
+    auto it = stat.find(key);
+    size_t st = 0;
+    if (it == stat.end()) {
+        hipFuncAttributes a{};
+        if (hipFuncGetAttributes(&a, key) == hipSuccess) st = a.sharedSizeBytes;
+        stat[key] = st;
+    } else {
+        st = it->second;
+    }
+    const size_t need = LDS_PER_CU / (size_t)(per_cu + 1) + 1;
+    return need > st ? need - st : 0;
 }
 
 }  // namespace eng

@@ -30,6 +30,8 @@ constexpr int BWIN = 256;        // DET column window per wave (samples)
 // 8 thresholds, so LDS no longer caps the waves per SIMD below what the
 // registers allow; the short series of a batch rarely have runs this long.
 constexpr int BLCAP = 64;
+constexpr int BNSPARE = 8;                  // spare bins per row (eight lanes share one)
+constexpr int BLROW = BLCAP + BNSPARE;      // LDS words per threshold row
 
 // One problem: a series at x + xoff (n = N_c + (m-1) tau samples, followed
 // by at least PAD zeros in the batch buffer), one (tau, emb), nt <= TG live
@@ -101,8 +103,7 @@ __device__ __forceinline__ LutParams batch_lut(const F *b, int nt, LutEntry<F> *
 // Per-wave LDS of the batched kernel.
 template <typename F, int TG>
 struct BatchWave {
-    uint32_t lh[TG * BLCAP];   // run lengths < BLCAP of the current (problem, kind), row t at t * BLCAP
-    uint32_t dummy[WAVE];
+    uint32_t lh[TG * BLROW];   // run lengths < BLCAP of the current (problem, kind), row t at t * BLROW
     LutEntry<F> lut[BNB];
     F bnd[TG];
     F win[BWIN];
@@ -114,7 +115,7 @@ struct BatchWave {
 // (u32) straight away.
 template <int TG>
 __device__ __forceinline__ void batch_scan(const uint32_t (&R)[8], uint32_t keep, uint32_t force,
-                                           uint32_t (&run)[TG], int nt, uint32_t *lh, uint32_t *dummy,
+                                           uint32_t (&run)[TG], int nt, uint32_t *lh, uint32_t spare,
                                            uint32_t *hrow, int64_t hs) {
     constexpr int B = rank_bits<TG>();
     uint32_t Pl[B];
@@ -129,9 +130,10 @@ __device__ __forceinline__ void batch_scan(const uint32_t (&R)[8], uint32_t keep
                 if (__builtin_amdgcn_ballot_w64((~nw | run[t]) != 0u) == 0ull) continue;
                 lead_in = false;
             }
-            uint32_t *lht = lh + t * BLCAP;
+            uint32_t *lht = lh + t * BLROW;
             consume<BLCAP>(
-                nw, run[t], [&](uint32_t len, bool ok) { lds_inc(ok ? &lht[ok ? len : 0u] : dummy); },
+                nw, run[t], spare, [&](uint32_t b) { lds_inc(&lht[b + 1]); },
+                [&](uint32_t len) { lds_inc(&lht[len]); },
                 [&](uint32_t first) {
                     const uint32_t tt = opaque(t);
                     if ((int64_t)first < hs) atomicAdd(&hrow[(size_t)tt * hs + first], 1u);
@@ -149,7 +151,7 @@ __device__ __forceinline__ void batch_close(const uint32_t (&run)[TG], int nt, u
         const uint32_t len = run[t];
         if (t < nt && len) {
             if (len < (uint32_t)BLCAP)
-                lds_inc(&lh[t * BLCAP + len]);
+                lds_inc(&lh[t * BLROW + len]);
             else if ((int64_t)len < hs)
                 atomicAdd(&hrow[(size_t)opaque(t) * hs + len], 1u);
         }
@@ -162,11 +164,11 @@ __device__ __forceinline__ void batch_flush(uint32_t *lh, int nt, uint32_t *hrow
     const int lane = threadIdx.x & (WAVE - 1);
     wave_sync();
     for (int e = lane; e < TG * BLCAP; e += WAVE) {
-        const uint32_t v = lh[e];
+        const int t = e / BLCAP, len = e - t * BLCAP;
+        const uint32_t v = lh[t * BLROW + len];
         if (v) {
-            const int t = e / BLCAP, len = e - t * BLCAP;
-            if (t < nt && len < hs) atomicAdd(&hrow[(size_t)t * hs + len], v);
-            lh[e] = 0u;
+            if (len && t < nt && len < hs) atomicAdd(&hrow[(size_t)t * hs + len], v);
+            lh[t * BLROW + len] = 0u;
         }
     }
     wave_sync();
@@ -186,7 +188,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     const int lane = threadIdx.x & (WAVE - 1);
     const int wave = wave_id();
     BatchWave<F, TG> &Wv = wv[wave];
-    for (int e = lane; e < TG * BLCAP; e += WAVE) Wv.lh[e] = 0u;
+    for (int e = lane; e < TG * BLROW; e += WAVE) Wv.lh[e] = 0u;
     const int64_t nitems = ipre[np];
     constexpr int MJ = sizeof(F) == 4 ? 16 : 8;   // column samples kept in registers (LAM)
 
@@ -272,7 +274,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(WPE))) void 
                 pack_ranks<F, TG>(s, R, Wv.lut, P, Wv.bnd, nt);
                 // rows past the column's end count as recurrent padding, taken back below
                 const uint32_t fill = nb == BLK ? 0u : (0xFFFFFFFFu << nb);
-                batch_scan<TG>(R, jv ? ~fill : 0u, jv ? 0u : 0xFFFFFFFFu, run, nt, Wv.lh, Wv.dummy + lane,
+                batch_scan<TG>(R, jv ? ~fill : 0u, jv ? 0u : 0xFFFFFFFFu, run, nt, Wv.lh, BLCAP - 1 + (lane >> 3),
                                cur_row, hs);
                 if (fill && jv) {
 #pragma unroll
@@ -312,7 +314,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(WPE))) void 
                 const int64_t nv64 = dlen - s0;
                 const int nvalid = nv64 <= 0 ? 0 : (nv64 >= BLK ? BLK : (int)nv64);
                 const uint32_t vmask = nvalid >= BLK ? 0xFFFFFFFFu : ((1u << nvalid) - 1u);
-                batch_scan<TG>(R, 0xFFFFFFFFu, ~vmask, run, nt, Wv.lh, Wv.dummy + lane, cur_row, hs);
+                batch_scan<TG>(R, 0xFFFFFFFFu, ~vmask, run, nt, Wv.lh, BLCAP - 1 + (lane >> 3), cur_row, hs);
             }
             if (rv) batch_close<TG>(run, nt, Wv.lh, cur_row, hs);
         }

@@ -21,7 +21,16 @@
 namespace rqa {
 
 constexpr int LCAP = 128;              // run lengths < LCAP go to the workgroup's LDS histogram
-constexpr uint32_t PRE = 0x80000000u;  // run counter flag: the segment's first run has not closed yet
+// Run counter flag: the segment's first run has not closed yet.  Bit 30, so
+// that every run counter (at most PRE + N_c, N_c < 2^30) is positive as a
+// signed integer and an all-ones word stands apart from it.
+constexpr uint32_t PRE = 0x40000000u;
+// Spare bins after each row of an LDS run histogram: a lane adds one there for
+// a word whose carried run counts nothing (none closes, it is empty, or it
+// goes to the slow path), so that the wave's adds that count nothing spread
+// over NSPARE addresses instead of meeting on one.
+constexpr int NSPARE = 16;
+constexpr int LROW = LCAP + NSPARE;    // LDS words per threshold row
 
 // Position of the lowest / highest set bit (as the hardware's find-first-bit
 // instructions): 0xFFFFFFFF for a zero word.  On the device they are the
@@ -43,15 +52,21 @@ inline uint32_t ffbl(uint32_t v) { return v ? (uint32_t)__builtin_ctz(v) : 0xFFF
 inline uint32_t ffbh(uint32_t v) { return v ? (uint32_t)__builtin_clz(v) : 0xFFFFFFFFu; }
 #endif
 
-// All-ones shifted right by (s mod 32), the shift count taken as the
-// hardware's funnel shift takes it (a word of ffbh(0) = 0xFFFFFFFF gives 1).
-RQA_HD uint32_t ones_shr(uint32_t s) {
+// All-ones shifted right by (s mod 32), as the hardware's shift takes its
+// count (ffbh(0) = 0xFFFFFFFF gives 1).
+RQA_HD uint32_t ones_shr(uint32_t s) { return 0xFFFFFFFFu >> (s & 31u); }
+
+// a + b, saturating at 0xFFFFFFFF (one clamped add on the device).
+RQA_HD uint32_t add_sat(uint32_t a, uint32_t b) {
 #if defined(__HIP_DEVICE_COMPILE__)
-    return __builtin_amdgcn_alignbit(0u, 0xFFFFFFFFu, s);
+    return __builtin_elementwise_add_sat(a, b);
 #else
-    return 0xFFFFFFFFu >> (s & 31u);
+    const uint32_t s = a + b;
+    return s < a ? 0xFFFFFFFFu : s;
 #endif
 }
+
+RQA_HD uint32_t umin(uint32_t a, uint32_t b) { return a < b ? a : b; }
 
 // Bits of a rank in [0, TG].
 template <int TG>
@@ -123,34 +138,41 @@ struct GeTree {
 
 // One 32-step word of one threshold.  nw: bit u set = step u not recurrent
 // (bit 0 = oldest).  run: recurrent steps carried in from earlier words, with
-// PRE added while the segment's first run is open.  emit(len, ok) counts a
-// closed run of length 1 <= len < CAP when ok, and does nothing countable
-// otherwise (the device form always issues its add, to a dummy bin when !ok,
-// so the common case needs no branch); slow(first) takes the run that closes
-// at the word's first non-recurrent step when it is >= CAP long or is the
-// segment's first run (first >= PRE: its length is first - PRE, 0 included).
+// PRE added while the segment's first run is open.  Counting goes through
+// three callbacks:
+//   first(b): the carried run, closed at the word's first non-recurrent step;
+//     b <= CAP - 2 counts a run of length b + 1, larger b (at most `spare`,
+//     the caller's spare bin CAP - 1 + k) counts nothing: no run closes, the
+//     closing run is empty, or it takes the slow path.  Always called, so the
+//     common case needs no branch;
+//   inner(len): a run strictly inside the word, 1 <= len <= 30;
+//   slow(first): the carried run when it is >= CAP long or is the segment's
+//     first run (first >= PRE: its length is first - PRE, 0 included).
 // CAP (at least 32) is the caller's LDS histogram size: LCAP for the large-N
-// kernels.
-template <uint32_t CAP = LCAP, class Emit, class Slow>
-RQA_HD void consume(uint32_t nw, uint32_t &run, Emit &&emit, Slow &&slow) {
+// kernels.  No select or compare is spent on "does a run close": for nw == 0
+// both find-first-bit results are all ones, so the saturating sum is all
+// ones (a spare bin, and negative as a signed integer, so not slow) and the
+// new carry min(run + 32, tail) is run + 32.
+template <uint32_t CAP = LCAP, class First, class Inner, class Slow>
+RQA_HD void consume(uint32_t nw, uint32_t &run, uint32_t spare, First &&first_bin, Inner &&inner, Slow &&slow) {
     static_assert(CAP >= 32, "inner runs (up to 30 steps) always take the histogram");
-    const uint32_t lead = ffbl(nw);     // recurrent steps before the first non-recurrent one
-    const uint32_t tail = ffbh(nw);     // recurrent steps after the last non-recurrent one
-    const uint32_t first = run + lead;  // the carried run, closed by that first step
-    const bool closes = nw != 0u;
-    emit(first, closes && first - 1u < CAP - 1u);
-    if (closes && first >= CAP) slow(first);
-    // runs strictly inside the word: recurrent steps between the first and
-    // last non-recurrent ones (for nw == 0 both shifts are by 31 and mid is 0)
-    uint32_t mid = ~nw & (0xFFFFFFFFu << (lead & 31u)) & ones_shr(tail);
+    const uint32_t lead = ffbl(nw);                 // recurrent steps before the first non-recurrent one
+    const uint32_t tail = ffbh(nw);                 // recurrent steps after the last non-recurrent one
+    const uint32_t first = add_sat(run, lead);      // the carried run, closed by that first step
+    first_bin(umin(first - 1u, spare));             // first = 0 wraps to a spare bin too
+    if ((int32_t)first >= (int32_t)CAP) slow(first);
+    // runs strictly inside the word: recurrent steps above the lowest
+    // non-recurrent one (~nw & -nw) and at or below the highest (for nw == 0
+    // mid is 0)
+    uint32_t mid = ~nw & (0u - nw) & ones_shr(tail);
     while (mid) {
         // both operands are non-zero: mid by the loop test, nw >> lo because
         // an inner run ends before the word's last non-recurrent step
         const uint32_t lo = (uint32_t)__builtin_ctz(mid);
-        emit((uint32_t)__builtin_ctz(nw >> lo), true);   // ends at the next non-recurrent step
-        mid &= mid + (1u << lo);                         // clear that run
+        inner((uint32_t)__builtin_ctz(nw >> lo));   // ends at the next non-recurrent step
+        mid &= mid + (1u << lo);                     // clear that run
     }
-    run = closes ? tail : run + 32u;
+    run = umin(run + 32u, tail);
 }
 
 // Segment end: (prefix run, suffix run) for the stitch.  A segment whose

@@ -137,8 +137,8 @@ constexpr int PCAP = 2048;
 constexpr int PBINS = PCAP - LCAP;   // private bins per threshold
 
 struct Sink {
-    uint32_t *lh;                  // LDS: row t at t * LCAP
-    uint32_t *dummy;               // LDS: this lane's bin for adds that count nothing
+    uint32_t *lh;                  // LDS: row t at t * LROW, bins 1 .. LCAP-1, then NSPARE spare bins
+    uint32_t spare;                // this lane's spare bin, LCAP - 1 + lane/4 (consume's numbering)
     uint32_t *priv;                // workgroup slice: row t at t * PBINS, bin len - LCAP
     unsigned long long *hist;      // result: row t at t * (nc + 1)
     int64_t nc;
@@ -158,14 +158,18 @@ __device__ __forceinline__ void priv_zero(uint32_t *priv, int nt, int nthreads) 
     for (int e = threadIdx.x; e < nt * PBINS; e += nthreads) priv[e] = 0u;
 }
 
-// Adds the workgroup's LDS and private bins to the result (after a barrier).
+static_assert(WAVE / 4 <= NSPARE, "four lanes share a spare bin");
+__device__ __forceinline__ uint32_t lane_spare() { return LCAP - 1 + ((threadIdx.x & (WAVE - 1)) >> 2); }
+
+// Adds the workgroup's LDS and private bins to the result (after a barrier):
+// bins 1 .. LCAP-1 of each LDS row (bin 0 and the spare bins count nothing).
 template <int TG>
 __device__ __forceinline__ void flush_hist(const uint32_t *lh, uint32_t *priv, int nt, unsigned long long *hist,
                                            int64_t nc, int nthreads) {
     for (int e = threadIdx.x; e < TG * LCAP; e += nthreads) {
-        const uint32_t v = lh[e];
-        const int t = e / LCAP;
-        if (v && t < nt) glob_add(&hist[(size_t)t * (nc + 1) + (e - t * LCAP)], v);
+        const int t = e / LCAP, b = e - t * LCAP;
+        const uint32_t v = lh[t * LROW + b];
+        if (v && b && t < nt) glob_add(&hist[(size_t)t * (nc + 1) + b], v);
     }
     for (int e = threadIdx.x; e < nt * PBINS; e += nthreads) {
         const uint32_t v = __hip_atomic_load(&priv[e], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -224,9 +228,10 @@ __device__ __forceinline__ void scan_words(const uint32_t (&R)[8], uint32_t keep
                 if (__builtin_amdgcn_ballot_w64((~nw | run[t]) != 0u) == 0ull) continue;
                 lead_in = false;
             }
-            uint32_t *lht = K.lh + t * LCAP;
+            uint32_t *lht = K.lh + t * LROW;
             consume(
-                nw, run[t], [&](uint32_t len, bool ok) { lds_inc(ok ? &lht[ok ? len : 0u] : K.dummy); },
+                nw, run[t], K.spare, [&](uint32_t b) { lds_inc(&lht[b + 1]); },
+                [&](uint32_t len) { lds_inc(&lht[len]); },
                 [&](uint32_t first) {
                     const uint32_t tt = opaque(t);
                     slow_run(first, cslot + (size_t)tt * cstride, K, tt);
@@ -370,17 +375,16 @@ __global__ __launch_bounds__(WG) RQA_WPE_LAM(F, NORM, TG) void k_lam(const F *__
                                             LamWork W, unsigned long long *__restrict__ hist,
                                             uint2 *__restrict__ carry, uint32_t *__restrict__ priv_base,
                                             uint8_t *__restrict__ tiles) {
-    __shared__ uint32_t lh[TG * LCAP];
-    __shared__ uint32_t dummy_bins[WAVE];
+    __shared__ uint32_t lh[TG * LROW];
     __shared__ LutEntry<F> lut_s[NB_MAX];
     __shared__ F bnd_s[TG];
-    for (int e = threadIdx.x; e < TG * LCAP; e += WG) lh[e] = 0;
+    for (int e = threadIdx.x; e < TG * LROW; e += WG) lh[e] = 0;
     load_lut(lut_s, lut, P.nb);
     if (threadIdx.x < TG) bnd_s[threadIdx.x] = S.s[threadIdx.x];
     priv_zero<TG>(priv_slice<TG>(priv_base), nt, WG);
     __syncthreads();
     uint32_t *priv = priv_slice<TG>(priv_base);
-    const Sink K{lh, dummy_bins + (threadIdx.x & (WAVE - 1)), priv, hist, nc};
+    const Sink K{lh, lane_spare(), priv, hist, nc};
 
     const int lane = threadIdx.x & (WAVE - 1);
     const int wave = wave_id();
@@ -464,18 +468,17 @@ __global__ __launch_bounds__(WG) void k_det(const F *__restrict__ x, int64_t nc,
                                             int nseg, const int2 *__restrict__ items, int64_t nitems,
                                             unsigned long long *__restrict__ hist, uint2 *__restrict__ carry,
                                             uint32_t *__restrict__ priv_base) {
-    __shared__ uint32_t lh[TG * LCAP];
-    __shared__ uint32_t dummy_bins[WAVE];
+    __shared__ uint32_t lh[TG * LROW];
     __shared__ LutEntry<F> lut_s[NB_MAX];
     __shared__ F win[WAVES][DET_WIN];
     __shared__ F bnd_s[TG];
-    for (int e = threadIdx.x; e < TG * LCAP; e += WG) lh[e] = 0;
+    for (int e = threadIdx.x; e < TG * LROW; e += WG) lh[e] = 0;
     load_lut(lut_s, lut, P.nb);
     if (threadIdx.x < TG) bnd_s[threadIdx.x] = S.s[threadIdx.x];
     priv_zero<TG>(priv_slice<TG>(priv_base), nt, WG);
     __syncthreads();
     uint32_t *priv = priv_slice<TG>(priv_base);
-    const Sink K{lh, dummy_bins + (threadIdx.x & (WAVE - 1)), priv, hist, nc};
+    const Sink K{lh, lane_spare(), priv, hist, nc};
 
     const int lane = threadIdx.x & (WAVE - 1);
     const int wave = wave_id();
@@ -547,7 +550,7 @@ __global__ __launch_bounds__(WG) void k_det(const F *__restrict__ x, int64_t nc,
 // needs the six tiles J0 .. J0+5, each read once and shared through LDS (one
 // 36-byte record per tile lane: conflict-free gathers), against three per
 // strip when the waves fetch their own.  The next block's records are in
-// registers while a block is gathered and scanned.  (One LDS buffer: 22 KB
+// registers while a block is gathered and scanned.  (One LDS buffer: 22.5 KB
 // per workgroup for 16 thresholds, which leaves registers, not LDS, to set
 // the number of waves per SIMD.)
 constexpr int TREC = 9;              // LDS words per tile lane (8 + 1 pad)
@@ -560,14 +563,13 @@ __global__ __launch_bounds__(WG) RQA_WPE_SCAN(TG, 7, 3) void k_det_scan(const ui
                                                  int64_t nitems, unsigned long long *__restrict__ hist,
                                                  uint2 *__restrict__ carry, uint32_t *__restrict__ priv_base) {
     static_assert(DRECS <= 2 * WG, "each thread moves at most two records per block");
-    __shared__ uint32_t lh[TG * LCAP];
-    __shared__ uint32_t dummy_bins[WAVE];
+    __shared__ uint32_t lh[TG * LROW];
     __shared__ uint32_t tl[DRECS * TREC];
-    for (int e = threadIdx.x; e < TG * LCAP; e += WG) lh[e] = 0;
+    for (int e = threadIdx.x; e < TG * LROW; e += WG) lh[e] = 0;
     priv_zero<TG>(priv_slice<TG>(priv_base), nt, WG);
     __syncthreads();
     uint32_t *priv = priv_slice<TG>(priv_base);
-    const Sink K{lh, dummy_bins + (threadIdx.x & (WAVE - 1)), priv, hist, nc};
+    const Sink K{lh, lane_spare(), priv, hist, nc};
 
     const int tid = threadIdx.x;
     const int lane = tid & (WAVE - 1);
@@ -666,14 +668,13 @@ __global__ __launch_bounds__(WG) RQA_WPE_SCAN(TG, 7, 4) void k_lam_hscan(const u
                                                   int seg_len, int nseg, const int2 *__restrict__ items,
                                                   int64_t nitems, unsigned long long *__restrict__ hist,
                                                   uint2 *__restrict__ carry, uint32_t *__restrict__ priv_base) {
-    __shared__ uint32_t lh[TG * LCAP];
-    __shared__ uint32_t dummy_bins[WAVE];
+    __shared__ uint32_t lh[TG * LROW];
     __shared__ uint32_t tl[WAVES][WAVE * TREC];
-    for (int e = threadIdx.x; e < TG * LCAP; e += WG) lh[e] = 0;
+    for (int e = threadIdx.x; e < TG * LROW; e += WG) lh[e] = 0;
     priv_zero<TG>(priv_slice<TG>(priv_base), nt, WG);
     __syncthreads();
     uint32_t *priv = priv_slice<TG>(priv_base);
-    const Sink K{lh, dummy_bins + (threadIdx.x & (WAVE - 1)), priv, hist, nc};
+    const Sink K{lh, lane_spare(), priv, hist, nc};
 
     const int lane = threadIdx.x & (WAVE - 1);
     const int wave = wave_id();
@@ -1491,9 +1492,8 @@ __global__ __launch_bounds__(WG) RQA_WPE_MX(SCAN) void k_lam_mx(const F *__restr
                                                unsigned long long *__restrict__ hist, uint2 *__restrict__ carry,
                                                uint32_t *__restrict__ priv_base, uint8_t *__restrict__ tiles,
                                                unsigned long long *__restrict__ stats) {
-    __shared__ uint32_t lh[SCAN ? TG * LCAP : 1];
+    __shared__ uint32_t lh[SCAN ? TG * LROW : 1];
     uint32_t st_rank = 0, st_flag = 0, st_exact = 0;
-    __shared__ uint32_t dummy_bins[WAVE];
     __shared__ uint8_t lut_s[NBX_LAM];
     __shared__ float bnd_s[NBND];
     // the exact re-decision's bounds in the input type (fp64 input: the
@@ -1504,7 +1504,7 @@ __global__ __launch_bounds__(WG) RQA_WPE_MX(SCAN) void k_lam_mx(const F *__restr
     // both rounded to fp32, scaled and split into window words
     __shared__ MxWin<2> win[WAVES];
     if constexpr (SCAN)
-        for (int e = threadIdx.x; e < TG * LCAP; e += WG) lh[e] = 0;
+        for (int e = threadIdx.x; e < TG * LROW; e += WG) lh[e] = 0;
     for (int e = threadIdx.x; e < P.nb / 4; e += WG) ((uint32_t *)lut_s)[e] = ((const uint32_t *)lut)[e];
     if (threadIdx.x < NBND) bnd_s[threadIdx.x] = (int)threadIdx.x < nt ? S.s[threadIdx.x] : __int_as_float(0x7F800000);
     if constexpr (sizeof(F) == 8)
@@ -1517,7 +1517,7 @@ __global__ __launch_bounds__(WG) RQA_WPE_MX(SCAN) void k_lam_mx(const F *__restr
     if constexpr (SCAN) priv_zero<TG>(priv_slice<TG>(priv_base), nt, WG);
     __syncthreads();
     uint32_t *priv = priv_slice<TG>(priv_base);
-    const Sink K{lh, dummy_bins + (threadIdx.x & (WAVE - 1)), priv, hist, nc};
+    const Sink K{lh, lane_spare(), priv, hist, nc};
 
     const int lane = threadIdx.x & (WAVE - 1);
     const int wave = wave_id();
@@ -1751,13 +1751,12 @@ __global__ __launch_bounds__(WG) RQA_WPE_SCAN(TG, 7, 4) void k_lam_vscan(const u
                                                   int nt, int seg_len, LamWork W,
                                                   unsigned long long *__restrict__ hist, uint2 *__restrict__ carry,
                                                   uint32_t *__restrict__ priv_base) {
-    __shared__ uint32_t lh[TG * LCAP];
-    __shared__ uint32_t dummy_bins[WAVE];
-    for (int e = threadIdx.x; e < TG * LCAP; e += WG) lh[e] = 0;
+    __shared__ uint32_t lh[TG * LROW];
+    for (int e = threadIdx.x; e < TG * LROW; e += WG) lh[e] = 0;
     priv_zero<TG>(priv_slice<TG>(priv_base), nt, WG);
     __syncthreads();
     uint32_t *priv = priv_slice<TG>(priv_base);
-    const Sink K{lh, dummy_bins + (threadIdx.x & (WAVE - 1)), priv, hist, nc};
+    const Sink K{lh, lane_spare(), priv, hist, nc};
 
     const int lane = threadIdx.x & (WAVE - 1);
     const int wave = wave_id();
@@ -1821,20 +1820,19 @@ __global__ __launch_bounds__(WG) RQA_WPE_MX(true) void k_det_mx(const float *__r
                                                const int2 *__restrict__ items, int64_t nitems,
                                                unsigned long long *__restrict__ hist, uint2 *__restrict__ carry,
                                                uint32_t *__restrict__ priv_base) {
-    __shared__ uint32_t lh[TG * LCAP];
-    __shared__ uint32_t dummy_bins[WAVE];
+    __shared__ uint32_t lh[TG * LROW];
     __shared__ uint8_t lut_s[NBX];
     __shared__ uint32_t skew[WAVES][WAVE * SKB / 4];
     __shared__ float bnd_s[NBND];
     // rw: x[s0 .. s0 + 31 + (m-1)tau]; cw: x[c0 .. c0 + 95 + (m-1)tau]
     __shared__ MxWin<3> win[WAVES];
-    for (int e = threadIdx.x; e < TG * LCAP; e += WG) lh[e] = 0;
+    for (int e = threadIdx.x; e < TG * LROW; e += WG) lh[e] = 0;
     for (int e = threadIdx.x; e < P.nb / 4; e += WG) ((uint32_t *)lut_s)[e] = ((const uint32_t *)lut)[e];
     if (threadIdx.x < NBND) bnd_s[threadIdx.x] = (int)threadIdx.x < nt ? S.s[threadIdx.x] : __int_as_float(0x7F800000);
     priv_zero<TG>(priv_slice<TG>(priv_base), nt, WG);
     __syncthreads();
     uint32_t *priv = priv_slice<TG>(priv_base);
-    const Sink K{lh, dummy_bins + (threadIdx.x & (WAVE - 1)), priv, hist, nc};
+    const Sink K{lh, lane_spare(), priv, hist, nc};
 
     const int lane = threadIdx.x & (WAVE - 1);
     const int wave = wave_id();

@@ -105,6 +105,13 @@ int g_gram = 0;
 // stores align) or one (k_rp; accrqa_amd_set_rp_wide(0)).
 bool g_rp_wide = true;
 
+// Resident workgroups (four waves each, so waves per SIMD) of the three tile
+// scans, k_lam_vscan, k_lam_hscan and k_det_scan: their LDS is padded with
+// dynamic LDS until no more fit per CU; 0 = as many as their registers and
+// LDS allow.  Seven measured faster than eight on the scans' VALU, LDS-atomic
+// and branch mix (accrqa_amd_set_scan_waves).
+int g_scan_waves = 7;
+
 // MFMA-form decision statistics (diagnostic): when on, the split form's rank
 // kernel counts elements ranked, flagged by the table and re-decided in exact
 // order into the context's mxstats buffer (3 u64).
@@ -754,10 +761,11 @@ int launch_group_slabs(const F *dx, int64_t nc, int tau, int m, const rqa::Bound
                                        (unsigned long long *)nullptr);
                     prof_end(st);
                     HIP_TRY(hipGetLastError());
-                    const unsigned gv = grid_for(rqa::k_lam_vscan<TG>, LW.nitems, c.ncu);
+                    const size_t pv = lds_cap_pad(rqa::k_lam_vscan<TG>, g_scan_waves);
+                    const unsigned gv = grid_for(rqa::k_lam_vscan<TG>, LW.nitems, c.ncu, pv);
                     if ((e = ensure_priv<TG>(gv))) return e;
                     prof_begin("k_lam_vscan", st);
-                    hipLaunchKernelGGL((rqa::k_lam_vscan<TG>), dim3(gv), dim3(rqa::WG), 0, st, (const uint8_t *)tiles,
+                    hipLaunchKernelGGL((rqa::k_lam_vscan<TG>), dim3(gv), dim3(rqa::WG), pv, st, (const uint8_t *)tiles,
                                        nbands, nc, nt, L, LW, hist, carry, (uint32_t *)c.priv.p);
                     prof_end(st);
                     HIP_TRY(hipGetLastError());
@@ -776,10 +784,11 @@ int launch_group_slabs(const F *dx, int64_t nc, int tau, int m, const rqa::Bound
         }
         const int64_t h0 = c.slab_lamh[s], h1 = c.slab_lamh[s + 1];
         if (h1 > h0) {
-            const unsigned g = grid_for(rqa::k_lam_hscan<TG>, h1 - h0, c.ncu);
+            const size_t ph = lds_cap_pad(rqa::k_lam_hscan<TG>, g_scan_waves);
+            const unsigned g = grid_for(rqa::k_lam_hscan<TG>, h1 - h0, c.ncu, ph);
             if ((e = ensure_priv<TG>(g))) return e;
             prof_begin("k_lam_hscan", st);
-            hipLaunchKernelGGL((rqa::k_lam_hscan<TG>), dim3(g), dim3(rqa::WG), 0, st, (const uint8_t *)tiles, nbands,
+            hipLaunchKernelGGL((rqa::k_lam_hscan<TG>), dim3(g), dim3(rqa::WG), ph, st, (const uint8_t *)tiles, nbands,
                                nc, nt, L, nseg, (const int2 *)c.items_lamh.p + h0, h1 - h0, hist, carry,
                                (uint32_t *)c.priv.p);
             prof_end(st);
@@ -787,10 +796,11 @@ int launch_group_slabs(const F *dx, int64_t nc, int tau, int m, const rqa::Bound
         }
         const int64_t d0 = c.slab_dets[s], d1 = c.slab_dets[s + 1];
         if (want_det && d1 > d0) {
-            const unsigned g = grid_for_wg(rqa::k_det_scan<TG>, d1 - d0, c.ncu);
+            const size_t pd = lds_cap_pad(rqa::k_det_scan<TG>, g_scan_waves);
+            const unsigned g = grid_for_wg(rqa::k_det_scan<TG>, d1 - d0, c.ncu, pd);
             if ((e = ensure_priv<TG>(g))) return e;
             prof_begin("k_det_scan", st);
-            hipLaunchKernelGGL((rqa::k_det_scan<TG>), dim3(g), dim3(rqa::WG), 0, st, (const uint8_t *)tiles, nbands,
+            hipLaunchKernelGGL((rqa::k_det_scan<TG>), dim3(g), dim3(rqa::WG), pd, st, (const uint8_t *)tiles, nbands,
                                nc, nt, L, nseg, (const int2 *)c.items_dets.p + d0, d1 - d0, hist2, carry2,
                                (uint32_t *)c.priv.p);
             prof_end(st);
@@ -916,10 +926,11 @@ int launch_group(const F *dx, int64_t nc, int tau, int m, const F *bounds, int n
                     }
                     prof_end(st);
                     HIP_TRY(hipGetLastError());
-                    const unsigned gv = grid_for(rqa::k_lam_vscan<TG>, LW.nitems, c.ncu);
+                    const size_t pv = lds_cap_pad(rqa::k_lam_vscan<TG>, g_scan_waves);
+                    const unsigned gv = grid_for(rqa::k_lam_vscan<TG>, LW.nitems, c.ncu, pv);
                     if ((e = ensure_priv<TG>(gv))) return e;
                     prof_begin("k_lam_vscan", st);
-                    hipLaunchKernelGGL((rqa::k_lam_vscan<TG>), dim3(gv), dim3(rqa::WG), 0, st,
+                    hipLaunchKernelGGL((rqa::k_lam_vscan<TG>), dim3(gv), dim3(rqa::WG), pv, st,
                                        (const uint8_t *)tiles, nbands, nc, nt, L, LW, hist, carry,
                                        (uint32_t *)c.priv.p);
                     prof_end(st);
@@ -953,10 +964,11 @@ int launch_group(const F *dx, int64_t nc, int tau, int m, const F *bounds, int n
             HIP_TRY(hipGetLastError());
         }
         if (tiles && c.n_items_lamh > 0) {
-            const unsigned g = grid_for(rqa::k_lam_hscan<TG>, c.n_items_lamh, c.ncu);
+            const size_t ph = lds_cap_pad(rqa::k_lam_hscan<TG>, g_scan_waves);
+            const unsigned g = grid_for(rqa::k_lam_hscan<TG>, c.n_items_lamh, c.ncu, ph);
             if ((e = ensure_priv<TG>(g))) return e;
             prof_begin("k_lam_hscan", st);
-            hipLaunchKernelGGL((rqa::k_lam_hscan<TG>), dim3(g), dim3(rqa::WG), 0, st, (const uint8_t *)tiles, nbands,
+            hipLaunchKernelGGL((rqa::k_lam_hscan<TG>), dim3(g), dim3(rqa::WG), ph, st, (const uint8_t *)tiles, nbands,
                                nc, nt, L, nseg, (const int2 *)c.items_lamh.p, c.n_items_lamh, hist, carry,
                                (uint32_t *)c.priv.p);
             prof_end(st);
@@ -987,10 +999,11 @@ int launch_group(const F *dx, int64_t nc, int tau, int m, const F *bounds, int n
             bool done = false;
             if (tiles) {
                 const int64_t nig = c.n_items_dets;
-                const unsigned g = grid_for_wg(rqa::k_det_scan<TG>, nig, c.ncu);
+                const size_t pd = lds_cap_pad(rqa::k_det_scan<TG>, g_scan_waves);
+                const unsigned g = grid_for_wg(rqa::k_det_scan<TG>, nig, c.ncu, pd);
                 if ((e = ensure_priv<TG>(g))) return e;
                 prof_begin("k_det_scan", st);
-                hipLaunchKernelGGL((rqa::k_det_scan<TG>), dim3(g), dim3(rqa::WG), 0, st, (const uint8_t *)tiles,
+                hipLaunchKernelGGL((rqa::k_det_scan<TG>), dim3(g), dim3(rqa::WG), pd, st, (const uint8_t *)tiles,
                                    nbands, nc, nt, L, nseg,
                                    (const int2 *)c.items_dets.p, nig, hist, carry, (uint32_t *)c.priv.p);
                 prof_end(st);
@@ -1678,6 +1691,13 @@ int accrqa_amd_set_rp_wide(int on) {
     if (on < 0 || on > 1) return E_INVALID;
     std::lock_guard<std::mutex> lk(g_mu);
     g_rp_wide = on != 0;
+    return E_OK;
+}
+
+int accrqa_amd_set_scan_waves(int waves) {
+    if (waves < 0 || waves > 8) return E_INVALID;
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_scan_waves = waves;
     return E_OK;
 }
 

@@ -249,6 +249,14 @@ int accrqa_amd_mfma16_probe(const uint16_t *a, const uint16_t *b, const float *c
  * Returns 0 or ERR_INVALID_ARGUMENT. */
 int accrqa_amd_set_rp_wide(int on);
 
+/* Waves per SIMD of the three tile scans (tuning aid, no counterpart in the
+ * reference): at most `waves` workgroups of four waves each stay resident per
+ * CU, held there by padding their LDS with dynamic LDS; 0 = as many as their
+ * registers and LDS allow.  The default, 7, was measured faster than 8 on
+ * this part.  Results are identical for every value.  Returns 0 or
+ * ERR_INVALID_ARGUMENT (waves outside 0..8). */
+int accrqa_amd_set_scan_waves(int waves);
+
 /* Gram operand precision of the split MFMA form's LAM rank pass, for the
  * measured comparison of precisions (experiment, no counterpart in the
  * reference): 0 = FP16 hi/lo (the default and the only production form),

@@ -82,7 +82,7 @@ static Result direct(const std::vector<int> &b) {
 }
 
 template <uint32_t CAP>
-static Result scanned(const std::vector<int> &b, uint32_t start_run) {
+static Result scanned(const std::vector<int> &b, uint32_t start_run, uint32_t spare) {
     const long L = (long)b.size();
     Result r{std::vector<long>(L + 2, 0), -1, -1};
     uint32_t run = start_run;
@@ -94,17 +94,26 @@ static Result scanned(const std::vector<int> &b, uint32_t start_run) {
             if (!b[w0 + u]) nw |= 1u << u;
         // steps past the end are filled as recurrent and taken back afterwards
         consume<CAP>(
-            nw, run,
-            [&](uint32_t len, bool ok) {
-                if (ok) r.hist[len]++;
-                CHECK(!ok || len < CAP, "emitted %u >= CAP\n", len);
+            nw, run, spare,
+            [&](uint32_t b) {
+                // b + 1 is the bin: a counted run below CAP, or a spare bin
+                if (b <= CAP - 2) {
+                    CHECK((long)b + 1 <= L, "run of %u steps in a sequence of %ld\n", b + 1, L);
+                    if ((long)b + 1 <= L) r.hist[b + 1]++;
+                }
+                CHECK(b <= spare, "first-run bin %u beyond the spare bin %u\n", b, spare);
+            },
+            [&](uint32_t len) {
+                CHECK(len >= 1 && len <= 30, "inner run of %u steps\n", len);
+                r.hist[len]++;
             },
             [&](uint32_t first) {
                 if (first >= PRE) {
                     CHECK(pre < 0, "prefix written twice\n");
                     pre = first - PRE;
                 } else {
-                    r.hist[first]++;
+                    CHECK((long)first <= L, "slow run of %u steps in a sequence of %ld\n", first, L);
+                    if ((long)first <= L) r.hist[first]++;
                 }
             });
         if (nb < 32) run -= (uint32_t)(32 - nb);
@@ -137,7 +146,7 @@ static void test_consume(std::mt19937_64 &g) {
             else if (mode == 4) b[k] = (g() % 50) != 0;          // long runs
             else b[k] = (g() % 50) == 0;                         // isolated ones
         }
-        const Result d = direct(b), s = scanned<CAP>(b, PRE);
+        const Result d = direct(b), s = scanned<CAP>(b, PRE, CAP - 1 + (uint32_t)(g() % NSPARE));
         CHECK(d.pre == s.pre, "L=%ld mode=%d pre %ld vs %ld\n", L, mode, d.pre, s.pre);
         CHECK(d.suf == s.suf, "L=%ld mode=%d suf %ld vs %ld\n", L, mode, d.suf, s.suf);
         for (long k = 0; k <= L; k++)

@@ -434,6 +434,36 @@ def test_split_and_direct_forms(amd, oracle):
         amd.set_split(amd.SPLIT_AUTO)
 
 
+def test_scan_waves_identical(amd, oracle):
+    """The tile scans held to 7 (the default), 4 and 1 waves per SIMD, and
+    uncapped (accrqa_amd_set_scan_waves): every histogram is the same, and
+    equals the oracle's, both norms, one and two threshold groups; waves
+    outside 0..8 are refused."""
+    rng = np.random.default_rng(4711)
+    w = np.cumsum(rng.standard_normal(2600))
+    x = ((w - w.mean()) / w.std()).astype(np.float32)
+    try:
+        for dist in (1, 2):
+            for thr in (np.linspace(0.1, 2.0, 16).astype(np.float32), np.linspace(0.05, 2.5, 21).astype(np.float32)):
+                ref = None
+                for waves in (7, 4, 1, 0):
+                    say("scan waves", dist, thr.size, waves)
+                    amd.set_scan_waves(waves)
+                    got = amd.histograms(x, 1, 9, thr, dist, which=7)
+                    if ref is None:
+                        ref = got
+                        for k, t in enumerate(thr):
+                            assert np.array_equal(got["lam"][k], oracle.hist("lam", x, t, 1, 9, dist)), (dist, t)
+                            assert np.array_equal(got["det"][k], oracle.hist("det", x, t, 1, 9, dist)), (dist, t)
+                    for key in ("rr", "det", "lam"):
+                        assert np.array_equal(ref[key], got[key]), (dist, thr.size, waves, key)
+        for bad in (-1, 9):
+            with pytest.raises(amd.AccrqaError):
+                amd.set_scan_waves(bad)
+    finally:
+        amd.set_scan_waves(7)
+
+
 def test_slab_form(amd, oracle):
     """The split form slab by slab (the tiles of one band of rows at a time,
     LAM and DET together, as for series whose tile store does not fit),
