@@ -137,8 +137,8 @@ constexpr int PCAP = 2048;
 constexpr int PBINS = PCAP - LCAP;   // private bins per threshold
 
 struct Sink {
-    uint32_t *lh;                  // LDS: row t at t * LROW, bins 1 .. LCAP-1, then NSPARE spare bins
-    uint32_t spare;                // this lane's spare bin, LCAP - 1 + lane/4 (consume's numbering)
+    uint32_t *lh;                  // LDS: row t at t * lrow<NSP>(), bins 1 .. LCAP-1, then NSP spare bins
+    uint32_t spare;                // this lane's spare bin (lane_spare), in consume's numbering
     uint32_t *priv;                // workgroup slice: row t at t * PBINS, bin len - LCAP
     unsigned long long *hist;      // result: row t at t * (nc + 1)
     int64_t nc;
@@ -158,17 +158,26 @@ __device__ __forceinline__ void priv_zero(uint32_t *priv, int nt, int nthreads) 
     for (int e = threadIdx.x; e < nt * PBINS; e += nthreads) priv[e] = 0u;
 }
 
-static_assert(WAVE / 4 <= NSPARE, "four lanes share a spare bin");
-__device__ __forceinline__ uint32_t lane_spare() { return LCAP - 1 + ((threadIdx.x & (WAVE - 1)) >> 2); }
+// A kernel's LDS histogram rows hold LCAP bins and NSP spare bins (WAVE / NSP
+// lanes share one): NSP = NSPARE where LDS is tight, WAVE where it is not.
+template <int NSP>
+constexpr int lrow() {
+    static_assert(NSP >= 1 && NSP <= WAVE && WAVE % NSP == 0, "spare bins per row");
+    return LCAP + NSP;
+}
+template <int NSP = NSPARE>
+__device__ __forceinline__ uint32_t lane_spare() {
+    return LCAP - 1 + ((threadIdx.x & (WAVE - 1)) / (WAVE / NSP));
+}
 
 // Adds the workgroup's LDS and private bins to the result (after a barrier):
 // bins 1 .. LCAP-1 of each LDS row (bin 0 and the spare bins count nothing).
-template <int TG>
+template <int TG, int NSP = NSPARE>
 __device__ __forceinline__ void flush_hist(const uint32_t *lh, uint32_t *priv, int nt, unsigned long long *hist,
                                            int64_t nc, int nthreads) {
     for (int e = threadIdx.x; e < TG * LCAP; e += nthreads) {
         const int t = e / LCAP, b = e - t * LCAP;
-        const uint32_t v = lh[t * LROW + b];
+        const uint32_t v = lh[t * lrow<NSP>() + b];
         if (v && b && t < nt) glob_add(&hist[(size_t)t * (nc + 1) + b], v);
     }
     for (int e = threadIdx.x; e < nt * PBINS; e += nthreads) {
@@ -212,7 +221,7 @@ __device__ __forceinline__ void close_run(uint2 *slot, uint32_t run) {
 // a run open form a prefix.  consume() would change nothing there (no run
 // closes or opens), so the wave skips them, testing up to the first threshold
 // that fails.
-template <int TG>
+template <int TG, int NSP = NSPARE>
 __device__ __forceinline__ void scan_words(const uint32_t (&R)[8], uint32_t keep, uint32_t force, uint32_t (&run)[TG],
                                            int nt, const Sink &K, uint2 *cslot, size_t cstride) {
     constexpr int B = rank_bits<TG>();
@@ -228,7 +237,7 @@ __device__ __forceinline__ void scan_words(const uint32_t (&R)[8], uint32_t keep
                 if (__builtin_amdgcn_ballot_w64((~nw | run[t]) != 0u) == 0ull) continue;
                 lead_in = false;
             }
-            uint32_t *lht = K.lh + t * LROW;
+            uint32_t *lht = K.lh + t * lrow<NSP>();
             consume(
                 nw, run[t], K.spare, [&](uint32_t b) { lds_inc(&lht[b + 1]); },
                 [&](uint32_t len) { lds_inc(&lht[len]); },
@@ -668,13 +677,13 @@ __global__ __launch_bounds__(WG) RQA_WPE_SCAN(TG, 7, 4) void k_lam_hscan(const u
                                                   int seg_len, int nseg, const int2 *__restrict__ items,
                                                   int64_t nitems, unsigned long long *__restrict__ hist,
                                                   uint2 *__restrict__ carry, uint32_t *__restrict__ priv_base) {
-    __shared__ uint32_t lh[TG * LROW];
+    __shared__ uint32_t lh[TG * lrow<WAVE>()];
     __shared__ uint32_t tl[WAVES][WAVE * TREC];
-    for (int e = threadIdx.x; e < TG * LROW; e += WG) lh[e] = 0;
+    for (int e = threadIdx.x; e < TG * lrow<WAVE>(); e += WG) lh[e] = 0;
     priv_zero<TG>(priv_slice<TG>(priv_base), nt, WG);
     __syncthreads();
     uint32_t *priv = priv_slice<TG>(priv_base);
-    const Sink K{lh, lane_spare(), priv, hist, nc};
+    const Sink K{lh, lane_spare<WAVE>(), priv, hist, nc};
 
     const int lane = threadIdx.x & (WAVE - 1);
     const int wave = wave_id();
@@ -728,7 +737,7 @@ __global__ __launch_bounds__(WG) RQA_WPE_SCAN(TG, 7, 4) void k_lam_hscan(const u
             for (int u = 0; u < BLK; u++) R[u & 7] |= (uint32_t)tb[gofs + u * (4 * TREC)] << (8 * (u >> 3));
             // columns past the part count as recurrent padding, taken back below
             const uint32_t fill = nb == BLK ? 0u : (0xFFFFFFFFu << nb);
-            scan_words<TG>(R, jv ? ~fill : 0u, jv ? 0u : 0xFFFFFFFFu, run, nt, K, cslot, cstride);
+            scan_words<TG, WAVE>(R, jv ? ~fill : 0u, jv ? 0u : 0xFFFFFFFFu, run, nt, K, cslot, cstride);
             if (fill && jv) {
 #pragma unroll
                 for (int t = 0; t < TG; t++) run[t] -= (uint32_t)(BLK - nb);
@@ -738,7 +747,7 @@ __global__ __launch_bounds__(WG) RQA_WPE_SCAN(TG, 7, 4) void k_lam_hscan(const u
     }
 
     __syncthreads();
-    flush_hist<TG>(lh, priv, nt, hist, nc, WG);
+    flush_hist<TG, WAVE>(lh, priv, nt, hist, nc, WG);
 }
 
 // ---------------------------------------------------------------- k_rp
@@ -1751,12 +1760,12 @@ __global__ __launch_bounds__(WG) RQA_WPE_SCAN(TG, 7, 4) void k_lam_vscan(const u
                                                   int nt, int seg_len, LamWork W,
                                                   unsigned long long *__restrict__ hist, uint2 *__restrict__ carry,
                                                   uint32_t *__restrict__ priv_base) {
-    __shared__ uint32_t lh[TG * LROW];
-    for (int e = threadIdx.x; e < TG * LROW; e += WG) lh[e] = 0;
+    __shared__ uint32_t lh[TG * lrow<WAVE>()];
+    for (int e = threadIdx.x; e < TG * lrow<WAVE>(); e += WG) lh[e] = 0;
     priv_zero<TG>(priv_slice<TG>(priv_base), nt, WG);
     __syncthreads();
     uint32_t *priv = priv_slice<TG>(priv_base);
-    const Sink K{lh, lane_spare(), priv, hist, nc};
+    const Sink K{lh, lane_spare<WAVE>(), priv, hist, nc};
 
     const int lane = threadIdx.x & (WAVE - 1);
     const int wave = wave_id();
@@ -1790,7 +1799,7 @@ __global__ __launch_bounds__(WG) RQA_WPE_SCAN(TG, 7, 4) void k_lam_vscan(const u
             if (i0 + BLK < ie) fetch(i0 + BLK);
             // rows past the part count as recurrent padding, taken back below
             const uint32_t fill = nb == BLK ? 0u : (0xFFFFFFFFu << nb);
-            scan_words<TG>(R, jv ? ~fill : 0u, jv ? 0u : 0xFFFFFFFFu, run, nt, K, cslot, cstride);
+            scan_words<TG, WAVE>(R, jv ? ~fill : 0u, jv ? 0u : 0xFFFFFFFFu, run, nt, K, cslot, cstride);
             if (fill && jv) {
 #pragma unroll
                 for (int t = 0; t < TG; t++) run[t] -= (uint32_t)(BLK - nb);
@@ -1800,7 +1809,7 @@ __global__ __launch_bounds__(WG) RQA_WPE_SCAN(TG, 7, 4) void k_lam_vscan(const u
     }
 
     __syncthreads();
-    flush_hist<TG>(lh, priv, nt, hist, nc, WG);
+    flush_hist<TG, WAVE>(lh, priv, nt, hist, nc, WG);
 }
 
 // ---------------------------------------------------------------- k_det_mx
