@@ -26,11 +26,12 @@ constexpr int BMAX_NC = 65535;   // u32 bins cannot overflow: every count is bel
 constexpr int BNB = 128;         // bucket table entries per wave
 constexpr int BWIN = 256;        // DET column window per wave (samples)
 // Run lengths below BLCAP go to the wave's LDS histogram.  Smaller than the
-// large-N kernels' LCAP: a wave's LDS (histogram, table, window) is 4.3 KB for
-// 8 thresholds, so LDS no longer caps the waves per SIMD below what the
-// registers allow; the short series of a batch rarely have runs this long.
+// large-N kernels' LCAP: a wave's LDS (histogram with a spare bin per lane and
+// row, table, window) is 6.0 KB for 8 thresholds, so LDS allows 6 waves per
+// SIMD, above what the registers allow; the short series of a batch rarely
+// have runs this long.
 constexpr int BLCAP = 64;
-constexpr int BNSPARE = 8;                  // spare bins per row (eight lanes share one)
+constexpr int BNSPARE = WAVE;               // spare bins per row, one per lane
 constexpr int BLROW = BLCAP + BNSPARE;      // LDS words per threshold row
 
 // One problem: a series at x + xoff (n = N_c + (m-1) tau samples, followed
@@ -274,7 +275,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(WPE))) void 
                 pack_ranks<F, TG>(s, R, Wv.lut, P, Wv.bnd, nt);
                 // rows past the column's end count as recurrent padding, taken back below
                 const uint32_t fill = nb == BLK ? 0u : (0xFFFFFFFFu << nb);
-                batch_scan<TG>(R, jv ? ~fill : 0u, jv ? 0u : 0xFFFFFFFFu, run, nt, Wv.lh, BLCAP - 1 + (lane >> 3),
+                batch_scan<TG>(R, jv ? ~fill : 0u, jv ? 0u : 0xFFFFFFFFu, run, nt, Wv.lh, BLCAP - 1 + lane,
                                cur_row, hs);
                 if (fill && jv) {
 #pragma unroll
@@ -314,7 +315,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(WPE))) void 
                 const int64_t nv64 = dlen - s0;
                 const int nvalid = nv64 <= 0 ? 0 : (nv64 >= BLK ? BLK : (int)nv64);
                 const uint32_t vmask = nvalid >= BLK ? 0xFFFFFFFFu : ((1u << nvalid) - 1u);
-                batch_scan<TG>(R, 0xFFFFFFFFu, ~vmask, run, nt, Wv.lh, BLCAP - 1 + (lane >> 3), cur_row, hs);
+                batch_scan<TG>(R, 0xFFFFFFFFu, ~vmask, run, nt, Wv.lh, BLCAP - 1 + lane, cur_row, hs);
             }
             if (rv) batch_close<TG>(run, nt, Wv.lh, cur_row, hs);
         }
