@@ -108,11 +108,11 @@ MEASURED_TRAFFIC = {
 # v_bitop3 and 1.87 for FP32 adds.
 _ISSUE_SRC = "profiles/ar1_m16_instruction_mix.md"
 MEASURED_VALU = {
-    ("ar1_m16", "k_lam_vscan"): 7.469e9,
-    ("ar1_m16", "k_lam_hscan"): 7.818e9,
-    ("ar1_m16", "k_det_scan"): 7.774e9,
+    ("ar1_m16", "k_lam_vscan"): 7.029e9,
+    ("ar1_m16", "k_lam_hscan"): 7.348e9,
+    ("ar1_m16", "k_det_scan"): 7.122e9,
     ("ar1_m16", "k_lam_mx"): 7.193e9,
-    ("ar1_m16", "k_lam"): 3.986e10,
+    ("ar1_m16", "k_lam"): 3.921e10,
 }
 CU_COUNT, CLOCK_GHZ = 256, 2.4
 
