@@ -672,13 +672,22 @@ __global__ __launch_bounds__(WG) RQA_WPE_SCAN(TG, 7, 3) void k_det_scan(const ui
 // of 32 columns.  A block's bytes are records 32h .. 32h+31 (h = which half
 // of column band Jc) of tiles (2J', Jc) and (2J'+1, Jc); the wave copies them
 // to LDS and lane l reads byte (l%32) of each record of its row block.
+//
+// LDS layout of the wave's 64 records (record r = 32a + column, a = the row
+// block): word w of record r at w*HW + (r%8)*8 + r/8.  Register q of lane
+// (a, row) needs word row%8 of records 32a + q + 8g, g = 0..3, i.e. the four
+// words at (row%8)*HW + 8q + 4a: one aligned 16-byte read, then byte row/8 of
+// each by byte permutes.  HW = 72 puts the 16 distinct reads of a register
+// (8 words x 2 row blocks) on distinct banks, and a store of word w by all 64
+// lanes covers 64 consecutive words.
+constexpr int HW = 72;
 template <int TG>
 __global__ __launch_bounds__(WG) RQA_WPE_SCAN(TG, 7, 4) void k_lam_hscan(const uint8_t *__restrict__ tiles, int64_t nbands, int64_t nc, int nt,
                                                   int seg_len, int nseg, const int2 *__restrict__ items,
                                                   int64_t nitems, unsigned long long *__restrict__ hist,
                                                   uint2 *__restrict__ carry, uint32_t *__restrict__ priv_base) {
     __shared__ uint32_t lh[TG * lrow<WAVE>()];
-    __shared__ uint32_t tl[WAVES][WAVE * TREC];
+    __shared__ __align__(16) uint32_t tl[WAVES][8 * HW];
     for (int e = threadIdx.x; e < TG * lrow<WAVE>(); e += WG) lh[e] = 0;
     priv_zero<TG>(priv_slice<TG>(priv_base), nt, WG);
     __syncthreads();
@@ -688,10 +697,12 @@ __global__ __launch_bounds__(WG) RQA_WPE_SCAN(TG, 7, 4) void k_lam_hscan(const u
     const int lane = threadIdx.x & (WAVE - 1);
     const int wave = wave_id();
     uint32_t *tw = tl[wave];
-    const uint8_t *tb = (const uint8_t *)tw;
     const int a = lane >> 5, rr = lane & 31;
-    // lane's byte in a record of its row block: register rr&7, byte rr>>3
-    const int gofs = (a * 32) * (4 * TREC) + 4 * (rr & 7) + (rr >> 3);
+    // this lane's record (r = lane) in the layout: word w at wofs + w*HW
+    const int wofs = (lane & 7) * 8 + (lane >> 3);
+    // the lane's row: word rr%8 of its row block's records, byte rr/8
+    const uint4 *rsrc = (const uint4 *)(tw + (rr & 7) * HW + 4 * a);
+    const uint32_t bsel = (uint32_t)(rr >> 3) | ((uint32_t)(4 + (rr >> 3)) << 8) | 0x0C0C0000u;
     const size_t cstride = (size_t)2 * nseg * nc;
 
     for (int64_t item = (int64_t)blockIdx.x * WAVES + wave; item < nitems; item += (int64_t)gridDim.x * WAVES) {
@@ -726,15 +737,21 @@ __global__ __launch_bounds__(WG) RQA_WPE_SCAN(TG, 7, 4) void k_lam_hscan(const u
             const int nb = ce - c0 < BLK ? (int)(ce - c0) : BLK;
             wave_sync();   // the previous block's gathers are done
             if (has_rows) {
-                uint32_t *d = tw + lane * TREC;
-                d[0] = v0.x; d[1] = v0.y; d[2] = v0.z; d[3] = v0.w;
-                d[4] = v1.x; d[5] = v1.y; d[6] = v1.z; d[7] = v1.w;
+                uint32_t *d = tw + wofs;
+                d[0 * HW] = v0.x; d[1 * HW] = v0.y; d[2 * HW] = v0.z; d[3 * HW] = v0.w;
+                d[4 * HW] = v1.x; d[5 * HW] = v1.y; d[6 * HW] = v1.z; d[7 * HW] = v1.w;
                 if (c0 + BLK < ce) fetch(c0 + BLK);
             }
             wave_sync();
-            uint32_t R[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
+            uint32_t R[8];
 #pragma unroll
-            for (int u = 0; u < BLK; u++) R[u & 7] |= (uint32_t)tb[gofs + u * (4 * TREC)] << (8 * (u >> 3));
+            for (int q = 0; q < 8; q++) {
+                // words g = 0..3 are records 32a + q + 8g: byte rr/8 of each is row rr's step q + 8g
+                const uint4 v = rsrc[2 * q];
+                const uint32_t lo = __builtin_amdgcn_perm(v.y, v.x, bsel);
+                const uint32_t hi = __builtin_amdgcn_perm(v.w, v.z, bsel);
+                R[q] = __builtin_amdgcn_perm(hi, lo, 0x05040100u);
+            }
             // columns past the part count as recurrent padding, taken back below
             const uint32_t fill = nb == BLK ? 0u : (0xFFFFFFFFu << nb);
             scan_words<TG, WAVE>(R, jv ? ~fill : 0u, jv ? 0u : 0xFFFFFFFFu, run, nt, K, cslot, cstride);
