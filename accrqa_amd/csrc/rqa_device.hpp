@@ -1173,12 +1173,15 @@ __device__ __forceinline__ uint32_t mx_key_bucket(float st, const MxParams &P) {
 
 // Second stage of one flagged element (table byte with bit 7 set): the one
 // bound within reach decides unless it lies within E_ij of s~, or several
-// bounds were within reach; amb is set for those pairs.
+// bounds were within reach; amb is set for those pairs.  cE and dE carry the
+// margin for the rounding of E (mx_params).  st need not be clamped at 0: a
+// negative st decides "below the bound" only when st + E_ij is, and the true
+// s lies within E_ij of st.
 __device__ __forceinline__ uint32_t mx_second(uint32_t byte, float st, float sum, const float *bnd, float cE,
                                               float dE, bool &amb) {
     const uint32_t idx = byte & 0x3Fu;
     const float d = st - bnd[idx];
-    const float E = fmaf(cE, sum, dE) * 1.0001f;
+    const float E = fmaf(cE, sum, dE);
     amb = (byte & 0x40u) != 0u || !(fabsf(d) > E);
     return idx + (d > 0.0f ? 1u : 0u);
 }
@@ -1463,7 +1466,7 @@ __device__ __forceinline__ void tile_ranks(const floatx16 &acc, const float (&ni
             if (W & (1u << (sh + c))) {
                 const uint32_t byte = (Rq[c] >> sh) & 0xFFu;
                 const float sum = ni[e] + nj;
-                const float st = fmaxf(fmaf(Z.c2, acc[e], sum), 0.0f);
+                const float st = fmaf(Z.c2, acc[e], sum);
                 bool amb;
                 const uint32_t rk = mx_second(byte, st, sum, bnd_s, P.cE, Z.dE, amb);
                 if (byte & 0x80u) {

@@ -626,6 +626,13 @@ int build_items(int64_t nc, int L, bool slab, hipStream_t st) {
 // instruction accumulates more coarsely than the 16-bit ones: measured within
 // 5400 u (|c| + sum |p|) per instruction, taken as 2^-9 (|c| + sum |p|), i.e.
 // at most 2^-9 C (1 + eps)^2 (A + B) < 90000 C u (A + B) more in s~.
+// The float nearest above v (v itself when it is exact).
+static float round_up_f(double v) {
+    float f = (float)v;
+    if ((double)f < v) f = std::nextafter(f, std::numeric_limits<float>::infinity());
+    return f;
+}
+
 rqa::MxParams mx_params(const float *b, int nt, int m, int nbmax, int extra_u = 0, int gram = rqa::GRAM_HILO) {
     rqa::MxParams P;
     const float lo = b[0] * 0.5f;
@@ -647,8 +654,10 @@ rqa::MxParams mx_params(const float *b, int nt, int m, int nbmax, int extra_u = 
                           : gram == rqa::GRAM_BF16 ? 131336.0
                           : gram == rqa::GRAM_FP8  ? 5636100.0 + 90000.0 * C
                                                    : 0.0;
-    P.cE = (float)((3.0 * m + 31.0 + 33.0 * C + extra_u + gram_u) * std::ldexp(1.0, -24));
-    P.dA = (float)(m * std::ldexp(1.0, gram == rqa::GRAM_FP8 ? -6 : -25));
+    // Both coefficients carry a factor 1 + 2^-12 and are rounded up, which covers
+    // the rounding of E = fma(cE, n_i + n_j, dE) itself on the device.
+    P.cE = round_up_f((3.0 * m + 31.0 + 33.0 * C + extra_u + gram_u) * std::ldexp(1.0, -24) * (1.0 + std::ldexp(1.0, -12)));
+    P.dA = round_up_f(m * std::ldexp(1.0, gram == rqa::GRAM_FP8 ? -6 : -25) * (1.0 + std::ldexp(1.0, -12)));
     P.amax = gram == rqa::GRAM_FP8 ? 8 : 14;
     return P;
 }
