@@ -109,9 +109,9 @@ MEASURED_TRAFFIC = {
 _ISSUE_SRC = "profiles/ar1_m16_instruction_mix.md"
 MEASURED_VALU = {
     ("ar1_m16", "k_lam_vscan"): 7.029e9,
-    ("ar1_m16", "k_lam_hscan"): 7.348e9,
+    ("ar1_m16", "k_lam_hscan"): 7.298e9,
     ("ar1_m16", "k_det_scan"): 7.122e9,
-    ("ar1_m16", "k_lam_mx"): 7.193e9,
+    ("ar1_m16", "k_lam_mx"): 6.907e9,
     ("ar1_m16", "k_lam"): 3.921e10,
 }
 CU_COUNT, CLOCK_GHZ = 256, 2.4
