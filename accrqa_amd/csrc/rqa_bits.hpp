@@ -22,8 +22,9 @@ namespace rqa {
 
 constexpr int LCAP = 128;              // run lengths < LCAP go to the workgroup's LDS histogram
 // Run counter flag: the segment's first run has not closed yet.  Bit 30, so
-// that every run counter (at most PRE + N_c, N_c < 2^30) is positive as a
-// signed integer and an all-ones word stands apart from it.
+// that every run counter (at most PRE plus a segment of at most 2^29 steps and
+// a word) is positive as a signed integer and an all-ones word stands apart
+// from it.
 constexpr uint32_t PRE = 0x40000000u;
 // Spare bins after each row of an LDS run histogram: a lane adds one there for
 // a word whose carried run counts nothing (none closes, it is empty, or it

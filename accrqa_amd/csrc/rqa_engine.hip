@@ -444,14 +444,17 @@ void prof_collect(hipStream_t) {
 // have items of very different sizes, so they get many), in multiples of 32
 // steps, at least 256.
 // Every segment boundary costs a carry slot per (threshold, sequence) and a
-// join in k_stitch, so segments stay as long as the parallelism allows.
+// join in k_stitch, so segments stay as long as the parallelism allows.  At
+// most 2^29 steps: a run counter (the open-prefix flag, bit 30, plus at most
+// a segment and a word) then stays below 2^31, as the line scan's signed test
+// needs.
 int seg_len_for(int64_t nc) {
     const int64_t nbands = (nc + rqa::WAVE - 1) / rqa::WAVE;
     const int64_t nseg = std::max<int64_t>(1, (65536 + nbands - 1) / nbands);
     int64_t L = (nc + nseg - 1) / nseg;
     L = std::max<int64_t>(L, 256);
     L = (L + rqa::BLK - 1) / rqa::BLK * rqa::BLK;
-    return (int)std::min<int64_t>(L, 1 << 30);
+    return (int)std::min<int64_t>(L, 1 << 29);
 }
 
 // Workgroup-private long-run bins for a launch of `grid` workgroups.
@@ -1639,7 +1642,7 @@ int accrqa_amd_set_form(int form) {
 }
 
 int accrqa_amd_set_slab_rows(int rows) {
-    if (rows < 0 || (rows > 0 && (rows < 256 || rows % rqa::WAVE != 0))) return E_INVALID;
+    if (rows < 0 || (rows > 0 && (rows < 256 || rows % rqa::WAVE != 0 || rows > (1 << 29)))) return E_INVALID;
     std::lock_guard<std::mutex> lk(g_mu);
     g_slab_rows = rows;
     return E_OK;

@@ -216,8 +216,8 @@ int accrqa_amd_set_split(int mode);
 /* Slab form of the split (no counterpart in the reference).  When the tile
  * store of all rows does not fit, the split form runs slab by slab, each slab
  * a band of rows whose tiles fit (LAM and DET together, one slab's tiles at a
- * time); slabs are as tall as fit.  rows > 0 (a multiple of 64, at least 256)
- * takes the slab form whenever the split form applies, with slabs of that
+ * time); slabs are as tall as fit.  rows > 0 (a multiple of 64, at least 256,
+ * at most 2^29) takes the slab form whenever the split form applies, with slabs of that
  * many rows (for tests); 0 restores the automatic choice.  Results are
  * identical either way.  Returns 0 or ERR_INVALID_ARGUMENT. */
 int accrqa_amd_set_slab_rows(int rows);
