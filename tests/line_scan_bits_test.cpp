@@ -154,8 +154,37 @@ static void test_consume(std::mt19937_64 &g) {
     }
 }
 
+// Carried runs at the segment-length limit (2^29 steps, the open-prefix flag
+// set or not) still take the slow path when they close, and an all-recurrent
+// word only adds 32 to them.
+static void test_long_carry() {
+    for (uint32_t flag : {PRE, 0u}) {
+        for (uint32_t nw : {0x1u, 0x10u, 0x80000000u, 0u}) {
+            uint32_t run = flag + (1u << 29) - 3u;
+            const uint32_t before = run;
+            bool slow = false;
+            uint32_t got = 0;
+            consume<LCAP>(
+                nw, run, LCAP - 1u,
+                [&](uint32_t b) { CHECK(b >= LCAP - 1u, "long run counted in the LDS bins (b=%u)\n", b); },
+                [&](uint32_t) { CHECK(false, "inner run in a word with one non-recurrent step\n"); },
+                [&](uint32_t f) {
+                    slow = true;
+                    got = f;
+                });
+            if (nw == 0u) {
+                CHECK(!slow && run == before + 32u, "all-recurrent word: slow %d run %u\n", (int)slow, run);
+            } else {
+                const uint32_t lead = (uint32_t)__builtin_ctz(nw);
+                CHECK(slow && got == before + lead, "closing long run: slow %d first %u\n", (int)slow, got);
+            }
+        }
+    }
+}
+
 int main() {
     std::mt19937_64 g(12345);
+    test_long_carry();
     test_planes<4>(g);
     test_planes<8>(g);
     test_planes<16>(g);
