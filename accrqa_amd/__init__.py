@@ -126,6 +126,8 @@ def load():
     L.accrqa_amd_set_rp_wide.restype = I
     L.accrqa_amd_set_scan_waves.argtypes = [I]
     L.accrqa_amd_set_scan_waves.restype = I
+    L.accrqa_amd_set_overlap.argtypes = [I]
+    L.accrqa_amd_set_overlap.restype = I
     L.accrqa_amd_set_gram.argtypes = [I]
     L.accrqa_amd_set_gram.restype = I
     L.accrqa_amd_gram_probe.argtypes = [I, P, P, P, P, P, P]
@@ -210,6 +212,14 @@ def set_rp_wide(on):
     e = load().accrqa_amd_set_rp_wide(1 if on else 0)
     if e:
         raise AccrqaError(e, "set_rp_wide")
+
+
+def set_overlap(on):
+    """Split form: the DET tile scan beside the LAM scans on a second stream
+    (default on), or after them.  Results are identical."""
+    e = load().accrqa_amd_set_overlap(1 if on else 0)
+    if e:
+        raise AccrqaError(e, "set_overlap")
 
 
 def set_scan_waves(waves):

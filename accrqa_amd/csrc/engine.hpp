@@ -43,8 +43,12 @@ struct Ctx {
     int device = -1;
     int ncu = 0;
     hipStream_t stream = nullptr;
+    // second stream of the split form: the DET tile scan runs there beside the
+    // LAM scans, after ev_tiles (the tiles are written) and before ev_det
+    hipStream_t side = nullptr;
+    hipEvent_t ev_tiles = nullptr, ev_det = nullptr;
     DevBuf x, hist, carry, metric, scan, counts, perm, lmins, lut, nrm, nmax, lut8, out_rr, out_det, out_lam, priv,
-        tiles, dump, tops, mxstats, probe, out_rp, bhist, bdesc, hist2, carry2;
+        tiles, dump, tops, mxstats, probe, out_rp, bhist, bdesc, hist2, carry2, priv2;
     // segment length of the current (tau, emb) block, and whether its split
     // form runs slab by slab (the tile store of all rows does not fit)
     int blk_L = 0;

@@ -105,6 +105,11 @@ int g_gram = 0;
 // stores align) or one (k_rp; accrqa_amd_set_rp_wide(0)).
 bool g_rp_wide = true;
 
+// The split form's DET tile scan on the context's second stream, beside the
+// LAM column and row scans on the call's stream (1), or after them (0).  Off
+// while kernels are profiled one by one.
+int g_overlap = 1;
+
 // Resident workgroups (four waves each, so waves per SIMD) of the three tile
 // scans, k_lam_vscan, k_lam_hscan and k_det_scan: their LDS is padded with
 // dynamic LDS until no more fit per CU; 0 = as many as their registers and
@@ -234,7 +239,7 @@ int init_ctx() {
                              &g_ctx.out_rr,  &g_ctx.out_det,    &g_ctx.out_lam,    &g_ctx.priv,
                              &g_ctx.tiles,   &g_ctx.dump,       &g_ctx.tops,
                              &g_ctx.mxstats, &g_ctx.probe, &g_ctx.out_rp, &g_ctx.bhist, &g_ctx.bdesc,
-                             &g_ctx.hist2,   &g_ctx.carry2};
+                             &g_ctx.hist2,   &g_ctx.carry2,     &g_ctx.priv2};
             for (DevBuf *b : all) {
                 if (b->p) (void)hipFree(b->p);
                 *b = DevBuf();
@@ -249,13 +254,20 @@ int init_ctx() {
             if (g_order.done) (void)hipEventDestroy(g_order.done);
             g_order = CallOrder();
             if (g_ctx.stream) (void)hipStreamDestroy(g_ctx.stream);
-            g_ctx.stream = nullptr;
+            if (g_ctx.side) (void)hipStreamDestroy(g_ctx.side);
+            if (g_ctx.ev_tiles) (void)hipEventDestroy(g_ctx.ev_tiles);
+            if (g_ctx.ev_det) (void)hipEventDestroy(g_ctx.ev_det);
+            g_ctx.stream = g_ctx.side = nullptr;
+            g_ctx.ev_tiles = g_ctx.ev_det = nullptr;
             (void)hipSetDevice(cur);
         }
         hipDeviceProp_t prop;
         HIP_TRY(hipGetDeviceProperties(&prop, dev));
         g_ctx.ncu = prop.multiProcessorCount;
         HIP_TRY(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
+        HIP_TRY(hipStreamCreateWithFlags(&g_ctx.side, hipStreamNonBlocking));
+        HIP_TRY(hipEventCreateWithFlags(&g_ctx.ev_tiles, hipEventDisableTiming));
+        HIP_TRY(hipEventCreateWithFlags(&g_ctx.ev_det, hipEventDisableTiming));
         g_ctx.device = dev;
     }
     return E_OK;
@@ -715,6 +727,29 @@ int launch_gram_variant(int gram, const float *dx, int64_t nc, int tau, int m, c
 // are those of the whole-store form; DET runs slab by slab beside LAM, in its
 // own histogram and carry buffers, since the tiles are gone once the slabs
 // are done.
+bool overlap_on() { return g_overlap != 0 && !g_prof.on; }
+
+// k_det_scan over DET items [d0, d1) into hist2 / carry2 on the context's
+// second stream, after everything `st` holds so far (the tiles it scans),
+// with its own private bins; ev_det marks its end.  The LAM scans that follow
+// on `st` write only hist and carry, so the two run side by side.
+template <int TG>
+int det_scan_beside(const uint8_t *tiles, int64_t nbands, int64_t nc, int nt, int L, int nseg, int64_t d0,
+                    int64_t d1, unsigned long long *hist2, uint2 *carry2, hipStream_t st) {
+    Ctx &c = g_ctx;
+    int e;
+    const size_t pd = lds_cap_pad(rqa::k_det_scan<TG>, g_scan_waves);
+    const unsigned g = grid_for_wg(rqa::k_det_scan<TG>, d1 - d0, c.ncu, pd);
+    if ((e = ensure(c.priv2, (size_t)g * TG * rqa::PBINS * sizeof(uint32_t)))) return e;
+    HIP_TRY(hipEventRecord(c.ev_tiles, st));
+    HIP_TRY(hipStreamWaitEvent(c.side, c.ev_tiles, 0));
+    hipLaunchKernelGGL((rqa::k_det_scan<TG>), dim3(g), dim3(rqa::WG), pd, c.side, tiles, nbands, nc, nt, L, nseg,
+                       (const int2 *)c.items_dets.p + d0, d1 - d0, hist2, carry2, (uint32_t *)c.priv2.p);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(c.ev_det, c.side));
+    return E_OK;
+}
+
 template <typename F, int NORM, int TG>
 int launch_group_slabs(const F *dx, int64_t nc, int tau, int m, const rqa::Bounds<F, TG> &B, int nt,
                        const rqa::LutEntry<F> *d_lut, const rqa::LutParams &P, bool want_rr, bool want_det,
@@ -755,12 +790,26 @@ int launch_group_slabs(const F *dx, int64_t nc, int tau, int m, const rqa::Bound
                            (const unsigned int *)c.nmax.p, (uint8_t *)c.lut8.p);
         HIP_TRY(hipGetLastError());
     }
+    // the DET scan of slab s runs beside its LAM scans; slab s + 1's rank pass
+    // rewrites the tiles, so it waits for that scan
+    const bool beside = want_det && overlap_on();
+    bool det_pending = false;
     for (int s = 0; s < c.nslabs; s++) {
         const int64_t ia = (int64_t)s * L / rqa::BLK;
+        if (det_pending) {
+            HIP_TRY(hipStreamWaitEvent(st, c.ev_det, 0));
+            det_pending = false;
+        }
         // the kernels index tiles from the store's start; this slab's block ia is the buffer's first
         uint8_t *tiles = (uint8_t *)((uintptr_t)c.tiles.p - (uintptr_t)(tile_start(ia, nbands) * rqa::TILE_BYTES));
         const int64_t v0 = c.slab_lamv[s], v1 = c.slab_lamv[s + 1];
         const rqa::LamWork LW{(const int2 *)c.items_lamv.p + v0, v1 - v0, nseg, 2 * nseg};
+        const int64_t d0 = c.slab_dets[s], d1 = c.slab_dets[s + 1];
+        auto fork_det = [&]() -> int {
+            if (!beside || det_pending || d1 <= d0) return E_OK;
+            det_pending = true;
+            return det_scan_beside<TG>(tiles, nbands, nc, nt, L, nseg, d0, d1, hist2, carry2, st);
+        };
         if (LW.nitems > 0) {
             bool done = false;
             if constexpr (NORM == 1) {
@@ -773,6 +822,7 @@ int launch_group_slabs(const F *dx, int64_t nc, int tau, int m, const rqa::Bound
                                        (unsigned long long *)nullptr);
                     prof_end(st);
                     HIP_TRY(hipGetLastError());
+                    if ((e = fork_det())) return e;
                     const size_t pv = lds_cap_pad(rqa::k_lam_vscan<TG>, g_scan_waves);
                     const unsigned gv = grid_for(rqa::k_lam_vscan<TG>, LW.nitems, c.ncu, pv);
                     if ((e = ensure_priv<TG>(gv))) return e;
@@ -794,6 +844,7 @@ int launch_group_slabs(const F *dx, int64_t nc, int tau, int m, const rqa::Bound
                 HIP_TRY(hipGetLastError());
             }
         }
+        if ((e = fork_det())) return e;
         const int64_t h0 = c.slab_lamh[s], h1 = c.slab_lamh[s + 1];
         if (h1 > h0) {
             const size_t ph = lds_cap_pad(rqa::k_lam_hscan<TG>, g_scan_waves);
@@ -806,8 +857,7 @@ int launch_group_slabs(const F *dx, int64_t nc, int tau, int m, const rqa::Bound
             prof_end(st);
             HIP_TRY(hipGetLastError());
         }
-        const int64_t d0 = c.slab_dets[s], d1 = c.slab_dets[s + 1];
-        if (want_det && d1 > d0) {
+        if (want_det && d1 > d0 && !det_pending) {
             const size_t pd = lds_cap_pad(rqa::k_det_scan<TG>, g_scan_waves);
             const unsigned g = grid_for_wg(rqa::k_det_scan<TG>, d1 - d0, c.ncu, pd);
             if ((e = ensure_priv<TG>(g))) return e;
@@ -819,6 +869,7 @@ int launch_group_slabs(const F *dx, int64_t nc, int tau, int m, const rqa::Bound
             HIP_TRY(hipGetLastError());
         }
     }
+    if (det_pending) HIP_TRY(hipStreamWaitEvent(st, c.ev_det, 0));
     prof_begin("k_stitch", st);
     hipLaunchKernelGGL(rqa::k_stitch, dim3(stitch_blocks(nc), nt), dim3(rqa::STT), 0, st, (const uint2 *)carry, nt,
                        nc, nc, 0, 1, L, nseg, 2 * nseg, hist);
@@ -896,6 +947,20 @@ int launch_group(const F *dx, int64_t nc, int tau, int m, const F *bounds, int n
     }
     unsigned long long *hist = (unsigned long long *)c.hist.p;
     uint2 *carry = (uint2 *)c.carry.p;
+    // with the tile store, the DET scan runs beside the LAM scans, into hist2
+    // and carry2, once the rank pass has written the tiles
+    bool det_forked = false;
+    auto fork_det = [&]() -> int {
+        if (!tiles || !want_det || nc <= 1 || det_forked || !overlap_on()) return E_OK;
+        int e2;
+        const size_t hs2 = (size_t)nc + 1;
+        if ((e2 = ensure(c.hist2, (size_t)nt * hs2 * 8))) return e2;
+        if ((e2 = ensure(c.carry2, (size_t)nt * (size_t)(nc - 1) * nseg * sizeof(uint2)))) return e2;
+        fill_zero((unsigned long long *)c.hist2.p, (size_t)nt * hs2, st);
+        det_forked = true;
+        return det_scan_beside<TG>(tiles, nbands, nc, nt, L, nseg, 0, c.n_items_dets,
+                                   (unsigned long long *)c.hist2.p, (uint2 *)c.carry2.p, st);
+    };
 
     if (want_lam) {
         fill_zero(hist, (size_t)nt * hs, st);
@@ -938,6 +1003,7 @@ int launch_group(const F *dx, int64_t nc, int tau, int m, const F *bounds, int n
                     }
                     prof_end(st);
                     HIP_TRY(hipGetLastError());
+                    if ((e = fork_det())) return e;
                     const size_t pv = lds_cap_pad(rqa::k_lam_vscan<TG>, g_scan_waves);
                     const unsigned gv = grid_for(rqa::k_lam_vscan<TG>, LW.nitems, c.ncu, pv);
                     if ((e = ensure_priv<TG>(gv))) return e;
@@ -975,6 +1041,7 @@ int launch_group(const F *dx, int64_t nc, int tau, int m, const F *bounds, int n
             prof_end(st);
             HIP_TRY(hipGetLastError());
         }
+        if ((e = fork_det())) return e;
         if (tiles && c.n_items_lamh > 0) {
             const size_t ph = lds_cap_pad(rqa::k_lam_hscan<TG>, g_scan_waves);
             const unsigned g = grid_for(rqa::k_lam_hscan<TG>, c.n_items_lamh, c.ncu, ph);
@@ -1003,6 +1070,20 @@ int launch_group(const F *dx, int64_t nc, int tau, int m, const F *bounds, int n
         hipLaunchKernelGGL((rqa::k_rr_out<F>), dim3(1), dim3(64), 0, st, (const unsigned long long *)counts, nt, nc,
                            d_perm, d_rr);
         HIP_TRY(hipGetLastError());
+    }
+    if (want_det && det_forked) {
+        unsigned long long *hist2 = (unsigned long long *)c.hist2.p;
+        HIP_TRY(hipStreamWaitEvent(st, c.ev_det, 0));
+        prof_begin("k_stitch", st);
+        hipLaunchKernelGGL(rqa::k_stitch, dim3(stitch_blocks(nc - 1), nt), dim3(rqa::STT), 0, st,
+                           (const uint2 *)c.carry2.p, nt, nc - 1, nc, 1, 0, L, nseg, nseg, hist2);
+        prof_end(st);
+        HIP_TRY(hipGetLastError());
+        prof_begin("k_finalize", st);
+        if ((e = finalize<F>(hist2, nc, nt, 1, d_lmins, nl, d_det, d_perm, T, nullptr, st))) return e;
+        prof_end(st);
+        if ((e = dump_rows(hist2, nt, hs, d_perm, c.dump_det, st))) return e;
+        return E_OK;
     }
     if (want_det) {
         fill_zero(hist, (size_t)nt * hs, st);
@@ -1703,6 +1784,13 @@ int accrqa_amd_set_rp_wide(int on) {
     if (on < 0 || on > 1) return E_INVALID;
     std::lock_guard<std::mutex> lk(g_mu);
     g_rp_wide = on != 0;
+    return E_OK;
+}
+
+int accrqa_amd_set_overlap(int on) {
+    if (on < 0 || on > 1) return E_INVALID;
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_overlap = on;
     return E_OK;
 }
 

@@ -257,6 +257,14 @@ int accrqa_amd_set_rp_wide(int on);
  * ERR_INVALID_ARGUMENT (waves outside 0..8). */
 int accrqa_amd_set_scan_waves(int waves);
 
+/* Split form (no counterpart in the reference): 1 (the default) runs the DET
+ * scan over the rank tiles on a second stream of the library's, beside the
+ * LAM column and row scans on the call's stream, joined before the DET
+ * measures are formed; 0 runs it after them.  Kernel profiling
+ * (accrqa_amd_profile) runs them one after the other either way.  Results are
+ * identical.  Returns 0 or ERR_INVALID_ARGUMENT. */
+int accrqa_amd_set_overlap(int on);
+
 /* Gram operand precision of the split MFMA form's LAM rank pass, for the
  * measured comparison of precisions (experiment, no counterpart in the
  * reference): 0 = FP16 hi/lo (the default and the only production form),

@@ -434,6 +434,51 @@ def test_split_and_direct_forms(amd, oracle):
         amd.set_split(amd.SPLIT_AUTO)
 
 
+def test_overlap_identical(amd, oracle):
+    """The split form's DET scan beside the LAM scans on a second stream
+    (accrqa_amd_set_overlap(1), the default) against the same scans one after
+    the other: every histogram and measure is the same, and equals the
+    oracle's, for the whole tile store and slab by slab, the MFMA and exact
+    forms, both norms, fp32 and fp64, one and two threshold groups, and DET or
+    LAM asked alone; several calls in a row keep the order."""
+    rng = np.random.default_rng(5150)
+    w = np.cumsum(rng.standard_normal(3001))
+    x64 = (w - w.mean()) / w.std()
+    thr16 = np.linspace(0.1, 2.0, 16)
+    thr21 = np.linspace(0.05, 2.5, 21)
+    try:
+        for slab in (0, 512):
+            amd.set_slab_rows(slab)
+            for form in (amd.FORM_AUTO, amd.FORM_EXACT):
+                amd.set_form(form)
+                for dt in (np.float32, np.float64):
+                    x = x64.astype(dt)
+                    for dist in (1, 2):
+                        for thr in (thr16.astype(dt), thr21.astype(dt)):
+                            say("overlap", slab, form, dt.__name__, dist, thr.size)
+                            got = {}
+                            for on in (0, 1, 1):
+                                amd.set_overlap(on)
+                                got[on] = (amd.histograms(x, 1, 10, thr, dist, which=7),
+                                           amd.DET(x, [1], [10], [2, 3], thr, dist),
+                                           amd.LAM(x, [1], [10], [2, 3], thr, dist),
+                                           amd.histograms(x, 1, 10, thr, dist, which=2))
+                            a, b = got[0], got[1]
+                            for key in ("rr", "det", "lam"):
+                                assert np.array_equal(a[0][key], b[0][key]), (slab, form, dt, dist, thr.size, key)
+                            assert same_bits(a[1], b[1]) and same_bits(a[2], b[2]), (slab, form, dt, dist, thr.size)
+                            assert np.array_equal(a[3]["det"], b[3]["det"]), (slab, form, dt, dist, "DET alone")
+                            if slab == 0 and dt == np.float32 and thr.size == 16:
+                                for k, t in enumerate(thr):
+                                    assert np.array_equal(b[0]["det"][k], oracle.hist("det", x, t, 1, 10, dist)), (form, dist, t)
+                                    assert np.array_equal(b[0]["lam"][k], oracle.hist("lam", x, t, 1, 10, dist)), (form, dist, t)
+        assert amd.load().accrqa_amd_set_overlap(2) != 0 and amd.load().accrqa_amd_set_overlap(-1) != 0
+    finally:
+        amd.set_overlap(1)
+        amd.set_slab_rows(0)
+        amd.set_form(amd.FORM_AUTO)
+
+
 def test_scan_waves_identical(amd, oracle):
     """The tile scans held to 7 (the default), 4 and 1 waves per SIMD, and
     uncapped (accrqa_amd_set_scan_waves): every histogram is the same, and
