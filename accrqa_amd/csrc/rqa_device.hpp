@@ -572,13 +572,16 @@ __global__ __launch_bounds__(WG) RQA_WPE_SCAN(TG, 7, 3) void k_det_scan(const ui
                                                  int64_t nitems, unsigned long long *__restrict__ hist,
                                                  uint2 *__restrict__ carry, uint32_t *__restrict__ priv_base) {
     static_assert(DRECS <= 2 * WG, "each thread moves at most two records per block");
-    __shared__ uint32_t lh[TG * LROW];
+    // 32 thresholds: 8 spare bins per row keep the workgroup's LDS under a
+    // fifth of the CU's, so five fit (16 spare bins: four)
+    constexpr int NSP = TG >= 32 ? 8 : NSPARE;
+    __shared__ uint32_t lh[TG * lrow<NSP>()];
     __shared__ uint32_t tl[DRECS * TREC];
-    for (int e = threadIdx.x; e < TG * LROW; e += WG) lh[e] = 0;
+    for (int e = threadIdx.x; e < TG * lrow<NSP>(); e += WG) lh[e] = 0;
     priv_zero<TG>(priv_slice<TG>(priv_base), nt, WG);
     __syncthreads();
     uint32_t *priv = priv_slice<TG>(priv_base);
-    const Sink K{lh, lane_spare(), priv, hist, nc};
+    const Sink K{lh, lane_spare<NSP>(), priv, hist, nc};
 
     const int tid = threadIdx.x;
     const int lane = tid & (WAVE - 1);
@@ -654,14 +657,14 @@ __global__ __launch_bounds__(WG) RQA_WPE_SCAN(TG, 7, 3) void k_det_scan(const ui
                 const int64_t nv64 = dlen - s0;
                 const int nvalid = nv64 <= 0 ? 0 : (nv64 >= BLK ? BLK : (int)nv64);
                 const uint32_t vmask = nvalid >= BLK ? 0xFFFFFFFFu : ((1u << nvalid) - 1u);
-                scan_words<TG>(R, 0xFFFFFFFFu, ~vmask, run, nt, K, cslot, cstride);
+                scan_words<TG, NSP>(R, 0xFFFFFFFFu, ~vmask, run, nt, K, cslot, cstride);
             }
         }
         if (rv && wact) close_segment<TG>(run, nt, cslot, cstride);
     }
 
     __syncthreads();
-    flush_hist<TG>(lh, priv, nt, hist, nc, WG);
+    flush_hist<TG, NSP>(lh, priv, nt, hist, nc, WG);
 }
 
 // ---------------------------------------------------------------- k_lam_hscan
