@@ -689,13 +689,16 @@ __global__ __launch_bounds__(WG) RQA_WPE_SCAN(TG, 7, 4) void k_lam_hscan(const u
                                                   int seg_len, int nseg, const int2 *__restrict__ items,
                                                   int64_t nitems, unsigned long long *__restrict__ hist,
                                                   uint2 *__restrict__ carry, uint32_t *__restrict__ priv_base) {
-    __shared__ uint32_t lh[TG * lrow<WAVE>()];
+    // one spare bin per lane up to 16 thresholds; at 32, 16 spare bins per row
+    // keep five workgroups' LDS within the CU's
+    constexpr int NSP = TG >= 32 ? NSPARE : WAVE;
+    __shared__ uint32_t lh[TG * lrow<NSP>()];
     __shared__ __align__(16) uint32_t tl[WAVES][8 * HW];
-    for (int e = threadIdx.x; e < TG * lrow<WAVE>(); e += WG) lh[e] = 0;
+    for (int e = threadIdx.x; e < TG * lrow<NSP>(); e += WG) lh[e] = 0;
     priv_zero<TG>(priv_slice<TG>(priv_base), nt, WG);
     __syncthreads();
     uint32_t *priv = priv_slice<TG>(priv_base);
-    const Sink K{lh, lane_spare<WAVE>(), priv, hist, nc};
+    const Sink K{lh, lane_spare<NSP>(), priv, hist, nc};
 
     const int lane = threadIdx.x & (WAVE - 1);
     const int wave = wave_id();
@@ -757,7 +760,7 @@ __global__ __launch_bounds__(WG) RQA_WPE_SCAN(TG, 7, 4) void k_lam_hscan(const u
             }
             // columns past the part count as recurrent padding, taken back below
             const uint32_t fill = nb == BLK ? 0u : (0xFFFFFFFFu << nb);
-            scan_words<TG, WAVE>(R, jv ? ~fill : 0u, jv ? 0u : 0xFFFFFFFFu, run, nt, K, cslot, cstride);
+            scan_words<TG, NSP>(R, jv ? ~fill : 0u, jv ? 0u : 0xFFFFFFFFu, run, nt, K, cslot, cstride);
             if (fill && jv) {
 #pragma unroll
                 for (int t = 0; t < TG; t++) run[t] -= (uint32_t)(BLK - nb);
@@ -767,7 +770,7 @@ __global__ __launch_bounds__(WG) RQA_WPE_SCAN(TG, 7, 4) void k_lam_hscan(const u
     }
 
     __syncthreads();
-    flush_hist<TG, WAVE>(lh, priv, nt, hist, nc, WG);
+    flush_hist<TG, NSP>(lh, priv, nt, hist, nc, WG);
 }
 
 // ---------------------------------------------------------------- k_rp
