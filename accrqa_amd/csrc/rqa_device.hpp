@@ -62,8 +62,9 @@ constexpr int WAVES = WG / WAVE;
 #endif
 // The tile scans up to 16 thresholds: 7 waves per SIMD (<= 73 VGPRs), one
 // or two more than the compiler's own choice (measured on the default
-// workload: -0.8 ms for each LAM scan, -1.9 ms for k_det_scan); 32
-// thresholds keep the occupancy their registers give.
+// workload: -0.8 ms for each LAM scan, -1.9 ms for k_det_scan).  At 32
+// thresholds the column and row scans take 6 (80 VGPRs, a few spilled; 3 %
+// off the million-sample call) and k_det_scan keeps what its registers give.
 #define RQA_WPE_SCAN(tg, n, n32) __attribute__((amdgpu_waves_per_eu((tg) == 32 ? (n32) : (n))))
 constexpr int DET_WIN = 992;       // LDS samples per wave for the k_det lane window
 
@@ -685,13 +686,13 @@ __global__ __launch_bounds__(WG) RQA_WPE_SCAN(TG, 7, 3) void k_det_scan(const ui
 // lanes covers 64 consecutive words.
 constexpr int HW = 72;
 template <int TG>
-__global__ __launch_bounds__(WG) RQA_WPE_SCAN(TG, 7, 4) void k_lam_hscan(const uint8_t *__restrict__ tiles, int64_t nbands, int64_t nc, int nt,
+__global__ __launch_bounds__(WG) RQA_WPE_SCAN(TG, 7, 6) void k_lam_hscan(const uint8_t *__restrict__ tiles, int64_t nbands, int64_t nc, int nt,
                                                   int seg_len, int nseg, const int2 *__restrict__ items,
                                                   int64_t nitems, unsigned long long *__restrict__ hist,
                                                   uint2 *__restrict__ carry, uint32_t *__restrict__ priv_base) {
-    // one spare bin per lane up to 16 thresholds; at 32, 16 spare bins per row
-    // keep five workgroups' LDS within the CU's
-    constexpr int NSP = TG >= 32 ? NSPARE : WAVE;
+    // one spare bin per lane up to 16 thresholds; at 32, 8 spare bins per row
+    // keep six workgroups' LDS within the CU's
+    constexpr int NSP = TG >= 32 ? 8 : WAVE;
     __shared__ uint32_t lh[TG * lrow<NSP>()];
     __shared__ __align__(16) uint32_t tl[WAVES][8 * HW];
     for (int e = threadIdx.x; e < TG * lrow<NSP>(); e += WG) lh[e] = 0;
@@ -1782,7 +1783,7 @@ __global__ __launch_bounds__(WG) RQA_WPE_MX(false) void k_rp_mx(const float *__r
 // exchange between lanes.  The next block's record is in flight while a
 // block is scanned.
 template <int TG>
-__global__ __launch_bounds__(WG) RQA_WPE_SCAN(TG, 7, 4) void k_lam_vscan(const uint8_t *__restrict__ tiles, int64_t nbands, int64_t nc,
+__global__ __launch_bounds__(WG) RQA_WPE_SCAN(TG, 7, 6) void k_lam_vscan(const uint8_t *__restrict__ tiles, int64_t nbands, int64_t nc,
                                                   int nt, int seg_len, LamWork W,
                                                   unsigned long long *__restrict__ hist, uint2 *__restrict__ carry,
                                                   uint32_t *__restrict__ priv_base) {
