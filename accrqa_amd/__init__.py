@@ -124,6 +124,8 @@ def load():
     L.accrqa_amd_mfma16_probe.restype = I
     L.accrqa_amd_set_rp_wide.argtypes = [I]
     L.accrqa_amd_set_rp_wide.restype = I
+    L.accrqa_amd_set_rp_sym.argtypes = [I]
+    L.accrqa_amd_set_rp_sym.restype = I
     L.accrqa_amd_set_scan_waves.argtypes = [I]
     L.accrqa_amd_set_scan_waves.restype = I
     L.accrqa_amd_set_overlap.argtypes = [I]
@@ -212,6 +214,15 @@ def set_rp_wide(on):
     e = load().accrqa_amd_set_rp_wide(1 if on else 0)
     if e:
         raise AccrqaError(e, "set_rp_wide")
+
+
+def set_rp_sym(on):
+    """Recurrence plot, MFMA form: the upper triangle only, each 64 x 64
+    square stored mirrored as well, where N_c % 16 == 0 and the plot is
+    16-byte aligned (default on), or every element.  The plots are identical."""
+    e = load().accrqa_amd_set_rp_sym(1 if on else 0)
+    if e:
+        raise AccrqaError(e, "set_rp_sym")
 
 
 def set_overlap(on):

@@ -1774,6 +1774,143 @@ __global__ __launch_bounds__(WG) RQA_WPE_MX(false) void k_rp_mx(const float *__r
 }
 #endif
 
+// ---------------------------------------------------------------- k_rp_mx_sym
+// The MFMA-form recurrence plot from the upper triangle only.  The plot is
+// symmetric bit for bit -- the reference's s_ij and s_ji are the same sums,
+// since (a - b)^2 = (b - a)^2 exactly, and every stage of the MFMA form
+// decides the reference's s -- so one 64 x 64 square (rows 64g.., columns
+// 64J.., g <= J) gives out[64g.., 64J..] and its mirror out[64J.., 64g..].
+// Work item = one square, numbered J (J + 1) / 2 + g.  Each 32-row block of
+// the square is decided as in k_rp_mx; with one bound every final rank is 0
+// or 1, so four elements' decisions (rank 0 -> 1) are their rank word XOR
+// 0x01010101.  The decisions go out through two LDS tiles of the wave, so
+// that every plot store is 16 bytes of one row:
+//   D, the block (32 rows x 64 columns, row-major): each lane writes its
+//     column's bytes one at a time; stored after each block;
+//   T, the mirror (64 rows = the square's columns, x 64 bytes): a byte
+//     transpose of the lane's four rank words and one cross-half swap per
+//     two words leave each lane 16 consecutive bytes of its column, one
+//     16-byte LDS store per tile; stored after the square's two blocks,
+//     except on the diagonal (g == J), where D covers the square.
+// Rows of both tiles are padded to 80 bytes, so the 16-byte accesses of
+// eight consecutive rows fall on all 32 banks.  The host takes this kernel
+// only where N_c % 16 == 0 and the plot is 16-byte aligned: then every
+// 16-byte store lies inside one row of the plot, and one is made only when
+// its row and first column are below N_c.  Windows read what k_rp_mx reads.
+struct RpStage {
+    static constexpr int RS = 80;   // bytes per row of D and T
+    uint8_t d[BLK * RS];
+    uint8_t t[WAVE * RS];
+};
+
+#ifndef RQA_HELPERS_ONLY
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void k_rp_mx_sym(
+    const float *__restrict__ x, const float *__restrict__ nrm, const unsigned int *__restrict__ nmax_bits,
+    int64_t nc, int tau, int m, float bound, const uint8_t *__restrict__ lut, MxParams P, char *__restrict__ out) {
+    __shared__ uint8_t lut_s[NBX];
+    __shared__ float bnd_s[NBND];
+    __shared__ MxWin<2> win[WAVES];
+    __shared__ __attribute__((aligned(16))) RpStage stg[WAVES];
+    for (int e = threadIdx.x; e < P.nb / 4; e += WG) ((uint32_t *)lut_s)[e] = ((const uint32_t *)lut)[e];
+    if (threadIdx.x < NBND) bnd_s[threadIdx.x] = threadIdx.x == 0 ? bound : __int_as_float(0x7F800000);
+    __syncthreads();
+    constexpr int RS = RpStage::RS;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wave = wave_id();
+    const int col = lane & 31;
+    const int h = lane >> 5;
+    const int rspan = 32 + (m - 1) * tau;   // <= MXW (checked by the host)
+    const int cspan = 64 + (m - 1) * tau;
+    MxWin<2> &Wn = win[wave];
+    float *rw = Wn.rw, *cw = Wn.cw;
+    mx_win_init(Wn, lane);
+    const float *pa[8];
+    gram_addrs(Wn, col, h, m, tau, pa);
+    const MxScale Z = mx_scale(nmax_bits, P);
+    RpStage &S = stg[wave];
+    const int64_t nbands = (nc + WAVE - 1) / WAVE;
+    const int64_t nsq = nbands * (nbands + 1) / 2;
+    for (int64_t it = (int64_t)blockIdx.x * WAVES + wave; it < nsq; it += (int64_t)gridDim.x * WAVES) {
+        int64_t J = (int64_t)((__builtin_sqrt(8.0 * (double)it + 1.0) - 1.0) * 0.5);
+        while (J > 0 && J * (J + 1) / 2 > it) J--;
+        while ((J + 1) * (J + 2) / 2 <= it) J++;
+        const int64_t g = it - J * (J + 1) / 2;
+        const int64_t j0 = J * WAVE, r0 = g * WAVE;
+        wave_sync();
+        for (int e = lane; e < cspan; e += WAVE) cw[e] = hl_word(x[j0 + e] * Z.s);
+        for (int k = 0; k < 2; k++) {
+            const int64_t i0 = r0 + BLK * k;
+            if (i0 >= nc) break;
+            for (int e = lane; e < rspan; e += WAVE) rw[e] = hl_word(x[i0 + e] * Z.s);
+            wave_sync();
+            float ni[16];
+            tile_row_norms(nrm, i0, h, ni);
+#pragma unroll
+            for (int p = 0; p < 2; p++) {
+                const floatx16 acc = p == 0 ? gram_tile<2, 0>(Wn, pa, col, h, m, tau) : gram_tile<2, 1>(Wn, pa, col, h, m, tau);
+                uint32_t Rq[4], fm, flw;
+                tile_ranks(acc, ni, nrm[j0 + 32 * p + col], lut_s, P, Z, bnd_s, Rq, fm, flw);
+                const int64_t j = j0 + 32 * p + col;
+                // exact re-decision of the ambiguous elements by their own lane
+                uint32_t f = j < nc ? fm : 0u;
+                while (f) {
+                    const uint32_t e = __builtin_ctz(f);
+                    f &= f - 1u;
+                    const int64_t i = i0 + 8 * (e >> 2) + 4 * h + (e & 3);
+                    if (i < nc) {
+                        const float sx = exact_sqdist<float>(x, i, j, tau, m);
+                        const uint32_t rk = rank_direct<float, 1>(sx, bnd_s, 1);
+                        const uint32_t sh = 8 * (e >> 2), c = e & 3;
+                        Rq[c] = (Rq[c] & ~(0xFFu << sh)) | (rk << sh);
+                    }
+                }
+                uint32_t Dw[4];
+#pragma unroll
+                for (int c = 0; c < 4; c++) Dw[c] = Rq[c] ^ 0x01010101u;
+                // D: element e is row 8(e>>2) + 4h + (e&3), column 32p + col
+                uint8_t *dd = S.d + 4 * h * RS + 32 * p + col;
+#pragma unroll
+                for (int e = 0; e < 16; e++) dd[(8 * (e >> 2) + (e & 3)) * RS] = (uint8_t)(Dw[e & 3] >> (8 * (e >> 2)));
+                // T: byte q of Dw[c] is row 8q + 4h + c; word q of the
+                // transpose holds rows 8q + 4h .. 8q + 4h + 3
+                const uint32_t lo01 = __builtin_amdgcn_perm(Dw[1], Dw[0], 0x05010400u);
+                const uint32_t hi01 = __builtin_amdgcn_perm(Dw[1], Dw[0], 0x07030602u);
+                const uint32_t lo23 = __builtin_amdgcn_perm(Dw[3], Dw[2], 0x05010400u);
+                const uint32_t hi23 = __builtin_amdgcn_perm(Dw[3], Dw[2], 0x07030602u);
+                const uint32_t w0 = __builtin_amdgcn_perm(lo23, lo01, 0x05040100u);
+                const uint32_t w1 = __builtin_amdgcn_perm(lo23, lo01, 0x07060302u);
+                const uint32_t w2 = __builtin_amdgcn_perm(hi23, hi01, 0x05040100u);
+                const uint32_t w3 = __builtin_amdgcn_perm(hi23, hi01, 0x07060302u);
+                // half 0 then holds rows 0..15 of its column, half 1 rows 16..31
+                const auto s02 = __builtin_amdgcn_permlane32_swap(w0, w2, false, false);
+                const auto s13 = __builtin_amdgcn_permlane32_swap(w1, w3, false, false);
+                *(uint4 *)(S.t + (32 * p + col) * RS + 32 * k + 16 * h) = make_uint4(s02[0], s02[1], s13[0], s13[1]);
+            }
+            wave_sync();
+            // the block: lane -> row (lane >> 2) + 16s, bytes 16 (lane & 3) .. + 15
+#pragma unroll
+            for (int s = 0; s < 2; s++) {
+                const int row = (lane >> 2) + 16 * s, ch = lane & 3;
+                const uint4 v = *(const uint4 *)(S.d + row * RS + 16 * ch);
+                const int64_t i = i0 + row, jc = j0 + 16 * ch;
+                if (i < nc && jc < nc) *(uint4 *)(out + i * nc + jc) = v;
+            }
+            wave_sync();
+        }
+        if (g < J) {
+            // the mirror: plot row 64J + row, columns 64g + 16 (lane & 3) .. + 15
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                const int row = (lane >> 2) + 16 * s, ch = lane & 3;
+                const uint4 v = *(const uint4 *)(S.t + row * RS + 16 * ch);
+                const int64_t jr = j0 + row, ic = r0 + 16 * ch;
+                if (jr < nc && ic < nc) *(uint4 *)(out + jr * nc + ic) = v;
+            }
+        }
+    }
+}
+#endif
+
 // ---------------------------------------------------------------- k_lam_vscan
 // The column part of LAM in the split form, read back from the rank tiles
 // (the MFMA column kernel then only ranks and stores).  Work item = (band J,

@@ -105,6 +105,11 @@ int g_gram = 0;
 // stores align) or one (k_rp; accrqa_amd_set_rp_wide(0)).
 bool g_rp_wide = true;
 
+// Recurrence plot, MFMA form: the upper triangle, each 64 x 64 square
+// stored mirrored as well (k_rp_mx_sym, where N_c % 16 == 0 and the plot is
+// 16-byte aligned), or every element (k_rp_mx; accrqa_amd_set_rp_sym(0)).
+bool g_rp_sym = true;
+
 // The split form's DET tile scan on the context's second stream, beside the
 // LAM column and row scans on the call's stream (1), or after them (0).  Off
 // while kernels are profiled one by one.
@@ -1525,15 +1530,27 @@ int rp_run(const F *dx, size_t n, int tau, int emb, F thr, int dist, O *d_out, h
             HIP_TRY(hipGetLastError());
             rqa::Bounds<float, 4> Bf;
             for (int t = 0; t < 4; t++) Bf.s[t] = (float)b;
-            const rqa::MxParams MP = mx_params(Bf.s, 1, emb, rqa::NBX_LAM);
+            // the upper triangle, each square stored mirrored as well, where
+            // its 16-byte stores align
+            const bool sym = g_rp_sym && nc % 16 == 0 && (uintptr_t)d_out % 16 == 0;
+            const rqa::MxParams MP = mx_params(Bf.s, 1, emb, sym ? rqa::NBX : rqa::NBX_LAM);
             hipLaunchKernelGGL((rqa::k_mx_lut<4>), dim3((MP.nb + 255) / 256), dim3(256), 0, st, Bf, 1, MP,
                                (const unsigned int *)c.nmax.p, (uint8_t *)c.lut8.p);
             HIP_TRY(hipGetLastError());
-            const unsigned g = grid_for(rqa::k_rp_mx, nblk, c.ncu);
-            prof_begin("k_rp_mx", st);
-            hipLaunchKernelGGL(rqa::k_rp_mx, dim3(g), dim3(rqa::WG), 0, st, (const float *)dx, (const float *)c.nrm.p,
-                               (const unsigned int *)c.nmax.p, nc, tau, emb, (float)b, (const uint8_t *)c.lut8.p, MP,
-                               (char *)d_out);
+            if (sym) {
+                const int64_t nbands = (nc + rqa::WAVE - 1) / rqa::WAVE;
+                const unsigned g = grid_for(rqa::k_rp_mx_sym, nbands * (nbands + 1) / 2, c.ncu);
+                prof_begin("k_rp_mx_sym", st);
+                hipLaunchKernelGGL(rqa::k_rp_mx_sym, dim3(g), dim3(rqa::WG), 0, st, (const float *)dx,
+                                   (const float *)c.nrm.p, (const unsigned int *)c.nmax.p, nc, tau, emb, (float)b,
+                                   (const uint8_t *)c.lut8.p, MP, (char *)d_out);
+            } else {
+                const unsigned g = grid_for(rqa::k_rp_mx, nblk, c.ncu);
+                prof_begin("k_rp_mx", st);
+                hipLaunchKernelGGL(rqa::k_rp_mx, dim3(g), dim3(rqa::WG), 0, st, (const float *)dx,
+                                   (const float *)c.nrm.p, (const unsigned int *)c.nmax.p, nc, tau, emb, (float)b,
+                                   (const uint8_t *)c.lut8.p, MP, (char *)d_out);
+            }
             prof_end(st);
             HIP_TRY(hipGetLastError());
             prof_collect(st);
@@ -1784,6 +1801,13 @@ int accrqa_amd_set_rp_wide(int on) {
     if (on < 0 || on > 1) return E_INVALID;
     std::lock_guard<std::mutex> lk(g_mu);
     g_rp_wide = on != 0;
+    return E_OK;
+}
+
+int accrqa_amd_set_rp_sym(int on) {
+    if (on < 0 || on > 1) return E_INVALID;
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_rp_sym = on != 0;
     return E_OK;
 }
 

@@ -780,6 +780,61 @@ def test_rp_mfma_form(amd, oracle):
         amd.set_form(amd.FORM_AUTO)
 
 
+def test_rp_mfma_symmetric(amd, oracle):
+    """The MFMA-form recurrence plot from the upper triangle (k_rp_mx_sym:
+    64 x 64 squares, each stored as is and mirrored through LDS), taken when
+    N_c % 16 == 0 and the plot is 16-byte aligned: N_c of one square, partial
+    squares and bands (16, 48, 80, 144, 208, 1040), whole ones (64, 128, 576),
+    tau = 0 and 2, m = 33, a spike 10^6 times the rest and a NaN sample (every
+    pair re-decided exactly), against the oracle and against every element
+    decided (accrqa_amd_set_rp_sym(0)); device plots at an address 16 bytes
+    past an aligned one (the mirrored form) and one byte past (every element)."""
+    import torch
+    rng = np.random.default_rng(97)
+    amd.set_form(amd.FORM_MFMA)
+    try:
+        cases = []
+        for nc, tau, emb in ((16, 1, 16), (48, 1, 8), (64, 1, 16), (80, 2, 9), (128, 0, 12), (144, 1, 16),
+                             (208, 2, 33), (576, 1, 16), (1040, 3, 10)):
+            n = nc + (emb - 1) * tau
+            w = np.cumsum(rng.standard_normal(n)).astype(np.float32)
+            w = ((w - w.mean()) / w.std()).astype(np.float32)
+            cases.append((w, tau, emb, float(0.35 * np.sqrt(emb))))
+        spike = (1e-3 * rng.standard_normal(256 + 8)).astype(np.float32)
+        spike[77] = 1e3
+        cases.append((spike, 1, 9, 4e-3))
+        nan = rng.standard_normal(320 + 9).astype(np.float32)
+        nan[150] = np.nan
+        cases.append((nan, 1, 10, 2.5))
+        for x, tau, emb, thr in cases:
+            nc = x.size - (emb - 1) * tau
+            assert nc % 16 == 0
+            want, e = oracle.RP(x, tau, emb, np.float32(thr), 1)
+            assert e == 0
+            for sym in (1, 0):
+                amd.set_rp_sym(sym)
+                say("rp sym", nc, tau, emb, thr, sym)
+                got = amd.RP(x, tau, emb, np.float32(thr), 1)
+                assert np.array_equal(got, want), (nc, tau, emb, thr, sym)
+            amd.set_rp_sym(1)
+        # device plots 16 bytes and one byte past an aligned address
+        x = cases[5][0]
+        nc = 144
+        want, _ = oracle.RP(x, 1, 16, np.float32(cases[5][3]), 1)
+        dx = torch.from_numpy(x).cuda()
+        for off in (16, 1):
+            say("rp sym device offset", off)
+            buf = torch.full((nc * nc + 32,), 7, dtype=torch.int8, device="cuda")
+            amd.rp_device(dx.data_ptr(), x.size, 1, 16, cases[5][3], 1, buf.data_ptr() + off)
+            torch.cuda.synchronize()
+            got = buf.cpu().numpy()
+            assert (got[:off] == 7).all() and (got[off + nc * nc:] == 7).all()
+            assert np.array_equal(got[off:off + nc * nc].reshape(nc, nc), want), off
+    finally:
+        amd.set_rp_sym(1)
+        amd.set_form(amd.FORM_AUTO)
+
+
 def test_compat_module_on_gpu(amd, oracle):
     """accrqa_amd.compat (the reference's Python interface) on the GPU: its
     cubes are the oracle's flat outputs in the reference's shapes, bit for
