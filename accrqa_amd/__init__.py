@@ -217,9 +217,10 @@ def set_rp_wide(on):
 
 
 def set_rp_sym(on):
-    """Recurrence plot, MFMA form: the upper triangle only, each 64 x 64
-    square stored mirrored as well, where N_c % 16 == 0 and the plot is
-    16-byte aligned (default on), or every element.  The plots are identical."""
+    """Recurrence plot of bytes, both forms: the upper triangle only, each
+    64 x 64 square stored mirrored as well, where N_c % 16 == 0 and the plot
+    is 16-byte aligned (default on), or every element.  The plots are
+    identical."""
     e = load().accrqa_amd_set_rp_sym(1 if on else 0)
     if e:
         raise AccrqaError(e, "set_rp_sym")

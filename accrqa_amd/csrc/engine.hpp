@@ -173,6 +173,10 @@ unsigned grid_for(Kern kern, int64_t nitems, int ncu, size_t dyn = 0) {
     return (unsigned)std::max<int64_t>(1, std::min<int64_t>(wgs, (int64_t)nb * ncu));
 }
 
+// A persistent grid rounded down to a multiple of the 8 XCDs (kernels that
+// number their waves XCD by XCD).
+inline unsigned xcd_round(unsigned g) { return g >= 8 ? g / 8 * 8 : g; }
+
 // Persistent grid for a kernel whose work items belong to whole workgroups.
 template <typename Kern>
 unsigned grid_for_wg(Kern kern, int64_t nitems, int ncu, size_t dyn = 0) {

@@ -882,6 +882,145 @@ __global__ __launch_bounds__(WG) void k_rp4(const F *__restrict__ x, int64_t nc,
     }
 }
 
+// ---------------------------------------------------------------- plot squares
+// The recurrence plot is symmetric bit for bit: s_ij and s_ji are the same
+// sums, since (a - b)^2 = (b - a)^2 and |a - b| = |b - a| exactly, and every
+// form decides the reference's s.  The symmetric kernels (k_rp_sym,
+// k_rp_mx_sym) decide the 64 x 64 squares with g <= J only (rows 64g..,
+// columns 64J..) and store each one as is
+// and mirrored (out[64J.., 64g..]), both as 16-byte row stores staged in
+// the wave's LDS:
+//   D, one 32-row block of the square, row-major (32 rows x 64 bytes),
+//     stored after each block;
+//   T, the mirror, one row per column of the square (64 rows x 64 bytes),
+//     stored after the square's two blocks, except on the diagonal (g == J),
+//     where D covers the square.
+// Rows of both are padded to 80 bytes, so the 16-byte accesses of eight
+// consecutive rows fall on all 32 banks.  The host takes these kernels only
+// where N_c % 16 == 0 and the plot is 16-byte aligned: then every 16-byte
+// store lies inside one row of the plot, and one is made only when its row
+// and first column are below N_c.  Decisions of rows or columns past N_c
+// come from the zero padding of the series and are not stored.
+struct RpStage {
+    static constexpr int RS = 80;   // bytes per row of D and T
+    uint8_t d[BLK * RS];
+    uint8_t t[WAVE * RS];
+};
+
+// Work item it -> square (g, J).  Squares go in groups of 16 x 16 (rows
+// 1024G.., columns 1024H.., G <= H, group number H (H + 1) / 2 + G; square
+// (g, J) = (16G + (it % 256) / 16, 16H + it % 16)), so that the waves at work
+// at one time store their rows in runs of 1,024 bytes rather than 64
+// (measured: groups of 8 and 32 were slower).  False for the squares of a
+// group below the diagonal or past the last band.
+constexpr int RP_GROUP = 16;
+__device__ __forceinline__ bool rp_square(int64_t it, int64_t nbands, int64_t &g, int64_t &J) {
+    const int64_t grp = it / (RP_GROUP * RP_GROUP);
+    const int loc = (int)(it - grp * (RP_GROUP * RP_GROUP));
+    int64_t H = (int64_t)((__builtin_sqrt(8.0 * (double)grp + 1.0) - 1.0) * 0.5);
+    while (H > 0 && H * (H + 1) / 2 > grp) H--;
+    while ((H + 1) * (H + 2) / 2 <= grp) H++;
+    g = RP_GROUP * (grp - H * (H + 1) / 2) + loc / RP_GROUP;
+    J = RP_GROUP * H + loc % RP_GROUP;
+    return g <= J && J < nbands;
+}
+
+// The wave's first work item: its slot among the grid's waves, numbered XCD
+// by XCD.  Workgroups go to the eight XCDs in turn (workgroup b to XCD
+// b % 8), so with a grid of a multiple of 8 workgroups (the host rounds it)
+// the squares of one group are taken by waves of one XCD, whose L2 then
+// sees both halves of every 128-byte line the group stores.
+__device__ __forceinline__ int64_t rp_first_item(int wave) {
+    if (gridDim.x % 8) return (int64_t)blockIdx.x * WAVES + wave;
+    return ((int64_t)(blockIdx.x % 8) * (gridDim.x / 8) + blockIdx.x / 8) * WAVES + wave;
+}
+
+// Work items of a plot of nbands bands: whole groups of squares.
+__host__ __device__ inline int64_t rp_items(int64_t nbands) {
+    const int64_t ng = (nbands + RP_GROUP - 1) / RP_GROUP;
+    return ng * (ng + 1) / 2 * (RP_GROUP * RP_GROUP);
+}
+
+// D to out[i0 + row][j0 + 16 ch ..]: lane -> row (lane >> 2) + 16s, ch = lane & 3.
+__device__ __forceinline__ void rp_store_block(const RpStage &S, int lane, int64_t i0, int64_t j0, int64_t nc,
+                                               char *__restrict__ out) {
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+        const int row = (lane >> 2) + 16 * s, ch = lane & 3;
+        const uint4 v = *(const uint4 *)(S.d + row * RpStage::RS + 16 * ch);
+        const int64_t i = i0 + row, jc = j0 + 16 * ch;
+        if (i < nc && jc < nc) *(uint4 *)(out + i * nc + jc) = v;
+    }
+}
+
+// T to out[j0 + row][r0 + 16 ch ..]: lane -> row (lane >> 2) + 16s, ch = lane & 3.
+__device__ __forceinline__ void rp_store_mirror(const RpStage &S, int lane, int64_t j0, int64_t r0, int64_t nc,
+                                                char *__restrict__ out) {
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        const int row = (lane >> 2) + 16 * s, ch = lane & 3;
+        const uint4 v = *(const uint4 *)(S.t + row * RpStage::RS + 16 * ch);
+        const int64_t jr = j0 + row, ic = r0 + 16 * ch;
+        if (jr < nc && ic < nc) *(uint4 *)(out + jr * nc + ic) = v;
+    }
+}
+
+// ---------------------------------------------------------------- k_rp_sym
+// The symmetric recurrence plot in the exact form, bytes: per 32-row block
+// of a square, lane = column, the sums of k_rp; the lane's 32 decisions,
+// four rows per word, go to D one byte at a time and to T as two 16-byte
+// LDS stores.  Reads are k_rp's.
+template <typename F, int NORM>
+__global__ __launch_bounds__(WG) void k_rp_sym(const F *__restrict__ x, int64_t nc, int tau, int m, F bound,
+                                               char *__restrict__ out) {
+    __shared__ __attribute__((aligned(16))) RpStage stg[WAVES];
+    constexpr int RS = RpStage::RS;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wave = wave_id();
+    RpStage &S = stg[wave];
+    const int64_t nbands = (nc + WAVE - 1) / WAVE;
+    const int64_t nsq = rp_items(nbands);
+    for (int64_t it = rp_first_item(wave); it < nsq; it += (int64_t)gridDim.x * WAVES) {
+        int64_t g, J;
+        if (!rp_square(it, nbands, g, J)) continue;
+        const int64_t j0 = J * WAVE, r0 = g * WAVE;
+        const int64_t j = j0 + lane;
+        for (int k = 0; k < 2; k++) {
+            const int64_t i0 = r0 + BLK * k;
+            if (i0 >= nc) break;
+            F s[BLK];
+#pragma unroll
+            for (int u = 0; u < BLK; u++) s[u] = F(0);
+            F xj = x[j];
+            for (int kk = 0; kk < m; kk++) {
+                const F xjn = x[j + (int64_t)(kk + 1 < m ? kk + 1 : kk) * tau];
+                const F *xr = x + i0 + (int64_t)kk * tau;
+#pragma unroll
+                for (int u = 0; u < BLK; u++) s[u] = dim_step<NORM>(s[u], xr[u], xj);
+                xj = xjn;
+            }
+            // byte c of word q: row 4q + c
+            uint32_t w[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                w[q] = 0u;
+#pragma unroll
+                for (int c = 0; c < 4; c++) w[q] |= (s[4 * q + c] <= bound ? 1u : 0u) << (8 * c);
+            }
+            wave_sync();   // the last block's D and the last square's T are stored
+            uint8_t *dd = S.d + lane;
+#pragma unroll
+            for (int u = 0; u < BLK; u++) dd[u * RS] = (uint8_t)(w[u >> 2] >> (8 * (u & 3)));
+            uint4 *tt = (uint4 *)(S.t + lane * RS + 32 * k);
+            tt[0] = make_uint4(w[0], w[1], w[2], w[3]);
+            tt[1] = make_uint4(w[4], w[5], w[6], w[7]);
+            wave_sync();
+            rp_store_block(S, lane, i0, j0, nc, out);
+        }
+        if (g < J) rp_store_mirror(S, lane, j0, r0, nc, out);
+    }
+}
+
 // ---------------------------------------------------------------- k_rr
 // Work item = (band of 64 columns, row segment q) over rows i <= max column
 // of the band.  Pair weight: 2 if i < j, 1 if i == j, 0 otherwise.
@@ -1775,34 +1914,14 @@ __global__ __launch_bounds__(WG) RQA_WPE_MX(false) void k_rp_mx(const float *__r
 #endif
 
 // ---------------------------------------------------------------- k_rp_mx_sym
-// The MFMA-form recurrence plot from the upper triangle only.  The plot is
-// symmetric bit for bit -- the reference's s_ij and s_ji are the same sums,
-// since (a - b)^2 = (b - a)^2 exactly, and every stage of the MFMA form
-// decides the reference's s -- so one 64 x 64 square (rows 64g.., columns
-// 64J.., g <= J) gives out[64g.., 64J..] and its mirror out[64J.., 64g..].
-// Work item = one square, numbered J (J + 1) / 2 + g.  Each 32-row block of
-// the square is decided as in k_rp_mx; with one bound every final rank is 0
-// or 1, so four elements' decisions (rank 0 -> 1) are their rank word XOR
-// 0x01010101.  The decisions go out through two LDS tiles of the wave, so
-// that every plot store is 16 bytes of one row:
-//   D, the block (32 rows x 64 columns, row-major): each lane writes its
-//     column's bytes one at a time; stored after each block;
-//   T, the mirror (64 rows = the square's columns, x 64 bytes): a byte
-//     transpose of the lane's four rank words and one cross-half swap per
-//     two words leave each lane 16 consecutive bytes of its column, one
-//     16-byte LDS store per tile; stored after the square's two blocks,
-//     except on the diagonal (g == J), where D covers the square.
-// Rows of both tiles are padded to 80 bytes, so the 16-byte accesses of
-// eight consecutive rows fall on all 32 banks.  The host takes this kernel
-// only where N_c % 16 == 0 and the plot is 16-byte aligned: then every
-// 16-byte store lies inside one row of the plot, and one is made only when
-// its row and first column are below N_c.  Windows read what k_rp_mx reads.
-struct RpStage {
-    static constexpr int RS = 80;   // bytes per row of D and T
-    uint8_t d[BLK * RS];
-    uint8_t t[WAVE * RS];
-};
-
+// The symmetric recurrence plot (squares as in k_rp_sym) in the MFMA form.
+// Each 32-row block of a square is decided as in k_rp_mx; with one bound
+// every final rank is 0 or 1, so four elements' decisions (rank 0 -> 1) are
+// their rank word XOR 0x01010101.  D gets each lane's column one byte at a
+// time.  For T, byte q of rank word c is row 8q + 4h + c of the lane's
+// column: a byte transpose of the four words and one cross-half swap per
+// two words leave half 0 with rows 0..15 and half 1 with rows 16..31, one
+// 16-byte LDS store per tile.  Windows read what k_rp_mx reads.
 #ifndef RQA_HELPERS_ONLY
 __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void k_rp_mx_sym(
     const float *__restrict__ x, const float *__restrict__ nrm, const unsigned int *__restrict__ nmax_bits,
@@ -1829,12 +1948,10 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void k_
     const MxScale Z = mx_scale(nmax_bits, P);
     RpStage &S = stg[wave];
     const int64_t nbands = (nc + WAVE - 1) / WAVE;
-    const int64_t nsq = nbands * (nbands + 1) / 2;
-    for (int64_t it = (int64_t)blockIdx.x * WAVES + wave; it < nsq; it += (int64_t)gridDim.x * WAVES) {
-        int64_t J = (int64_t)((__builtin_sqrt(8.0 * (double)it + 1.0) - 1.0) * 0.5);
-        while (J > 0 && J * (J + 1) / 2 > it) J--;
-        while ((J + 1) * (J + 2) / 2 <= it) J++;
-        const int64_t g = it - J * (J + 1) / 2;
+    const int64_t nsq = rp_items(nbands);
+    for (int64_t it = rp_first_item(wave); it < nsq; it += (int64_t)gridDim.x * WAVES) {
+        int64_t g, J;
+        if (!rp_square(it, nbands, g, J)) continue;
         const int64_t j0 = J * WAVE, r0 = g * WAVE;
         wave_sync();
         for (int e = lane; e < cspan; e += WAVE) cw[e] = hl_word(x[j0 + e] * Z.s);
@@ -1887,26 +2004,10 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void k_
                 *(uint4 *)(S.t + (32 * p + col) * RS + 32 * k + 16 * h) = make_uint4(s02[0], s02[1], s13[0], s13[1]);
             }
             wave_sync();
-            // the block: lane -> row (lane >> 2) + 16s, bytes 16 (lane & 3) .. + 15
-#pragma unroll
-            for (int s = 0; s < 2; s++) {
-                const int row = (lane >> 2) + 16 * s, ch = lane & 3;
-                const uint4 v = *(const uint4 *)(S.d + row * RS + 16 * ch);
-                const int64_t i = i0 + row, jc = j0 + 16 * ch;
-                if (i < nc && jc < nc) *(uint4 *)(out + i * nc + jc) = v;
-            }
+            rp_store_block(S, lane, i0, j0, nc, out);
             wave_sync();
         }
-        if (g < J) {
-            // the mirror: plot row 64J + row, columns 64g + 16 (lane & 3) .. + 15
-#pragma unroll
-            for (int s = 0; s < 4; s++) {
-                const int row = (lane >> 2) + 16 * s, ch = lane & 3;
-                const uint4 v = *(const uint4 *)(S.t + row * RS + 16 * ch);
-                const int64_t jr = j0 + row, ic = r0 + 16 * ch;
-                if (jr < nc && ic < nc) *(uint4 *)(out + jr * nc + ic) = v;
-            }
-        }
+        if (g < J) rp_store_mirror(S, lane, j0, r0, nc, out);
     }
 }
 #endif

@@ -105,9 +105,10 @@ int g_gram = 0;
 // stores align) or one (k_rp; accrqa_amd_set_rp_wide(0)).
 bool g_rp_wide = true;
 
-// Recurrence plot, MFMA form: the upper triangle, each 64 x 64 square
-// stored mirrored as well (k_rp_mx_sym, where N_c % 16 == 0 and the plot is
-// 16-byte aligned), or every element (k_rp_mx; accrqa_amd_set_rp_sym(0)).
+// Recurrence plot of bytes: the upper triangle, each 64 x 64 square stored
+// mirrored as well (k_rp_mx_sym, k_rp_sym; where N_c % 16 == 0 and the plot
+// is 16-byte aligned), or every element (k_rp_mx, k_rp4, k_rp;
+// accrqa_amd_set_rp_sym(0)).
 bool g_rp_sym = true;
 
 // The split form's DET tile scan on the context's second stream, beside the
@@ -1539,7 +1540,7 @@ int rp_run(const F *dx, size_t n, int tau, int emb, F thr, int dist, O *d_out, h
             HIP_TRY(hipGetLastError());
             if (sym) {
                 const int64_t nbands = (nc + rqa::WAVE - 1) / rqa::WAVE;
-                const unsigned g = grid_for(rqa::k_rp_mx_sym, nbands * (nbands + 1) / 2, c.ncu);
+                const unsigned g = xcd_round(grid_for(rqa::k_rp_mx_sym, rqa::rp_items(nbands), c.ncu));
                 prof_begin("k_rp_mx_sym", st);
                 hipLaunchKernelGGL(rqa::k_rp_mx_sym, dim3(g), dim3(rqa::WG), 0, st, (const float *)dx,
                                    (const float *)c.nrm.p, (const unsigned int *)c.nmax.p, nc, tau, emb, (float)b,
@@ -1557,8 +1558,28 @@ int rp_run(const F *dx, size_t n, int tau, int emb, F thr, int dist, O *d_out, h
             return E_OK;
         }
     }
-    // four columns per lane when its 4-element stores align
     const uintptr_t oa = (uintptr_t)d_out;
+    if constexpr (sizeof(O) == 1) {
+        // the upper triangle, each square stored mirrored as well
+        if (g_rp_sym && nc % 16 == 0 && oa % 16 == 0) {
+            const int64_t nsq = rqa::rp_items((nc + rqa::WAVE - 1) / rqa::WAVE);
+            prof_begin("k_rp_sym", st);
+            if (dist == 1) {
+                const unsigned g = xcd_round(grid_for(rqa::k_rp_sym<F, 1>, nsq, c.ncu));
+                hipLaunchKernelGGL((rqa::k_rp_sym<F, 1>), dim3(g), dim3(rqa::WG), 0, st, dx, nc, tau, emb, b,
+                                   (char *)d_out);
+            } else {
+                const unsigned g = xcd_round(grid_for(rqa::k_rp_sym<F, 2>, nsq, c.ncu));
+                hipLaunchKernelGGL((rqa::k_rp_sym<F, 2>), dim3(g), dim3(rqa::WG), 0, st, dx, nc, tau, emb, b,
+                                   (char *)d_out);
+            }
+            prof_end(st);
+            HIP_TRY(hipGetLastError());
+            prof_collect(st);
+            return E_OK;
+        }
+    }
+    // four columns per lane when its 4-element stores align
     if (g_rp_wide && nc % 4 == 0 && oa % (sizeof(O) == 1 ? 4 : 16) == 0) {
         constexpr int R4 = rqa::rp4_rows<F>();
         const int64_t nb4 = ((nc + R4 - 1) / R4) * ((nc + 4 * rqa::WAVE - 1) / (4 * rqa::WAVE));

@@ -607,13 +607,13 @@ def config4_batched(accrqa_amd):
 def rp_bench(accrqa_amd, torch, x, thr, reps=3):
     """accrqa_RP through the device entry point, series and plot in HBM, on a
     65,536-row plot (4.3 GB of bytes) of the default workload's series, at
-    m = 16 (k_rp_mx_sym, the MFMA form on the upper triangle, each square
-    stored mirrored as well) and m = 3 (the exact form: k_rp4, four adjacent
-    columns per lane), tau = 1, Euclidean; one byte per element written,
-    priced against 8 TB/s.  At m = 16 `every_element_form_ms` is k_rp_mx
-    (every element decided, accrqa_amd_set_rp_sym(0)); at m = 3
-    `one_column_form_ms` is k_rp (one column per lane,
-    accrqa_amd_set_rp_wide(0)); both with the same plot.
+    m = 16 (k_rp_mx_sym, the MFMA form) and m = 3 (k_rp_sym, the exact form),
+    both deciding the upper triangle and storing each 64 x 64 square
+    mirrored as well, tau = 1, Euclidean; one byte per element written,
+    priced against 8 TB/s.  `every_element_form_ms` is the plot with every
+    element decided (accrqa_amd_set_rp_sym(0): k_rp_mx, or k_rp4 with four
+    adjacent columns per lane), `one_column_form_ms` at m = 3 is k_rp (one
+    column per lane, accrqa_amd_set_rp_wide(0)); all with the same plot.
     Beside it, the reference's own accrqa_RP (char output, OpenMP over rows)
     timed on a 6,000-row prefix."""
     dev = torch.device("cuda:0")
@@ -625,12 +625,12 @@ def rp_bench(accrqa_amd, torch, x, thr, reps=3):
         dx = torch.from_numpy(xs).to(dev)
         dout = torch.empty(nc * nc, dtype=torch.int8, device=dev)
         times, ones = {}, {}
-        # the form's knob: four columns per lane (exact form, m = 3), or the
-        # upper triangle stored mirrored (MFMA form, m = 16); 0 = its other form
-        knob = accrqa_amd.set_rp_wide if m == 3 else accrqa_amd.set_rp_sym
+        # (symmetric form, four columns per lane): the default first
+        forms = ((1, 1), (0, 1), (0, 0)) if m == 3 else ((1, 1), (0, 1))
         try:
-            for wide in (0, 1):
-                knob(wide)
+            for form in forms:
+                accrqa_amd.set_rp_sym(form[0])
+                accrqa_amd.set_rp_wide(form[1])
                 accrqa_amd.rp_device(dx.data_ptr(), xs.size, 1, m, thr, 1, dout.data_ptr(), st.cuda_stream)
                 torch.cuda.synchronize()
                 a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -639,17 +639,19 @@ def rp_bench(accrqa_amd, torch, x, thr, reps=3):
                     accrqa_amd.rp_device(dx.data_ptr(), xs.size, 1, m, thr, 1, dout.data_ptr(), st.cuda_stream)
                 b.record(st)
                 torch.cuda.synchronize()
-                times[wide] = a.elapsed_time(b) / reps / 1e3
-                ones[wide] = int(dout.sum(dtype=torch.int64).item())
+                times[form] = a.elapsed_time(b) / reps / 1e3
+                ones[form] = int(dout.sum(dtype=torch.int64).item())
         finally:
             accrqa_amd.set_rp_wide(1)
             accrqa_amd.set_rp_sym(1)
-        t = times[1]
+        t = times[(1, 1)]
         el = float(nc) * nc
         res[f"m{m}"] = {"n_c": nc, "ms_per_plot": t * 1e3, "elements_per_s": el / t,
-                        "write_GB_s": el / t / 1e9, "write_frac_of_8TB_s": el / t / 8e12, "ones": ones[1]}
-        other = "one_column_form_ms" if m == 3 else "every_element_form_ms"
-        res[f"m{m}"].update({other: times[0] * 1e3, "same_plot_ones": ones[0] == ones[1]})
+                        "write_GB_s": el / t / 1e9, "write_frac_of_8TB_s": el / t / 8e12, "ones": ones[(1, 1)],
+                        "every_element_form_ms": times[(0, 1)] * 1e3,
+                        "same_plot_ones": len(set(ones.values())) == 1}
+        if (0, 0) in times:
+            res[f"m{m}"]["one_column_form_ms"] = times[(0, 0)] * 1e3
         del dout, dx
         torch.cuda.empty_cache()
     sys.path.insert(0, os.path.join(ROOT, "oracle"))

@@ -249,10 +249,10 @@ int accrqa_amd_mfma16_probe(const uint16_t *a, const uint16_t *b, const float *c
  * Returns 0 or ERR_INVALID_ARGUMENT. */
 int accrqa_amd_set_rp_wide(int on);
 
-/* Recurrence plot, MFMA form (tuning aid, no counterpart in the reference):
- * 1 (the default) decides the upper triangle only and stores each 64 x 64
- * square mirrored as well, where N_c % 16 == 0 and the plot is 16-byte
- * aligned (every store is then 16 bytes of one row); 0 decides every
+/* Recurrence plot of bytes, both forms (tuning aid, no counterpart in the
+ * reference): 1 (the default) decides the upper triangle only and stores
+ * each 64 x 64 square mirrored as well, where N_c % 16 == 0 and the plot is
+ * 16-byte aligned (every store is then 16 bytes of one row); 0 decides every
  * element.  The plots are identical.  Returns 0 or ERR_INVALID_ARGUMENT. */
 int accrqa_amd_set_rp_sym(int on);
 

@@ -713,10 +713,12 @@ def test_rp_wide_form(amd, oracle):
     at and past the block sizes (partial last blocks), against the oracle and
     against one column per lane (accrqa_amd_set_rp_wide(0)), both norms,
     fp32 and fp64, char and int, tau = 0; a plot at an odd device address
-    takes the one-column form."""
+    takes the one-column form.  The symmetric form is off here
+    (accrqa_amd_set_rp_sym(0)); test_rp_exact_symmetric covers it."""
     import torch
     rng = np.random.default_rng(83)
     amd.set_form(amd.FORM_EXACT)
+    amd.set_rp_sym(0)
     try:
         for nc, tau, emb in ((4, 1, 2), (32, 1, 1), (100, 2, 2), (256, 1, 4), (260, 0, 3), (1028, 1, 5),
                              (1040, 3, 9)):
@@ -747,6 +749,54 @@ def test_rp_wide_form(amd, oracle):
         got = buf.cpu().numpy()
         assert got[0] == 7 and np.array_equal(got[1:].reshape(96, 96), want)
     finally:
+        amd.set_rp_wide(1)
+        amd.set_rp_sym(1)
+        amd.set_form(amd.FORM_AUTO)
+
+
+def test_rp_exact_symmetric(amd, oracle):
+    """The exact-form recurrence plot of bytes from the upper triangle
+    (k_rp_sym: 64 x 64 squares, each stored as is and mirrored through LDS),
+    taken when N_c % 16 == 0 and the plot is 16-byte aligned: N_c of one
+    square, partial squares and bands, whole ones, tau = 0, both norms, fp32
+    and fp64, against the oracle and against the four-column and one-column
+    forms (accrqa_amd_set_rp_sym(0)); a device plot 16 bytes past an aligned
+    address takes it, one byte past does not."""
+    import torch
+    rng = np.random.default_rng(101)
+    amd.set_form(amd.FORM_EXACT)
+    try:
+        for nc, tau, emb in ((16, 1, 3), (48, 2, 2), (64, 1, 5), (128, 0, 3), (208, 1, 4), (576, 3, 9),
+                             (1040, 1, 3)):
+            n = nc + (emb - 1) * tau
+            w = np.cumsum(rng.standard_normal(n))
+            w = (w - w.mean()) / w.std()
+            for dt in (np.float32, np.float64):
+                x = w.astype(dt)
+                for dist in (1, 2):
+                    thr = dt(0.3 * np.sqrt(emb) if dist == 1 else 0.3)
+                    want, e = oracle.RP(x, tau, emb, thr, dist)
+                    assert e == 0
+                    for sym, wide in ((1, 1), (0, 1), (0, 0)):
+                        amd.set_rp_sym(sym)
+                        amd.set_rp_wide(wide)
+                        say("rp exact sym", nc, tau, emb, dt.__name__, dist, sym, wide)
+                        assert np.array_equal(amd.RP(x, tau, emb, thr, dist), want), (nc, dt, dist, sym, wide)
+                    amd.set_rp_sym(1)
+                    amd.set_rp_wide(1)
+        x = np.cumsum(rng.standard_normal(144 + 2)).astype(np.float32)
+        want, _ = oracle.RP(x, 1, 3, np.float32(0.9), 1)
+        dx = torch.from_numpy(x).cuda()
+        for off in (16, 1):
+            say("rp exact sym device offset", off)
+            buf = torch.full((144 * 144 + 32,), 7, dtype=torch.int8, device="cuda")
+            amd.rp_device(dx.data_ptr(), x.size, 1, 3, 0.9, 1, buf.data_ptr() + off)
+            torch.cuda.synchronize()
+            got = buf.cpu().numpy()
+            assert (got[:off] == 7).all() and (got[off + 144 * 144:] == 7).all()
+            assert np.array_equal(got[off:off + 144 * 144].reshape(144, 144), want), off
+    finally:
+        amd.set_rp_sym(1)
         amd.set_rp_wide(1)
         amd.set_form(amd.FORM_AUTO)
 
