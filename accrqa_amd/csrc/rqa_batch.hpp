@@ -306,8 +306,14 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(WPE))) void 
                     for (int k = k0; k < k1; k++) {
                         const F *xr = xs + s0 + (int64_t)k * tau;
                         const F *wl = Wv.win + lane + (k - k0) * tau;
+                        // all 32 window reads in flight before the arithmetic
+                        // (left to itself the compiler waits on every pair)
+                        F wv[BLK];
 #pragma unroll
-                        for (int u = 0; u < BLK; u++) s[u] = dim_step<NORM>(s[u], xr[u], wl[u]);
+                        for (int u = 0; u < BLK; u++) wv[u] = wl[u];
+                        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                        for (int u = 0; u < BLK; u++) s[u] = dim_step<NORM>(s[u], xr[u], wv[u]);
                     }
                 }
                 uint32_t R[8];
