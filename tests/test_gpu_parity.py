@@ -146,6 +146,8 @@ def test_special_values(amd, oracle):
     for dt in (np.float32, np.float64):
         x = rng.standard_normal(700).astype(dt)
         x[[5, 300]] = np.nan
+        x[600] = np.copysign(np.nan, -1.0)   # a NaN with the sign bit set
+        assert np.signbit(x[600])
         x[[100, 101, 102]] = np.inf
         x[400] = -np.inf
         x[500:520] = 0.0
