@@ -191,7 +191,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     BatchWave<F, TG> &Wv = wv[wave];
     for (int e = lane; e < TG * BLROW; e += WAVE) Wv.lh[e] = 0u;
     const int64_t nitems = ipre[np];
-    constexpr int MJ = sizeof(F) == 4 ? 16 : 8;   // column samples kept in registers (LAM)
+    constexpr int MJ = 8;   // column samples kept in registers (LAM); dimensions past them load per block
 
     int cur = -1, cur_kind = -1;   // (problem, kind) whose runs the LDS histogram holds
     uint32_t *cur_row = nullptr;

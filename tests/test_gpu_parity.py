@@ -679,11 +679,12 @@ def test_rqa_batch_matches_single_series(amd, oracle):
     (ENTR, summed in another order, within 1e-6 / 1e-12 relative); series of
     different lengths (one shorter than a 64-column band, one spanning several
     segments of the large-N path), thresholds per series (one never recurs),
-    several (tau, emb), both norms, fp32 and fp64, every measure subset; one
-    series also against the oracle directly."""
+    several (tau, emb) -- emb = 11 past the eight column samples the batched
+    kernel keeps in registers --, both norms, fp32 and fp64, every measure
+    subset; one series also against the oracle directly."""
     rng = np.random.default_rng(73)
     lens = [50, 300, 777, 1500, 2100, 64, 5000]
-    taus, embs, lm = [1, 3], [2, 5], [2, 3]
+    taus, embs, lm = [1, 3], [2, 5, 11], [2, 3]
     for dt in (np.float32, np.float64):
         xs = [np.cumsum(rng.standard_normal(n)).astype(dt) for n in lens]
         thr = np.array([[0.1 * s.std(), 0.3 * s.std(), -1.0, 0.7 * s.std()] for s in xs], dtype=dt)
