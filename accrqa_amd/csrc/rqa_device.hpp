@@ -1189,6 +1189,7 @@ __global__ __launch_bounds__(WG) void k_fp64_diag(const float *__restrict__ x, i
 // is then decided exactly.
 constexpr int NBX = 16384;       // byte buckets of the MFMA rank table (LDS), DET pass
 constexpr int NBX_LAM = 32768;   // the same for the LAM pass (which has the LDS to spare)
+constexpr int NBX_RP = 4096;     // the same for the symmetric plot (four workgroups per CU)
 constexpr int MXW = 128;     // LDS row window: 32 + (m-1)*tau samples at most
 constexpr int NBND = 64;     // bound slots reachable by a 6-bit table index
 
@@ -1923,10 +1924,10 @@ __global__ __launch_bounds__(WG) RQA_WPE_MX(false) void k_rp_mx(const float *__r
 // two words leave half 0 with rows 0..15 and half 1 with rows 16..31, one
 // 16-byte LDS store per tile.  Windows read what k_rp_mx reads.
 #ifndef RQA_HELPERS_ONLY
-__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void k_rp_mx_sym(
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void k_rp_mx_sym(
     const float *__restrict__ x, const float *__restrict__ nrm, const unsigned int *__restrict__ nmax_bits,
     int64_t nc, int tau, int m, float bound, const uint8_t *__restrict__ lut, MxParams P, char *__restrict__ out) {
-    __shared__ uint8_t lut_s[NBX];
+    __shared__ uint8_t lut_s[NBX_RP];
     __shared__ float bnd_s[NBND];
     __shared__ MxWin<2> win[WAVES];
     __shared__ __attribute__((aligned(16))) RpStage stg[WAVES];

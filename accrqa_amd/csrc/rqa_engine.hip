@@ -1534,7 +1534,7 @@ int rp_run(const F *dx, size_t n, int tau, int emb, F thr, int dist, O *d_out, h
             // the upper triangle, each square stored mirrored as well, where
             // its 16-byte stores align
             const bool sym = g_rp_sym && nc % 16 == 0 && (uintptr_t)d_out % 16 == 0;
-            const rqa::MxParams MP = mx_params(Bf.s, 1, emb, sym ? rqa::NBX : rqa::NBX_LAM);
+            const rqa::MxParams MP = mx_params(Bf.s, 1, emb, sym ? rqa::NBX_RP : rqa::NBX_LAM);
             hipLaunchKernelGGL((rqa::k_mx_lut<4>), dim3((MP.nb + 255) / 256), dim3(256), 0, st, Bf, 1, MP,
                                (const unsigned int *)c.nmax.p, (uint8_t *)c.lut8.p);
             HIP_TRY(hipGetLastError());
