@@ -1299,10 +1299,12 @@ __global__ void k_mx_lut(Bounds<float, TG> S, int nt, MxParams P, const unsigned
             near++;
         }
     }
+    // several bounds within reach: index NBND - 1, the NaN slot, so that the
+    // second stage finds every such pair ambiguous without a test of its own
     uint32_t e = (uint32_t)r_lo;
     if (near == 1) e = 0x80u | (uint32_t)idx;
-    if (near >= 2) e = 0xC0u;
-    if (!(emax < __int_as_float(0x7F800000))) e = 0xC0u;   // non-finite or huge norms: all exact
+    if (near >= 2) e = 0xC0u | (uint32_t)(NBND - 1);
+    if (!(emax < __int_as_float(0x7F800000))) e = 0xC0u | (uint32_t)(NBND - 1);   // non-finite or huge norms: all exact
     lut[b] = (uint8_t)e;
 }
 
@@ -1320,17 +1322,26 @@ __device__ __forceinline__ uint32_t mx_key_bucket(float st, const MxParams &P) {
 
 // Second stage of one flagged element (table byte with bit 7 set): the one
 // bound within reach decides unless it lies within E_ij of s~, or several
-// bounds were within reach; amb is set for those pairs.  cE and dE carry the
-// margin for the rounding of E (mx_params).  st need not be clamped at 0: a
-// negative st decides "below the bound" only when st + E_ij is, and the true
-// s lies within E_ij of st.
+// bounds were within reach; amb is set for those pairs.  A byte for several
+// bounds names bound slot NBND - 1, which holds NaN: d is then NaN and the
+// pair ambiguous (its rank byte, NBND - 1 or NBND, is replaced by the exact
+// re-decision; padding elements keep it and are masked out).  cE and dE carry
+// the margin for the rounding of E (mx_params).  st need not be clamped at 0:
+// a negative st decides "below the bound" only when st + E_ij is, and the
+// true s lies within E_ij of st.
 __device__ __forceinline__ uint32_t mx_second(uint32_t byte, float st, float sum, const float *bnd, float cE,
                                               float dE, bool &amb) {
     const uint32_t idx = byte & 0x3Fu;
     const float d = st - bnd[idx];
     const float E = fmaf(cE, sum, dE);
-    amb = (byte & 0x40u) != 0u || !(fabsf(d) > E);
+    amb = !(fabsf(d) > E);
     return idx + (d > 0.0f ? 1u : 0u);
+}
+
+// Bound slot t of the MFMA kernels' LDS copy: the group's bounds, +inf past
+// them, NaN in the last slot (mx_second).
+__device__ __forceinline__ float mx_bound_slot(int t, int nt, const float *s) {
+    return t == NBND - 1 ? __int_as_float(0x7FC00000) : (t < nt ? s[t] : __int_as_float(0x7F800000));
 }
 
 // Exact-order fp32 squared distance of one pair (the reference's sum).
@@ -1682,7 +1693,7 @@ __global__ __launch_bounds__(WG) RQA_WPE_MX(SCAN) void k_lam_mx(const F *__restr
     if constexpr (SCAN)
         for (int e = threadIdx.x; e < TG * LROW; e += WG) lh[e] = 0;
     for (int e = threadIdx.x; e < P.nb / 4; e += WG) ((uint32_t *)lut_s)[e] = ((const uint32_t *)lut)[e];
-    if (threadIdx.x < NBND) bnd_s[threadIdx.x] = (int)threadIdx.x < nt ? S.s[threadIdx.x] : __int_as_float(0x7F800000);
+    if (threadIdx.x < NBND) bnd_s[threadIdx.x] = mx_bound_slot((int)threadIdx.x, nt, S.s);
     if constexpr (sizeof(F) == 8)
         if (threadIdx.x < NBND) bndx_s[threadIdx.x] = (int)threadIdx.x < nt ? SX.s[threadIdx.x] : (F)__builtin_inf();
     const F *bndx;
@@ -1857,7 +1868,7 @@ __global__ __launch_bounds__(WG) RQA_WPE_MX(false) void k_rp_mx(const float *__r
     __shared__ float bnd_s[NBND];
     __shared__ MxWin<2> win[WAVES];
     for (int e = threadIdx.x; e < P.nb / 4; e += WG) ((uint32_t *)lut_s)[e] = ((const uint32_t *)lut)[e];
-    if (threadIdx.x < NBND) bnd_s[threadIdx.x] = threadIdx.x == 0 ? bound : __int_as_float(0x7F800000);
+    if (threadIdx.x < NBND) bnd_s[threadIdx.x] = mx_bound_slot((int)threadIdx.x, 1, &bound);
     __syncthreads();
     const int lane = threadIdx.x & (WAVE - 1);
     const int wave = wave_id();
@@ -1932,7 +1943,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void k_
     __shared__ MxWin<2> win[WAVES];
     __shared__ __attribute__((aligned(16))) RpStage stg[WAVES];
     for (int e = threadIdx.x; e < P.nb / 4; e += WG) ((uint32_t *)lut_s)[e] = ((const uint32_t *)lut)[e];
-    if (threadIdx.x < NBND) bnd_s[threadIdx.x] = threadIdx.x == 0 ? bound : __int_as_float(0x7F800000);
+    if (threadIdx.x < NBND) bnd_s[threadIdx.x] = mx_bound_slot((int)threadIdx.x, 1, &bound);
     __syncthreads();
     constexpr int RS = RpStage::RS;
     const int lane = threadIdx.x & (WAVE - 1);
@@ -2103,7 +2114,7 @@ __global__ __launch_bounds__(WG) RQA_WPE_MX(true) void k_det_mx(const float *__r
     __shared__ MxWin<3> win[WAVES];
     for (int e = threadIdx.x; e < TG * LROW; e += WG) lh[e] = 0;
     for (int e = threadIdx.x; e < P.nb / 4; e += WG) ((uint32_t *)lut_s)[e] = ((const uint32_t *)lut)[e];
-    if (threadIdx.x < NBND) bnd_s[threadIdx.x] = (int)threadIdx.x < nt ? S.s[threadIdx.x] : __int_as_float(0x7F800000);
+    if (threadIdx.x < NBND) bnd_s[threadIdx.x] = mx_bound_slot((int)threadIdx.x, nt, S.s);
     priv_zero<TG>(priv_slice<TG>(priv_base), nt, WG);
     __syncthreads();
     uint32_t *priv = priv_slice<TG>(priv_base);
