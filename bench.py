@@ -117,7 +117,7 @@ MEASURED_VALU = {
     ("ar1_m16", "k_lam_vscan"): 7.029e9,
     ("ar1_m16", "k_lam_hscan"): 7.298e9,
     ("ar1_m16", "k_det_scan"): 7.122e9,
-    ("ar1_m16", "k_lam_mx"): 6.907e9,
+    ("ar1_m16", "k_lam_mx"): 6.634e9,
     ("ar1_m16", "k_lam"): 3.921e10,
 }
 CU_COUNT, CLOCK_GHZ = 256, 2.4
