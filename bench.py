@@ -272,6 +272,9 @@ def main():
                     help="skip the untimed extras: decision counters, m = 32 / 64 and fp64-input steps")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first (calibrates HBM byte counters)")
+    ap.add_argument("--rp-only", action="store_true",
+                    help="time only the recurrence plot (the rp extra) and print it as one JSON line "
+                         "(for counter passes over the plot kernels)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -304,6 +307,9 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    if args.rp_only:
+        print(json.dumps({"rp": rp_bench(accrqa_amd, torch, x, float(c["thr"][5]))}))
+        return
 
     def step():
         accrqa_amd.rqa_device(dx.data_ptr(), x.size, tau, emb, thr, lmin, dist, d_rr.data_ptr(), d_det.data_ptr(),
